@@ -1,0 +1,521 @@
+/*
+ * lstc_oracle.c -- CPU restatement of the LSTC triangle-counting path.
+ *
+ * TEST INFRASTRUCTURE ONLY: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load this library, and only as the checker.
+ * The product never links or calls it.
+ *
+ * Reference files restated (paths relative to the reference root):
+ *   Med_dataset/without_part_preprocess.py:193-208      cleaning (DGL calls)
+ *   Med_dataset/without_part_preprocess.py:19-111,256   orientation K9/K10/K11
+ *   preprocess_CPU/Undir2Dir.c:107-133                  orientation (2nd witness)
+ *   src/WithoutPartition/ShareKernelTCV62_update_v1.cu:12-191   count semantics
+ *   src/WithPartition/without_sorting_tc.cu:244-451     owned-row count, sum over parts
+ *   preprocess_GPU/Halo_cleanned_preprocess/round_clean_subgraph.cu:69-272,357-703
+ *                                                       halo subgraph
+ *   preprocess_GPU/generate_clean_undir_dataset/Halo_creation_preprocess.py:362
+ *                                                       random k-way assignment
+ */
+#include "lstc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* RNG: SplitMix64 (Steele, Lea, Flood 2014) used as a counter-based hash.   */
+/* ------------------------------------------------------------------------ */
+#define GOLDEN 0x9E3779B97F4A7C15ull
+
+uint64_t lo_mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+uint64_t lo_stream_key(uint64_t seed, uint64_t stream)
+{
+    return lo_mix64(lo_mix64(seed + GOLDEN) ^ (stream * 0xD1B54A32D192ED03ull));
+}
+
+uint64_t lo_rng64(uint64_t key, uint64_t ctr)
+{
+    return lo_mix64(key + (ctr + 1) * GOLDEN);
+}
+
+static inline uint32_t mulhi32(uint32_t r, uint32_t n)
+{
+    return (uint32_t)(((uint64_t)r * (uint64_t)n) >> 32);
+}
+
+static inline uint32_t prob_to_u32(double p)
+{
+    double x = p * 4294967296.0;
+    if (x <= 0.0) return 0;
+    if (x >= 4294967295.0) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Generators                                                                */
+/* ------------------------------------------------------------------------ */
+
+/* RMAT (Chakrabarti et al. 2004): one 32-bit uniform per level, two levels per
+ * 64-bit draw; quadrant order a:(0,0) b:(0,1) c:(1,0) d:(1,1); level 0 sets the
+ * most significant bit.  Stream 1.                                           */
+void lo_gen_rmat(uint32_t scale, uint64_t m_raw, double a, double b, double c,
+                 uint64_t seed, uint32_t *src, uint32_t *dst)
+{
+    const uint64_t key = lo_stream_key(seed, 1);
+    const uint32_t tA = prob_to_u32(a), tAB = prob_to_u32(a + b), tABC = prob_to_u32(a + b + c);
+    const uint32_t nh = (scale + 1) / 2;
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)m_raw; ++i) {
+        uint32_t s = 0, d = 0;
+        uint64_t r = 0;
+        for (uint32_t l = 0; l < scale; ++l) {
+            if ((l & 1u) == 0) r = lo_rng64(key, (uint64_t)i * nh + (l >> 1));
+            uint32_t u = (l & 1u) ? (uint32_t)(r >> 32) : (uint32_t)r;
+            uint32_t bit = 1u << (scale - 1 - l);
+            if (u < tA) {
+            } else if (u < tAB) {
+                d |= bit;
+            } else if (u < tABC) {
+                s |= bit;
+            } else {
+                s |= bit;
+                d |= bit;
+            }
+        }
+        src[i] = s;
+        dst[i] = d;
+    }
+}
+
+/* Uniform pairs (G(n, M) sampling; cleaning removes loops and repeats).  Stream 2. */
+void lo_gen_er(uint32_t n, uint64_t m_raw, uint64_t seed, uint32_t *src, uint32_t *dst)
+{
+    const uint64_t key = lo_stream_key(seed, 2);
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)m_raw; ++i) {
+        uint64_t r = lo_rng64(key, (uint64_t)i);
+        src[i] = mulhi32((uint32_t)r, n);
+        dst[i] = mulhi32((uint32_t)(r >> 32), n);
+    }
+}
+
+/* Chung-Lu (Chung & Lu 2002): expected weights w_i = w_max (1 + i/i0)^(-1/(gamma-1)),
+ * i0 chosen so the continuous approximation of the mean equals mean_deg.
+ * Integer weights W_i = floor(w_i * 2^16 + 0.5); endpoints drawn by inverse
+ * CDF over the inclusive prefix of W.  Stream 3, one 64-bit draw per endpoint. */
+static double cl_mean(double i0, double beta, double n, double w_max)
+{
+    return w_max * i0 / (1.0 - beta) * (pow(1.0 + n / i0, 1.0 - beta) - 1.0) / n;
+}
+
+int lo_gen_chung_lu(uint32_t n, double gamma, double mean_deg, double w_max, uint64_t m_raw,
+                    uint64_t seed, uint32_t *src, uint32_t *dst)
+{
+    if (n == 0 || !(gamma > 1.0) || fabs(gamma - 2.0) < 1e-9) return -1;
+    const double beta = 1.0 / (gamma - 1.0);
+    double lo = 1e-9, hi = (double)n;
+    for (int it = 0; it < 200; ++it) {
+        double mid = 0.5 * (lo + hi);
+        if (cl_mean(mid, beta, (double)n, w_max) < mean_deg) lo = mid; else hi = mid;
+    }
+    const double i0 = 0.5 * (lo + hi);
+    uint64_t *cw = (uint64_t *)malloc(sizeof(uint64_t) * n);
+    if (!cw) return -2;
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        double w = w_max * pow(1.0 + (double)i / i0, -beta);
+        acc += (uint64_t)(w * 65536.0 + 0.5);
+        cw[i] = acc;
+    }
+    const uint64_t total = acc;
+    const uint64_t key = lo_stream_key(seed, 3);
+    #pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < (int64_t)m_raw; ++e) {
+        for (int side = 0; side < 2; ++side) {
+            uint64_t r = lo_rng64(key, 2 * (uint64_t)e + (uint64_t)side);
+            uint64_t x = (uint64_t)(((unsigned __int128)r * total) >> 64);
+            /* first i with cw[i] > x */
+            uint32_t l = 0, h = n;
+            while (l < h) {
+                uint32_t mid = l + (h - l) / 2;
+                if (cw[mid] > x) h = mid; else l = mid + 1;
+            }
+            if (side == 0) src[e] = l; else dst[e] = l;
+        }
+    }
+    free(cw);
+    return 0;
+}
+
+static inline uint64_t prob_to_u64(double p)
+{
+    if (p <= 0.0) return 0;
+    if (p >= 1.0) return UINT64_MAX;
+    return (uint64_t)(p * 18446744073709551616.0);
+}
+
+/* Grid: vertex v = r*cols + c has slots 3v (right), 3v+1 (down), 3v+2 (diagonal
+ * to (r+1, c+1), present with probability p_diag via stream 4 counter r*(cols-1)+c).
+ * A slot that does not exist or is absent is emitted as the loop (v, v).      */
+void lo_gen_grid(uint32_t rows, uint32_t cols, double p_diag, uint64_t seed,
+                 uint32_t *src, uint32_t *dst)
+{
+    const uint64_t key = lo_stream_key(seed, 4);
+    const uint64_t thr = prob_to_u64(p_diag);
+    const uint64_t nv = (uint64_t)rows * cols;
+    #pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < (int64_t)nv; ++v) {
+        uint32_t r = (uint32_t)(v / cols), c = (uint32_t)(v % cols);
+        uint32_t vv = (uint32_t)v;
+        src[3 * v + 0] = vv; dst[3 * v + 0] = (c + 1 < cols) ? vv + 1 : vv;
+        src[3 * v + 1] = vv; dst[3 * v + 1] = (r + 1 < rows) ? vv + cols : vv;
+        uint32_t dd = vv;
+        if (r + 1 < rows && c + 1 < cols) {
+            uint64_t h = lo_rng64(key, (uint64_t)r * (cols - 1) + c);
+            if (thr == UINT64_MAX || h < thr) dd = vv + cols + 1;
+        }
+        src[3 * v + 2] = vv; dst[3 * v + 2] = dd;
+    }
+}
+
+uint64_t lo_grid_num_diagonals(uint32_t rows, uint32_t cols, double p_diag, uint64_t seed)
+{
+    const uint64_t key = lo_stream_key(seed, 4);
+    const uint64_t thr = prob_to_u64(p_diag);
+    uint64_t d = 0;
+    if (rows < 2 || cols < 2) return 0;
+    for (uint64_t cell = 0; cell < (uint64_t)(rows - 1) * (cols - 1); ++cell)
+        if (thr == UINT64_MAX || lo_rng64(key, cell) < thr) ++d;
+    return d;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LSD radix sort of u64 keys (16-bit digits), used by lo_clean.             */
+/* ------------------------------------------------------------------------ */
+static void radix_sort_u64(uint64_t *a, uint64_t *tmp, uint64_t n, int key_bits)
+{
+    uint64_t *in = a, *out = tmp;
+    size_t *cnt = (size_t *)malloc(sizeof(size_t) * 65536);
+    for (int shift = 0; shift < key_bits; shift += 16) {
+        memset(cnt, 0, sizeof(size_t) * 65536);
+        for (uint64_t i = 0; i < n; ++i) cnt[(in[i] >> shift) & 0xFFFF]++;
+        size_t s = 0;
+        for (int d = 0; d < 65536; ++d) { size_t c = cnt[d]; cnt[d] = s; s += c; }
+        for (uint64_t i = 0; i < n; ++i) out[cnt[(in[i] >> shift) & 0xFFFF]++] = in[i];
+        uint64_t *t = in; in = out; out = t;
+    }
+    if (in != a) memcpy(a, in, sizeof(uint64_t) * n);
+    free(cnt);
+}
+
+static int bits_for(uint64_t x) /* bits needed to represent values < x */
+{
+    int b = 0;
+    while (b < 64 && (x - 1) >> b) ++b;
+    return b == 0 ? 1 : b;
+}
+
+/* Restates dgl.remove_self_loop + dgl.to_bidirected + isolated-node removal
+ * (Med_dataset/without_part_preprocess.py:193,201,207-208): the result is the
+ * simple undirected graph as a symmetric CSR with ascending rows; with
+ * compact != 0, vertices without edges are removed and the rest renumbered in
+ * increasing id order (DGL's remove_nodes keeps the relative order).        */
+void lo_clean(uint32_t n, uint64_t m_raw, const uint32_t *src, const uint32_t *dst,
+              int compact, uint32_t *n_out, uint64_t *m_out,
+              uint64_t *sym_off, uint32_t *sym_col)
+{
+    uint64_t *keys = (uint64_t *)malloc(sizeof(uint64_t) * (2 * m_raw + 1));
+    uint64_t *tmp = (uint64_t *)malloc(sizeof(uint64_t) * (2 * m_raw + 1));
+    uint64_t nk = 0;
+    for (uint64_t i = 0; i < m_raw; ++i) {
+        uint32_t s = src[i], d = dst[i];
+        if (s == d) continue;
+        keys[nk++] = ((uint64_t)s << 32) | d;
+        keys[nk++] = ((uint64_t)d << 32) | s;
+    }
+    radix_sort_u64(keys, tmp, nk, 32 + bits_for(n));
+    uint64_t nu = 0;
+    for (uint64_t i = 0; i < nk; ++i)
+        if (i == 0 || keys[i] != keys[i - 1]) keys[nu++] = keys[i];
+    /* new ids */
+    uint32_t *deg = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
+    for (uint64_t i = 0; i < nu; ++i) deg[keys[i] >> 32]++;
+    uint32_t *nid = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t nn = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        if (!compact || deg[v] > 0) nid[v] = nn++;
+        else nid[v] = UINT32_MAX;
+    }
+    memset(sym_off, 0, sizeof(uint64_t) * ((size_t)nn + 1));
+    for (uint32_t v = 0; v < n; ++v)
+        if (nid[v] != UINT32_MAX) sym_off[nid[v] + 1] = deg[v];
+    for (uint32_t i = 0; i < nn; ++i) sym_off[i + 1] += sym_off[i];
+    for (uint64_t i = 0; i < nu; ++i) sym_col[i] = nid[(uint32_t)keys[i]];
+    *n_out = nn;
+    *m_out = nu / 2;
+    free(nid);
+    free(deg);
+    free(tmp);
+    free(keys);
+}
+
+/* Restates Find_size -> cp.cumsum -> Convert (Med_dataset/without_part_preprocess.py
+ * :19-60, :256, :62-111) with the same predicate as preprocess_CPU/Undir2Dir.c:117-129:
+ * arc s->d is kept iff deg[s] < deg[d], or deg[s] == deg[d] and d < s.
+ * Rows keep the input (ascending) order.                                     */
+void lo_orient(uint32_t n, const uint64_t *sym_off, const uint32_t *sym_col,
+               uint64_t *off, uint32_t *col)
+{
+    off[0] = 0;
+    for (uint32_t s = 0; s < n; ++s) {
+        uint64_t ds = sym_off[s + 1] - sym_off[s], cnt = 0;
+        for (uint64_t j = sym_off[s]; j < sym_off[s + 1]; ++j) {
+            uint32_t d = sym_col[j];
+            uint64_t dd = sym_off[d + 1] - sym_off[d];
+            if (ds < dd || (ds == dd && d < s)) ++cnt;
+        }
+        off[s + 1] = off[s] + cnt;
+    }
+    #pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t s = 0; s < (int64_t)n; ++s) {
+        uint64_t ds = sym_off[s + 1] - sym_off[s], pos = off[s];
+        for (uint64_t j = sym_off[s]; j < sym_off[s + 1]; ++j) {
+            uint32_t d = sym_col[j];
+            uint64_t dd = sym_off[d + 1] - sym_off[d];
+            if (ds < dd || (ds == dd && d < (uint32_t)s)) col[pos++] = d;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Counting                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* CPU merge-intersection counter: for every arc (u, w) with u owned, count
+ * |N+(u) n N+(w)| by a two-pointer merge of the sorted rows.  Sum over rows is
+ * the value the reference obtains by cub::DeviceReduce::Sum over g_sum
+ * (src/WithoutPartition/ShareKernelTCV62_update_v1.cu:271-277;
+ *  src/WithPartition/without_sorting_tc.cu:400,422-428,451).               */
+uint64_t lo_count_merge(uint32_t n_rows, uint32_t n_owned, const uint64_t *off,
+                        const uint32_t *col, uint64_t *per_row, int threads)
+{
+    uint64_t total = 0;
+    (void)n_rows;
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads();
+#else
+    threads = 1;
+#endif
+    #pragma omp parallel for schedule(dynamic, 256) reduction(+ : total) num_threads(threads)
+    for (int64_t u = 0; u < (int64_t)n_owned; ++u) {
+        const uint64_t b0 = off[u], e0 = off[u + 1];
+        uint64_t row = 0;
+        for (uint64_t j = b0; j < e0; ++j) {
+            const uint32_t w = col[j];
+            uint64_t p = b0, q = off[w];
+            const uint64_t pe = e0, qe = off[w + 1];
+            while (p < pe && q < qe) {
+                uint32_t x = col[p], y = col[q];
+                if (x < y) ++p;
+                else if (y < x) ++q;
+                else { ++row; ++p; ++q; }
+            }
+        }
+        if (per_row) per_row[u] = row;
+        total += row;
+    }
+    return total;
+}
+
+/* The same merge counter over an explicit list of rows (a bounded sample). */
+uint64_t lo_count_merge_list(const uint64_t *off, const uint32_t *col, const uint32_t *rows,
+                             uint64_t nrows, int threads)
+{
+    uint64_t total = 0;
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads();
+#else
+    threads = 1;
+#endif
+    #pragma omp parallel for schedule(dynamic, 64) reduction(+ : total) num_threads(threads)
+    for (int64_t i = 0; i < (int64_t)nrows; ++i) {
+        const uint32_t u = rows[i];
+        const uint64_t b0 = off[u], e0 = off[u + 1];
+        for (uint64_t j = b0; j < e0; ++j) {
+            const uint32_t w = col[j];
+            uint64_t p = b0, q = off[w];
+            const uint64_t qe = off[w + 1];
+            while (p < e0 && q < qe) {
+                uint32_t x = col[p], y = col[q];
+                if (x < y) ++p;
+                else if (y < x) ++q;
+                else { ++total; ++p; ++q; }
+            }
+        }
+    }
+    return total;
+}
+
+/* Search (src/WithoutPartition/ShareKernelTCV62_update_v1.cu:12-42): is key in
+ * neb[0..last] (inclusive)?  Endpoints first, then bisection of (0, last).    */
+static int k1_search(uint32_t key, const uint32_t *neb, int64_t last)
+{
+    if (key < neb[0] || key > neb[last]) return 0;
+    if (key == neb[0] || key == neb[last]) return 1;
+    int64_t lo = 1, hi = last - 1;
+    while (lo <= hi) {
+        int64_t mid = (hi + lo) / 2;
+        if (neb[mid] < key) lo = mid + 1;
+        else if (neb[mid] > key) hi = mid - 1;
+        else return 1;
+    }
+    return 0;
+}
+
+/* Find_Triangle (src/WithoutPartition/ShareKernelTCV62_update_v1.cu:43-191) as a
+ * sequential loop: rows with size_list1 = d+(u)-1 < 1 give 0 (:64-67); else
+ * N+(u) is staged in tiles of 128 (:71-77, :100-115, :143-147) and every x of
+ * every N+(w), w in N+(u), is searched in each tile (:78-98, :116-136, :149-169).
+ * Defects D1-D3 of the reference are not reproduced (zero-initialised
+ * accumulators, integer trip counts).                                       */
+uint64_t lo_count_rows_k1(uint32_t n_rows, uint32_t n_owned, const uint64_t *off,
+                          const uint32_t *col, uint64_t *g_sum)
+{
+    const int64_t TILE = 128;
+    uint64_t total = 0;
+    (void)n_rows;
+    for (uint32_t u = 0; u < n_owned; ++u) {
+        const int64_t start = (int64_t)off[u], end = (int64_t)off[u + 1] - 1;
+        const int64_t size_list1 = end - start;
+        uint64_t tri = 0;
+        if (size_list1 >= 1) {
+            for (int64_t t0 = 0; t0 <= size_list1; t0 += TILE) {
+                int64_t last = size_list1 - t0;
+                if (last > TILE - 1) last = TILE - 1;
+                const uint32_t *neb = col + start + t0;
+                for (int64_t j = start; j <= end; ++j) {
+                    const uint32_t w = col[j];
+                    for (uint64_t q = off[w]; q < off[w + 1]; ++q)
+                        tri += (uint64_t)k1_search(col[q], neb, last);
+                }
+            }
+        }
+        if (g_sum) g_sum[u] = tri;
+        total += tri;
+    }
+    return total;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Partition and halo                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Seeded uniform k-way assignment replacing the unseeded np.random.randint
+ * (preprocess_GPU/generate_clean_undir_dataset/Halo_creation_preprocess.py:288,325,362).
+ * part[v] = floor(k * U32(v) / 2^32), U32(v) = high half of rng64(stream 5, v). */
+void lo_partition_random(uint32_t n, uint32_t k, uint64_t seed, uint32_t *part)
+{
+    const uint64_t key = lo_stream_key(seed, 5);
+    for (uint32_t v = 0; v < n; ++v)
+        part[v] = mulhi32((uint32_t)(lo_rng64(key, v) >> 32), k);
+}
+
+/* create_subgraph_with_halo (preprocess_GPU/Halo_cleanned_preprocess/
+ * round_clean_subgraph.cu:357-703) for one part p:
+ *   P = ascending {v : part[v] == p}                         (:393-397)
+ *   H = ascending unique N(P) \ P                            (:424, :468-495)
+ *   local ids: P block then H block                          (:527-530, :600-602)
+ *   arcs: global-degree orientation restricted to P u H      (:160-272)
+ *   rows ascending by local id                               (:323-346, :656)
+ * mode 1 replaces H by N+(P) \ P, which keeps every triangle whose lowest-rank
+ * vertex is owned (the intent of the pruning loop :671-838).  The global degree
+ * is the symmetric row length (Halo_creation_preprocess.py:244).            */
+void lo_halo_extract(uint32_t n, const uint64_t *sym_off, const uint32_t *sym_col,
+                     const uint32_t *part, uint32_t p, int mode,
+                     uint32_t *n_local, uint32_t *n_owned, uint64_t *m_local,
+                     uint64_t *off_out, uint32_t *col_out, uint32_t *l2g)
+{
+    uint32_t *loc = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    unsigned char *inH = (unsigned char *)calloc(n ? n : 1, 1);
+    uint32_t np = 0, nh = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        if (part[v] != p) continue;
+        const uint64_t dv = sym_off[v + 1] - sym_off[v];
+        for (uint64_t j = sym_off[v]; j < sym_off[v + 1]; ++j) {
+            const uint32_t d = sym_col[j];
+            if (part[d] == p) continue;
+            if (mode == 1) {
+                const uint64_t dd = sym_off[d + 1] - sym_off[d];
+                if (!(dv < dd || (dv == dd && d < v))) continue;
+            }
+            inH[d] = 1;
+        }
+    }
+    for (uint32_t v = 0; v < n; ++v) if (part[v] == p) loc[v] = np++;
+    for (uint32_t v = 0; v < n; ++v) {
+        if (part[v] == p) continue;
+        loc[v] = inH[v] ? np + nh++ : UINT32_MAX;
+    }
+    const uint32_t nl = np + nh;
+    /* local -> global */
+    uint32_t *g = (uint32_t *)malloc(sizeof(uint32_t) * (nl ? nl : 1));
+    for (uint32_t v = 0; v < n; ++v) if (loc[v] != UINT32_MAX) g[loc[v]] = v;
+    uint64_t m = 0;
+    if (off_out) off_out[0] = 0;
+    for (uint32_t i = 0; i < nl; ++i) {
+        const uint32_t s = g[i];
+        const uint64_t ds = sym_off[s + 1] - sym_off[s];
+        uint64_t cnt = 0;
+        /* owned targets first (their local ids are all below np), then halo */
+        for (int pass = 0; pass < 2; ++pass) {
+            for (uint64_t j = sym_off[s]; j < sym_off[s + 1]; ++j) {
+                const uint32_t d = sym_col[j];
+                const uint32_t ld = loc[d];
+                if (ld == UINT32_MAX) continue;
+                if ((pass == 0) != (ld < np)) continue;
+                const uint64_t dd = sym_off[d + 1] - sym_off[d];
+                if (!(ds < dd || (ds == dd && d < s))) continue;
+                if (col_out) col_out[m + cnt] = ld;
+                ++cnt;
+            }
+        }
+        m += cnt;
+        if (off_out) off_out[i + 1] = m;
+    }
+    if (l2g && off_out) memcpy(l2g, g, sizeof(uint32_t) * nl);
+    *n_local = nl;
+    *n_owned = np;
+    *m_local = m;
+    free(g);
+    free(inH);
+    free(loc);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Statistics                                                                */
+/* ------------------------------------------------------------------------ */
+void lo_stats(uint32_t n, const uint64_t *off, const uint32_t *col, lo_stats_t *st)
+{
+    uint32_t *indeg = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
+    memset(st, 0, sizeof(*st));
+    st->n = n;
+    st->m = off[n];
+    for (uint64_t j = 0; j < off[n]; ++j) indeg[col[j]]++;
+    for (uint32_t u = 0; u < n; ++u) {
+        uint64_t d = off[u + 1] - off[u];
+        if (d > st->max_out) st->max_out = d;
+        st->sum_out_sq += d * d;
+        st->sum_in_out += (uint64_t)indeg[u] * d;
+        if (d >= 2) st->rows_ge2++;
+    }
+    free(indeg);
+}

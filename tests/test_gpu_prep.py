@@ -1,0 +1,94 @@
+"""GPU parity of the generators and of clean + orient (lstc_clean_orient).
+
+Bit-exact: raw samples, symmetric CSR and oriented CSR must equal the oracle's.
+"""
+import numpy as np
+import pytest
+import torch
+
+import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(t, dtype):
+    return t.cpu().numpy().view(dtype)
+
+
+@pytest.mark.parametrize("kind", ["rmat", "er", "chung_lu", "grid"])
+def test_generators_bit_exact(lstc, oracle, kind):
+    if kind == "rmat":
+        n, s, d = lstc.generate_rmat(13, 16, seed=7)
+        n2, s2, d2 = oracle.gen_rmat(13, 16, seed=7)
+    elif kind == "er":
+        n, s, d = lstc.generate_er(12345, 20, seed=3)
+        n2, s2, d2 = oracle.gen_er(12345, 20, seed=3)
+    elif kind == "chung_lu":
+        n, s, d = lstc.generate_chung_lu(30000, 2.1, 50, 5e3, seed=5)
+        n2, s2, d2 = oracle.gen_chung_lu(30000, 2.1, 50, 5e3, seed=5)
+    else:
+        n, s, d = lstc.generate_grid(37, 53, 0.3, seed=2)
+        n2, s2, d2 = oracle.gen_grid(37, 53, 0.3, seed=2)
+    assert n == n2
+    assert np.array_equal(_np(s, np.uint32), s2)
+    assert np.array_equal(_np(d, np.uint32), d2)
+
+
+def test_partition_bit_exact(lstc, oracle):
+    p = lstc.partition_random(100003, 8, seed=11)
+    assert np.array_equal(_np(p, np.uint32), oracle.partition_random(100003, 8, seed=11))
+
+
+def _check_clean_orient(lstc, oracle, n, s, d, compact=True):
+    s = np.ascontiguousarray(s, np.uint32)
+    d = np.ascontiguousarray(d, np.uint32)
+    ds = torch.from_numpy(s.view(np.int32)).cuda() if len(s) else torch.zeros(0, dtype=torch.int32).cuda()
+    dd = torch.from_numpy(d.view(np.int32)).cuda() if len(d) else torch.zeros(0, dtype=torch.int32).cuda()
+    g = lstc.clean_orient(n, ds, dd, compact=compact, keep_symmetric=True)
+    sym, og = oracle.clean_orient(n, s, d, compact)
+    assert g.n == og.n and g.m == og.m
+    off, col = g.to_numpy()
+    assert np.array_equal(off, og.off)
+    assert np.array_equal(col, og.col)
+    soff, scol = g.sym_to_numpy()
+    assert np.array_equal(soff, sym.off)
+    assert np.array_equal(scol, sym.col)
+    return g, og
+
+
+@pytest.mark.parametrize("name,builder", [
+    ("toy_test_cu", lambda O: G.toy_test_cu()),
+    ("toy_edge_filter_cu", lambda O: G.toy_edge_filter_cu()),
+    ("multigraph", lambda O: G.random_multigraph(3000, 20000, seed=5)),
+    ("K200", lambda O: G.complete(200)),
+    ("grid", lambda O: O.gen_grid(40, 40, 0.3, 1)),
+    ("rmat_s12", lambda O: O.gen_rmat(12, 16, seed=3)),
+    ("er", lambda O: O.gen_er(70000, 8, seed=4)),
+])
+@pytest.mark.parametrize("compact", [True, False])
+def test_clean_orient_bit_exact(lstc, oracle, name, builder, compact):
+    _check_clean_orient(lstc, oracle, *builder(oracle), compact=compact)
+
+
+def test_clean_orient_degenerate(lstc, oracle):
+    _check_clean_orient(lstc, oracle, 7, np.array([], np.uint32), np.array([], np.uint32))
+    _check_clean_orient(lstc, oracle, 7, np.array([], np.uint32), np.array([], np.uint32), compact=False)
+    _check_clean_orient(lstc, oracle, 9, np.array([1, 4, 4], np.uint32), np.array([1, 4, 4], np.uint32))
+    _check_clean_orient(lstc, oracle, 9, np.array([8, 3, 8, 3, 3], np.uint32), np.array([3, 8, 3, 8, 8], np.uint32))
+
+
+def test_clean_orient_rejects_out_of_range(lstc):
+    s = torch.tensor([0, 5], dtype=torch.int32).cuda()
+    d = torch.tensor([1, 2], dtype=torch.int32).cuda()
+    with pytest.raises(lstc.LstcError):
+        lstc.clean_orient(4, s, d)
+
+
+def test_device_pipeline_rmat_s16(lstc, oracle):
+    """Whole device path at the CPU-runnable config: generate -> clean/orient -> count."""
+    n, s, d = lstc.generate_rmat(16, 16, seed=1)
+    g = lstc.clean_orient(n, s, d)
+    sym, og = oracle.clean_orient(*oracle.gen_rmat(16, 16, seed=1))
+    off, col = g.to_numpy()
+    assert np.array_equal(off, og.off) and np.array_equal(col, og.col)
+    assert lstc.count(g) == oracle.count(og)
