@@ -86,6 +86,77 @@ def cpu_baseline(g, target_s=15.0):
                       f"{edges} oriented edges) counted by the oracle merge counter in {dt:.1f} s"}
 
 
+def partitioned(T, g, k, seed, steps, ws):
+    """RMAT halo parts (k-way seeded random assignment) counted in one persistent launch."""
+    import torch
+    part = T.partition_random(g.n, k, seed=seed)
+    t0 = time.perf_counter()
+    parts = [T.halo_extract(g, part, p, mode=0, ws=ws) for p in range(k)]
+    torch.cuda.synchronize()
+    t_halo = time.perf_counter() - t0
+    plan = T.BatchPlan(parts)
+    plan.launch_graph()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        plan.launch_graph()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    per = plan.results()
+    plan.close()
+    return {"k": k, "mode": "P u N(P) (reference un-pruned halo)", "ms": round(ms, 3),
+            "gteps": round(g.m / (ms / 1e3) / 1e9, 4), "sum_parts": sum(per),
+            "halo_extract_s": round(t_halo, 3), "part_rows": [p.n for p in parts],
+            "part_owned": [p.owned for p in parts], "part_arcs": [p.m for p in parts]}
+
+
+def mixed_workload(T, seed, ws):
+    """Config: 2000x2000 grid + p=0.3 diagonals, Chung-Lu (1M, 2.1, 50, 5e4), 64 ER (n log-uniform)."""
+    import numpy as np
+    import torch
+    graphs, names = [], []
+    n, s, d = T.generate_grid(2000, 2000, 0.3, seed=seed)
+    graphs.append(T.clean_orient(n, s, d, ws=ws)); names.append("grid2000")
+    del s, d
+    n, s, d = T.generate_chung_lu(1_000_000, 2.1, 50.0, 5e4, seed=seed, ws=ws)
+    graphs.append(T.clean_orient(n, s, d, ws=ws)); names.append("chung_lu_1M")
+    del s, d
+    rng = np.random.default_rng(seed)
+    for i in range(64):
+        nn = int(round(10 ** rng.uniform(3, 5)))
+        n, s, d = T.generate_er(nn, 20.0, seed=seed * 1000 + i)
+        graphs.append(T.clean_orient(n, s, d, ws=ws)); names.append(f"er{i}_{nn}")
+    torch.cuda.synchronize()
+    return graphs, names
+
+
+def batched_vs_per_graph(T, graphs, reps, ws):
+    import torch
+    # per-graph plain path: one launch + synchronise + copy-back per graph
+    for g in graphs[:3]:
+        T.count(g, ws=ws)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        per_graph = [T.count(g, ws=ws) for g in graphs]
+    t_plain = (time.perf_counter() - t0) / reps
+    plan = T.BatchPlan(graphs)
+    plan.launch_graph()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.launch_graph()
+        batched = plan.results()
+    t_batch = (time.perf_counter() - t0) / reps
+    plan.close()
+    return {"graphs": len(graphs), "edges": int(sum(g.m for g in graphs)),
+            "per_graph_ms": round(t_plain * 1e3, 3), "batched_ms": round(t_batch * 1e3, 3),
+            "speedup": round(t_plain / t_batch, 3), "identical": per_graph == batched,
+            "triangles": int(sum(batched))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +165,7 @@ def main():
     ap.add_argument("--scale", type=int, default=23)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the partitioned and mixed workloads")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -126,6 +198,19 @@ def main():
     achieved = ba / (ms / 1e3) / 1e9
     log(f"count {tri} in {ms:.3f} ms/step -> {gteps:.3f} GTEPS; B_alg {ba/1e9:.1f} GB -> {achieved:.0f} GB/s")
 
+    extra = {}
+    if not args.no_extra:
+        extra["partitioned"] = partitioned(T, g, 8, args.seed, max(1, args.steps // 2), ws)
+        log(f"partitioned k=8: {extra['partitioned']['ms']} ms -> {extra['partitioned']['gteps']} GTEPS, "
+            f"sum of parts {extra['partitioned']['sum_parts']} (unpartitioned {tri})")
+        if extra["partitioned"]["sum_parts"] != tri:
+            raise SystemExit("partitioned sum differs from the unpartitioned count")
+        mg, _names = mixed_workload(T, args.seed, ws)
+        extra["mixed"] = batched_vs_per_graph(T, mg, 3, ws)
+        log(f"mixed workload: {extra['mixed']}")
+        if not extra["mixed"]["identical"]:
+            raise SystemExit("batched counts differ from per-graph counts")
+        del mg
     cpu = None if args.no_cpu else cpu_baseline(g)
     if cpu:
         log(f"cpu baseline {cpu['value']} GTEPS on {cpu['cores']} threads")
@@ -142,6 +227,9 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": cpu,
     }
+    if extra:
+        res["partitioned_s23_k8"] = extra["partitioned"]
+        res["batched_vs_per_graph"] = extra["mixed"]
     print(json.dumps(res))
 
 

@@ -81,15 +81,37 @@ lstc_status lstc_clean_orient(const uint32_t *d_src, const uint32_t *d_dst, uint
                               uint64_t *d_off, uint32_t *d_col, uint32_t *n_out, uint64_t *m_out,
                               void *d_ws, size_t ws_bytes, lstc_stream s);
 
+/* ------------------------------------------------------------------ (b) halo subgraphs
+ * Replaces create_subgraph_with_halo (preprocess_GPU/Halo_cleanned_preprocess/
+ * round_clean_subgraph.cu:357-902) for part p of a vertex assignment d_part[n]:
+ * the induced subgraph on P u H, P = {v : part[v] == p} ascending, H = N(P)\P
+ * (mode 0, the reference's un-pruned output) or N+(P)\P (mode 1, the
+ * count-preserving pruning), local ids P block then H block, arcs oriented by the
+ * GLOBAL (degree, id) rule, rows ascending.  Input: the symmetric CSR from
+ * lstc_clean_orient.  Owned rows are the first n_owned = |P| rows, as in the
+ * reference's file format (:850).  Both calls synchronise the stream.          */
+lstc_status lstc_halo_workspace(uint32_t n, size_t *bytes);
+lstc_status lstc_halo_sizes(const uint64_t *d_sym_off, const uint32_t *d_sym_col, uint32_t n,
+                            const uint32_t *d_part, uint32_t p, int mode, uint32_t *n_local,
+                            uint32_t *n_owned, uint64_t *m_local, void *d_ws, size_t ws_bytes,
+                            lstc_stream s);
+/* d_off_out[n_local+1], d_col_out[m_local], d_l2g_out[n_local] (local -> global, may be NULL). */
+lstc_status lstc_halo_extract(const uint64_t *d_sym_off, const uint32_t *d_sym_col, uint32_t n,
+                              const uint32_t *d_part, uint32_t p, int mode, uint64_t *d_off_out,
+                              uint32_t *d_col_out, uint32_t *d_l2g_out, uint32_t n_local_cap,
+                              uint64_t m_local_cap, void *d_ws, size_t ws_bytes, lstc_stream s);
+
 /* ------------------------------------------------------------------ (c) counting
  * Tuning knobs (picked by lstc_autotune; zero-initialised = defaults).       */
 typedef struct lstc_tune {
-    uint32_t wg_waves;      /* waves per workgroup (1..16), 0 = 4                */
-    uint32_t table_scale;   /* LDS hash capacity = table_scale * keys (2..8), 0 = 4 */
-    uint32_t switch_ratio;  /* per-arc side switch: iterate N+(u) against a hashed
-                               N+(w) when d+(u)*switch_ratio/16 < d+(w); 0 = off  */
-    uint32_t split_segments;/* max neighbour segments per work item (multiple of 64), 0 = 256 */
-    uint32_t reserved[4];
+    uint32_t table_scale;  /* LDS hash capacity >= table_scale * keys (2..8), 0 = 4     */
+    uint32_t small_max;    /* switch point: rows with d+ <= small_max get a wave-private
+                              1024-slot table, larger rows a workgroup-shared 4096-slot
+                              one (64..1024/table_scale), 0 = 256                       */
+    uint32_t small_group;  /* small rows per work ticket (4..256), 0 = 16              */
+    uint32_t grid_per_cu;  /* persistent workgroups per CU (1..16), 0 = 7              */
+    uint32_t unroll;       /* 64-entry chunks in flight per wave (4 or 8), 0 = 8       */
+    uint32_t reserved[3];
 } lstc_tune;
 
 /* Exact count over owned rows [0, n_owned) of an oriented CSR: the sum over
@@ -106,6 +128,87 @@ lstc_status lstc_count_workspace(uint32_t n_rows, uint32_t n_owned, uint64_t m,
 lstc_status lstc_count(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows,
                        uint32_t n_owned, uint64_t m, uint64_t *d_count, uint64_t *d_per_row,
                        const lstc_tune *tune, void *d_ws, size_t ws_bytes, lstc_stream s);
+
+/* ------------------------------------------------------------------ batching
+ * Many graphs (e.g. the k halo parts of a partition, or unrelated graphs of
+ * uneven size) counted by ONE persistent launch over one degree-binned ticket
+ * queue; d_counts[G] receives each graph's owned-row count.  Replaces the
+ * reference's per-part loop of launch + synchronise + reduce + copy
+ * (src/WithPartition/without_sorting_tc.cu:244-465).                          */
+typedef struct lstc_graph_desc {
+    const uint64_t *off;  /* device, n_rows + 1 */
+    const uint32_t *col;  /* device, m          */
+    uint64_t *per_row;    /* device, n_owned, or NULL */
+    uint64_t m;
+    uint32_t n_rows;
+    uint32_t n_owned;
+} lstc_graph_desc;
+
+lstc_status lstc_count_batch_workspace(const lstc_graph_desc *graphs, uint32_t G, const lstc_tune *tune,
+                                       size_t *bytes);
+/* Reads the host array `graphs` asynchronously, so it synchronises the stream. */
+lstc_status lstc_count_batch(const lstc_graph_desc *graphs, uint32_t G, uint64_t *d_counts,
+                             const lstc_tune *tune, void *d_ws, size_t ws_bytes, lstc_stream s);
+
+/* A plan owns device copies of the descriptors and its workspace (allocated at
+ * creation), so running it is pure kernel enqueues; launch_graph captures that
+ * sequence once into a hipGraph (per d_counts) and replays it. */
+typedef struct lstc_batch_plan lstc_batch_plan;
+lstc_status lstc_batch_plan_create(const lstc_graph_desc *graphs, uint32_t G, const lstc_tune *tune,
+                                   lstc_batch_plan **out);
+lstc_status lstc_batch_plan_run(lstc_batch_plan *plan, uint64_t *d_counts, lstc_stream s);
+lstc_status lstc_batch_plan_launch_graph(lstc_batch_plan *plan, uint64_t *d_counts, lstc_stream s);
+lstc_status lstc_batch_plan_destroy(lstc_batch_plan *plan);
+
+/* ------------------------------------------------------------------ serving
+ * No reference counterpart (the reference runs one process per graph).  A server
+ * owns `hbm_budget_bytes` of HBM (allocated once at creation): a pool for the
+ * requests' working sets and resident CSRs plus a reserve for batch work queues.
+ * Requests are admitted FIFO as memory frees up; admitted requests are
+ * generated and cleaned/oriented on a prep stream while the previous batch is
+ * counted on a count stream; every prepared request joins the next persistent
+ * batched launch (continuous batching).  A request whose working set exceeds
+ * the budget is refused at submission with LSTC_ERR_OVER_BUDGET.
+ * Latency = submission to result available on the host.                     */
+#define LSTC_REQ_RMAT 0u /* generate RMAT(scale, m_raw, a, b, c, seed), clean, orient */
+#define LSTC_REQ_ER 1u   /* generate n vertices, m_raw uniform pairs, seed, clean, orient */
+#define LSTC_REQ_CSR 2u  /* count a caller-resident oriented CSR (off, col, n_rows, m) */
+
+typedef struct lstc_request {
+    uint32_t kind;
+    uint32_t scale;      /* RMAT: n = 2^scale                        */
+    uint32_t n;          /* ER: vertices                             */
+    uint32_t n_rows;     /* CSR: rows (all owned)                    */
+    uint64_t m_raw;      /* RMAT / ER: raw edge samples              */
+    uint64_t seed;
+    double a, b, c;      /* RMAT quadrant probabilities              */
+    const uint64_t *off; /* CSR: device offsets, valid until the result is taken */
+    const uint32_t *col; /* CSR: device ids                          */
+    uint64_t m;          /* CSR: arcs                                */
+} lstc_request;
+
+typedef struct lstc_result {
+    uint64_t count;    /* exact triangle count                        */
+    uint64_t edges;    /* undirected edges of the counted graph        */
+    double latency_s;  /* submission -> result on the host             */
+} lstc_result;
+
+typedef struct lstc_server_stats {
+    uint64_t submitted, completed, rejected, batches, max_batch_graphs;
+    uint64_t pool_bytes, pool_used_bytes, edges, triangles;
+    double p50_s, p99_s, mean_s, max_s;
+} lstc_server_stats;
+
+typedef struct lstc_server lstc_server;
+lstc_status lstc_server_create(size_t hbm_budget_bytes, uint32_t max_batch, const lstc_tune *tune,
+                               lstc_server **out);
+lstc_status lstc_server_submit(lstc_server *srv, const lstc_request *req, uint64_t *id);
+/* LSTC_ERR_NOT_READY until the result exists; a taken result is removed. */
+lstc_status lstc_server_poll(lstc_server *srv, uint64_t id, lstc_result *out);
+lstc_status lstc_server_wait(lstc_server *srv, uint64_t id, double timeout_s, lstc_result *out);
+lstc_status lstc_server_get_stats(lstc_server *srv, lstc_server_stats *out);
+/* Finishes every submitted request, then frees the server. */
+lstc_status lstc_server_destroy(lstc_server *srv);
 
 #ifdef __cplusplus
 }

@@ -87,9 +87,10 @@ def _ptr(a):
 
 
 # ---------------------------------------------------------------- generators
-def gen_rmat(scale: int, edge_factor: int = 16, a=0.57, b=0.19, c=0.19, seed: int = 1):
-    """lo_gen_rmat: raw RMAT samples (m_raw = edge_factor * 2**scale)."""
-    m = int(edge_factor) << int(scale)
+def gen_rmat(scale: int, edge_factor: int = 16, a=0.57, b=0.19, c=0.19, seed: int = 1,
+             m_raw: int | None = None):
+    """lo_gen_rmat: raw RMAT samples (m_raw = edge_factor * 2**scale unless given)."""
+    m = (int(edge_factor) << int(scale)) if m_raw is None else int(m_raw)
     s = np.empty(m, np.uint32)
     d = np.empty(m, np.uint32)
     lib().lo_gen_rmat(scale, m, a, b, c, seed, s, d)

@@ -1,0 +1,118 @@
+"""GPU tests of the serving loop (lstc_server_*): exact results alone and in the
+mix, HBM-budget admission and rejection, caller-resident CSR requests."""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _req_list(rng, k, lo=1e3, hi=2e5):
+    out = []
+    for i in range(k):
+        m = int(round(10 ** rng.uniform(np.log10(lo), np.log10(hi))))
+        if rng.uniform() < 0.7:
+            scale = max(6, int(np.ceil(np.log2(max(m / 8, 64)))))
+            out.append(("rmat", scale, m, 1000 + i))
+        else:
+            n = max(64, m // 10)
+            out.append(("er", n, m, 2000 + i))
+    return out
+
+
+def _oracle_count(O, kind, a, m, seed):
+    if kind == "rmat":
+        n, s, d = O.gen_rmat(a, seed=seed, m_raw=m)
+    else:
+        n, s, d = O.gen_er(a, seed=seed, m_raw=m)
+    sym, g = O.clean_orient(n, s, d)
+    return O.count(g), g.m
+
+
+def _submit(srv, req):
+    kind, a, m, seed = req
+    return srv.submit_rmat(a, m, seed) if kind == "rmat" else srv.submit_er(a, m, seed)
+
+
+def test_results_exact_alone_and_in_mix(lstc, oracle):
+    rng = np.random.default_rng(5)
+    reqs = _req_list(rng, 40)
+    want = [_oracle_count(oracle, *r) for r in reqs]
+    srv = lstc.Server(2 << 30)
+    try:
+        # in the mix: staggered arrivals so later requests join while earlier ones run
+        ids = []
+        for r in reqs:
+            ids.append(_submit(srv, r))
+            time.sleep(0.0005)
+        got = [srv.wait(i, 120) for i in ids]
+        assert [g["count"] for g in got] == [w[0] for w in want]
+        assert [g["edges"] for g in got] == [w[1] for w in want]
+        st = srv.stats()
+        assert st["completed"] == len(reqs) and st["rejected"] == 0
+        assert st["pool_used_bytes"] == 0
+        # alone: one request at a time through the same server
+        for r, w in zip(reqs[:8], want[:8]):
+            assert srv.wait(_submit(srv, r), 60)["count"] == w[0]
+    finally:
+        srv.close()
+
+
+def test_over_budget_rejected(lstc, oracle):
+    srv = lstc.Server(8 << 20)  # 8 MiB
+    try:
+        with pytest.raises(lstc.LstcError) as e:
+            srv.submit_rmat(20, 16 << 20, 1)
+        assert e.value.status == -3  # LSTC_ERR_OVER_BUDGET
+        assert srv.stats()["rejected"] == 1
+        rid = srv.submit_er(500, 3000, 7)  # a small one still runs
+        assert srv.wait(rid, 60)["count"] == _oracle_count(oracle, "er", 500, 3000, 7)[0]
+    finally:
+        srv.close()
+
+
+def test_tight_budget_queues_and_stays_exact(lstc, oracle):
+    rng = np.random.default_rng(11)
+    reqs = _req_list(rng, 24, 2e4, 1e5)
+    want = [_oracle_count(oracle, *r)[0] for r in reqs]
+    srv = lstc.Server(48 << 20)  # room for only a few working sets at a time
+    try:
+        ids = [_submit(srv, r) for r in reqs]
+        got = [srv.wait(i, 120)["count"] for i in ids]
+        assert got == want
+        st = srv.stats()
+        assert st["batches"] >= 2 and st["pool_used_bytes"] == 0
+    finally:
+        srv.close()
+
+
+def test_resident_csr_requests(lstc, oracle):
+    graphs, want = [], []
+    for seed in range(5):
+        n, s, d = oracle.gen_rmat(11, 8, seed=seed)
+        sym, g = oracle.clean_orient(n, s, d)
+        graphs.append(lstc.Graph.from_numpy(g.off, g.col))
+        want.append(oracle.count(g))
+    srv = lstc.Server(256 << 20)
+    try:
+        ids = [srv.submit_graph(g) for g in graphs]
+        assert [srv.wait(i, 60)["count"] for i in ids] == want
+        assert [lstc.count(g) for g in graphs] == want
+    finally:
+        srv.close()
+
+
+def test_poll_not_ready_then_ready(lstc, oracle):
+    srv = lstc.Server(512 << 20)
+    try:
+        rid = srv.submit_rmat(14, 1 << 18, 3)
+        seen = None
+        t0 = time.time()
+        while seen is None and time.time() - t0 < 60:
+            seen = srv.poll(rid)
+        assert seen is not None
+        assert seen["count"] == _oracle_count(oracle, "rmat", 14, 1 << 18, 3)[0]
+        assert srv.poll(rid) is None  # a taken result is gone
+    finally:
+        srv.close()

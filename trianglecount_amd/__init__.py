@@ -5,7 +5,9 @@ built into trianglecount_amd/_lib/liblstc.so.  This package is the Python host
 side over that ABI; torch supplies device memory and streams only.
 """
 from ._lib import LstcError, Tune, GraphDesc, load, check, header_functions  # noqa: F401
-from .core import (Graph, Workspace, clean_orient, count, generate_chung_lu,  # noqa: F401
-                   generate_er, generate_grid, generate_rmat, partition_random, stream_handle)
+from .core import (BatchPlan, Graph, Server, Workspace, clean_orient, count, count_batch,  # noqa: F401
+                   generate_chung_lu,
+                   generate_er, generate_grid, generate_rmat, halo_extract, partition_random,
+                   stream_handle)
 
 __version__ = "1.0.0"

@@ -31,9 +31,9 @@ class LstcError(RuntimeError):
 
 class Tune(C.Structure):
     """lstc_tune (include/lstc.h)."""
-    _fields_ = [("wg_waves", C.c_uint32), ("table_scale", C.c_uint32),
-                ("switch_ratio", C.c_uint32), ("split_segments", C.c_uint32),
-                ("reserved", C.c_uint32 * 4)]
+    _fields_ = [("table_scale", C.c_uint32), ("small_max", C.c_uint32),
+                ("small_group", C.c_uint32), ("grid_per_cu", C.c_uint32),
+                ("unroll", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "reserved"}
@@ -41,9 +41,33 @@ class Tune(C.Structure):
 
 class GraphDesc(C.Structure):
     """lstc_graph_desc (include/lstc.h)."""
-    _fields_ = [("off", C.c_void_p), ("col", C.c_void_p), ("n_rows", C.c_uint32),
-                ("n_owned", C.c_uint32)]
+    _fields_ = [("off", C.c_void_p), ("col", C.c_void_p), ("per_row", C.c_void_p),
+                ("m", C.c_uint64), ("n_rows", C.c_uint32), ("n_owned", C.c_uint32)]
 
+
+class Request(C.Structure):
+    """lstc_request (include/lstc.h)."""
+    _fields_ = [("kind", C.c_uint32), ("scale", C.c_uint32), ("n", C.c_uint32), ("n_rows", C.c_uint32),
+                ("m_raw", C.c_uint64), ("seed", C.c_uint64), ("a", C.c_double), ("b", C.c_double),
+                ("c", C.c_double), ("off", C.c_void_p), ("col", C.c_void_p), ("m", C.c_uint64)]
+
+
+class Result(C.Structure):
+    """lstc_result (include/lstc.h)."""
+    _fields_ = [("count", C.c_uint64), ("edges", C.c_uint64), ("latency_s", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    """lstc_server_stats (include/lstc.h)."""
+    _fields_ = [(k, C.c_uint64) for k in ("submitted", "completed", "rejected", "batches", "max_batch_graphs",
+                                          "pool_bytes", "pool_used_bytes", "edges", "triangles")] + \
+               [(k, C.c_double) for k in ("p50_s", "p99_s", "mean_s", "max_s")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+LSTC_REQ_RMAT, LSTC_REQ_ER, LSTC_REQ_CSR = 0, 1, 2
 
 VP = C.c_void_p
 U32, U64, I32, DBL, SZ = C.c_uint32, C.c_uint64, C.c_int, C.c_double, C.c_size_t
@@ -63,8 +87,23 @@ SIGNATURES = {
     "lstc_partition_random": (ST, [U32, U32, U64, VP, VP]),
     "lstc_clean_orient_workspace": (ST, [U32, U64, PSZ]),
     "lstc_clean_orient": (ST, [VP, VP, U64, U32, I32, VP, VP, VP, VP, PU32, PU64, VP, SZ, VP]),
+    "lstc_halo_workspace": (ST, [U32, PSZ]),
+    "lstc_halo_sizes": (ST, [VP, VP, U32, VP, U32, I32, PU32, PU32, PU64, VP, SZ, VP]),
+    "lstc_halo_extract": (ST, [VP, VP, U32, VP, U32, I32, VP, VP, VP, U32, U64, VP, SZ, VP]),
     "lstc_count_workspace": (ST, [U32, U32, U64, C.POINTER(Tune), PSZ]),
     "lstc_count": (ST, [VP, VP, U32, U32, U64, VP, VP, C.POINTER(Tune), VP, SZ, VP]),
+    "lstc_count_batch_workspace": (ST, [C.POINTER(GraphDesc), U32, C.POINTER(Tune), PSZ]),
+    "lstc_count_batch": (ST, [C.POINTER(GraphDesc), U32, VP, C.POINTER(Tune), VP, SZ, VP]),
+    "lstc_batch_plan_create": (ST, [C.POINTER(GraphDesc), U32, C.POINTER(Tune), C.POINTER(VP)]),
+    "lstc_batch_plan_run": (ST, [VP, VP, VP]),
+    "lstc_batch_plan_launch_graph": (ST, [VP, VP, VP]),
+    "lstc_batch_plan_destroy": (ST, [VP]),
+    "lstc_server_create": (ST, [SZ, U32, C.POINTER(Tune), C.POINTER(VP)]),
+    "lstc_server_submit": (ST, [VP, C.POINTER(Request), PU64]),
+    "lstc_server_poll": (ST, [VP, U64, C.POINTER(Result)]),
+    "lstc_server_wait": (ST, [VP, U64, DBL, C.POINTER(Result)]),
+    "lstc_server_get_stats": (ST, [VP, C.POINTER(ServerStats)]),
+    "lstc_server_destroy": (ST, [VP]),
 }
 
 _lib = None

@@ -24,7 +24,7 @@ import torch
 from ._lib import LstcError, Tune, check, load
 
 __all__ = ["Graph", "generate_rmat", "generate_er", "generate_chung_lu", "generate_grid",
-           "clean_orient", "count", "partition_random", "Workspace", "stream_handle"]
+           "clean_orient", "count", "partition_random", "halo_extract", "Workspace", "stream_handle"]
 
 
 def _dev():
@@ -214,3 +214,174 @@ def count(g: Graph, n_owned: int | None = None, per_row: bool = False, tune: Tun
         import numpy as np
         return total, pr[:no].cpu().numpy().view(np.uint64)
     return total
+
+
+# ------------------------------------------------------------------ (b) halo
+def halo_extract(g: Graph, part: torch.Tensor, p: int, mode: int = 0, with_l2g: bool = False,
+                 ws: Workspace | None = None, stream=None) -> Graph:
+    """Halo subgraph of part ``p`` from the symmetric CSR kept in ``g`` (keep_symmetric=True).
+
+    Returns an oriented CSR whose first ``n_owned`` rows are the owned vertices;
+    ``meta['l2g']`` holds the local -> global map when asked.
+    """
+    if g.sym_off is None:
+        raise ValueError("halo_extract needs the symmetric CSR: clean_orient(..., keep_symmetric=True)")
+    L = load()
+    nb = C.c_size_t()
+    check(L.lstc_halo_workspace(g.n, C.byref(nb)), "lstc_halo_workspace")
+    wp, wb = (ws or _default_ws).get(nb.value)
+    sh = C.c_void_p(stream_handle(stream))
+    nl, no, ml = C.c_uint32(), C.c_uint32(), C.c_uint64()
+    check(L.lstc_halo_sizes(_ptr(g.sym_off), _ptr(g.sym_col), g.n, _ptr(part), p, mode, C.byref(nl),
+                            C.byref(no), C.byref(ml), wp, wb, sh), "lstc_halo_sizes")
+    off = empty_u64(nl.value + 1)
+    col = empty_u32(ml.value)
+    l2g = empty_u32(nl.value) if with_l2g else None
+    check(L.lstc_halo_extract(_ptr(g.sym_off), _ptr(g.sym_col), g.n, _ptr(part), p, mode, _ptr(off),
+                              _ptr(col), _ptr(l2g), nl.value, ml.value, wp, wb, sh), "lstc_halo_extract")
+    h = Graph(nl.value, ml.value, off, col, n_owned=no.value)
+    if with_l2g:
+        h.meta["l2g"] = l2g[: nl.value]
+    return h
+
+
+# ------------------------------------------------------------------ batching
+def _descs(graphs, per_row=None):
+    from ._lib import GraphDesc
+    arr = (GraphDesc * len(graphs))()
+    for i, g in enumerate(graphs):
+        arr[i].off = g.off.data_ptr()
+        arr[i].col = g.col.data_ptr()
+        arr[i].per_row = per_row[i].data_ptr() if per_row is not None else None
+        arr[i].m = g.m
+        arr[i].n_rows = g.n
+        arr[i].n_owned = g.owned
+    return arr
+
+
+def count_batch(graphs: list, tune: Tune | None = None, ws: Workspace | None = None, stream=None,
+                out: torch.Tensor | None = None, sync: bool = True):
+    """All graphs in one persistent launch; returns a list of per-graph counts."""
+    L = load()
+    G = len(graphs)
+    arr = _descs(graphs)
+    tp = C.byref(tune) if tune is not None else None
+    nb = C.c_size_t()
+    check(L.lstc_count_batch_workspace(arr, G, tp, C.byref(nb)), "lstc_count_batch_workspace")
+    wp, wb = (ws or _default_ws).get(nb.value)
+    res = out if out is not None else empty_u64(G)
+    check(L.lstc_count_batch(arr, G, _ptr(res), tp, wp, wb, C.c_void_p(stream_handle(stream))),
+          "lstc_count_batch")
+    if not sync:
+        return res
+    return [int(x) & 0xFFFFFFFFFFFFFFFF for x in res[:G].cpu().tolist()]
+
+
+class BatchPlan:
+    """Resident batch plan: descriptors + workspace on the device, optional hipGraph replay."""
+
+    def __init__(self, graphs: list, tune: Tune | None = None):
+        self.L = load()
+        self.graphs = list(graphs)  # keep the tensors alive
+        self.G = len(graphs)
+        self._arr = _descs(graphs)
+        self.h = C.c_void_p()
+        check(self.L.lstc_batch_plan_create(self._arr, self.G, C.byref(tune) if tune is not None else None,
+                                            C.byref(self.h)), "lstc_batch_plan_create")
+        self.counts = empty_u64(self.G)
+
+    def run(self, stream=None):
+        check(self.L.lstc_batch_plan_run(self.h, _ptr(self.counts), C.c_void_p(stream_handle(stream))),
+              "lstc_batch_plan_run")
+        return self.counts
+
+    def launch_graph(self, stream=None):
+        check(self.L.lstc_batch_plan_launch_graph(self.h, _ptr(self.counts), C.c_void_p(stream_handle(stream))),
+              "lstc_batch_plan_launch_graph")
+        return self.counts
+
+    def results(self):
+        torch.cuda.synchronize()
+        return [int(x) & 0xFFFFFFFFFFFFFFFF for x in self.counts[: self.G].cpu().tolist()]
+
+    def close(self):
+        if self.h:
+            self.L.lstc_batch_plan_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ serving
+class Server:
+    """Continuously batched count server (lstc_server_*) with an HBM budget."""
+
+    def __init__(self, hbm_budget_bytes: int, max_batch: int = 1024, tune: Tune | None = None):
+        from ._lib import Request, Result, ServerStats  # noqa: F401
+        self.L = load()
+        _dev()
+        self.h = C.c_void_p()
+        check(self.L.lstc_server_create(int(hbm_budget_bytes), int(max_batch),
+                                        C.byref(tune) if tune is not None else None, C.byref(self.h)),
+              "lstc_server_create")
+        self._keep = {}
+
+    def submit_rmat(self, scale: int, m_raw: int, seed: int, a=0.57, b=0.19, c=0.19) -> int:
+        from ._lib import LSTC_REQ_RMAT, Request
+        r = Request(kind=LSTC_REQ_RMAT, scale=scale, m_raw=m_raw, seed=seed, a=a, b=b, c=c)
+        return self._submit(r)
+
+    def submit_er(self, n: int, m_raw: int, seed: int) -> int:
+        from ._lib import LSTC_REQ_ER, Request
+        return self._submit(Request(kind=LSTC_REQ_ER, n=n, m_raw=m_raw, seed=seed))
+
+    def submit_graph(self, g: "Graph") -> int:
+        from ._lib import LSTC_REQ_CSR, Request
+        r = Request(kind=LSTC_REQ_CSR, n_rows=g.n, off=g.off.data_ptr(), col=g.col.data_ptr(), m=g.m)
+        rid = self._submit(r)
+        self._keep[rid] = g  # the device CSR must outlive the request
+        return rid
+
+    def _submit(self, r) -> int:
+        rid = C.c_uint64()
+        check(self.L.lstc_server_submit(self.h, C.byref(r), C.byref(rid)), "lstc_server_submit")
+        return rid.value
+
+    def poll(self, rid: int):
+        from ._lib import LSTC_ERR_NOT_READY, Result
+        res = Result()
+        st = self.L.lstc_server_poll(self.h, rid, C.byref(res))
+        if st == LSTC_ERR_NOT_READY:
+            return None
+        check(st, "lstc_server_poll")
+        self._keep.pop(rid, None)
+        return {"count": int(res.count), "edges": int(res.edges), "latency_s": float(res.latency_s)}
+
+    def wait(self, rid: int, timeout_s: float = 600.0):
+        from ._lib import Result
+        res = Result()
+        check(self.L.lstc_server_wait(self.h, rid, float(timeout_s), C.byref(res)), "lstc_server_wait")
+        self._keep.pop(rid, None)
+        return {"count": int(res.count), "edges": int(res.edges), "latency_s": float(res.latency_s)}
+
+    def stats(self) -> dict:
+        from ._lib import ServerStats
+        st = ServerStats()
+        check(self.L.lstc_server_get_stats(self.h, C.byref(st)), "lstc_server_get_stats")
+        return st.as_dict()
+
+    def close(self):
+        if self.h:
+            self.L.lstc_server_destroy(self.h)
+            self.h = C.c_void_p()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,61 @@
+// count_engine.h -- the persistent counting engine shared by lstc_count,
+// lstc_count_batch, batch plans and the server.
+#pragma once
+#include "lstc_internal.h"
+
+namespace lstc {
+
+// One graph as the kernels see it.
+struct GraphDev {
+    const uint64_t *off;
+    const uint32_t *col;
+    unsigned long long *per_row;  // may be null
+    uint32_t n_rows;
+    uint32_t n_owned;
+};
+
+// The graphs of one launch, passed by value.  A single graph travels inline
+// (no device table, so the launch is fully stream-ordered and capturable);
+// batches use a device table and a device prefix of owned-row counts.
+struct GraphSet {
+    const GraphDev *table;     // device [G] or null when G == 1
+    const uint64_t *row_base;  // device [G+1] or null when G == 1
+    GraphDev one;
+    uint64_t total_rows;       // sum of n_owned
+    uint32_t G;
+};
+
+struct HostGraph {
+    const uint64_t *off;
+    const uint32_t *col;
+    uint64_t *per_row;
+    uint32_t n_rows, n_owned;
+    uint64_t m;
+};
+
+// Resolved tuning (defaults filled in, ranges clamped).
+struct TuneR {
+    uint32_t table_scale;   // LDS table capacity >= table_scale * keys
+    uint32_t small_max;     // rows with d+ <= small_max use a wave-private slice
+    uint32_t small_group;   // small rows per ticket
+    uint32_t grid_per_cu;   // persistent workgroups per CU
+    uint32_t unroll;        // chunks in flight per wave (4 or 8)
+};
+TuneR resolve_tune(const lstc_tune *t);
+
+lstc_status validate_graph(const HostGraph &h, const char *who);
+
+// Workspace for a launch over G graphs with total_rows owned rows (excludes the
+// batch descriptor table, which the caller places).
+size_t count_ws_bytes(uint32_t G, uint64_t total_rows);
+
+// hosts[G] describe the graphs (used for sizes, per-row zeroing and, when G == 1,
+// as the inline descriptor).  For G > 1, d_table[G] and d_row_base[G+1] must
+// already hold the same descriptors and the owned-row prefix on the device.
+// d_counts[G] are overwritten.  Only kernels are enqueued (no copies, no
+// synchronisation), so the sequence can be captured into a hipGraph.
+lstc_status count_launch(const HostGraph *hosts, uint32_t G, const GraphDev *d_table, const uint64_t *d_row_base,
+                         unsigned long long *d_counts, const lstc_tune *tune, void *ws, size_t ws_bytes,
+                         hipStream_t st);
+
+}  // namespace lstc

@@ -1,0 +1,408 @@
+// count_engine.hip -- persistent triangle-counting engine (gfx950, wave64).
+//
+// Count of a graph = sum over owned rows u of sum_{w in N+(u)} |N+(u) n N+(w)|
+// (src/WithoutPartition/ShareKernelTCV62_update_v1.cu:43-191 summed as at
+// :271-277; owned rows only as in src/WithPartition/without_sorting_tc.cu:400).
+//
+// Plan (k_plan_rows): owned rows with d+(u) >= 2 of every graph of the launch
+// are binned by d+ into one queue, largest bin first.  Rows above small_max are
+// "big" tickets (one row each); smaller rows form "small" tickets of
+// small_group rows.
+//
+// Count (k_count, one persistent launch): each workgroup of kWaves waves takes
+// tickets from an atomic head.
+//   big row   : the whole workgroup builds one LDS hash table of N+(u) (up to
+//               4096 slots, load <= 1/2; rows above 2048 keys are tiled) and the
+//               waves split the 64-segment windows of N+(u) round-robin;
+//   small row : one wave, a private 1024-slot slice of the same LDS.
+// Walk of one window: one segment per lane (w, start, length), inclusive scan
+// of lengths, empty segments compacted out; then the flat sequence of entries
+// is consumed 64 per chunk.  A chunk inside one segment takes its base from the
+// current segment (scalar); otherwise each lane finds its segment from a
+// boundary mask (LDS flags + ballot + popcount).  UNROLL chunks are addressed
+// and loaded before the first probe, then all of them are probed in lockstep,
+// so each wave keeps UNROLL global loads in flight.  Hits are counted as
+// popcount(ballot(hit)) into a scalar; the workgroup issues one uint64 atomic
+// per graph it counted.
+#include "count_engine.h"
+#include "wave.h"
+
+namespace lstc {
+
+static constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+static constexpr int kWaves = 4;
+static constexpr uint32_t kSliceLog2 = 10;
+static constexpr uint32_t kSlice = 1u << kSliceLog2;  // wave-private slice (small rows)
+static constexpr uint32_t kBigLog2 = 12;
+static constexpr uint32_t kBigCap = 1u << kBigLog2;   // shared table (big rows)
+static constexpr uint32_t kBigTile = kBigCap / 2;     // keys per table fill of a big row
+static constexpr int kBins = 6;
+
+// ctl (uint32) layout
+enum { C_CNT = 0, C_CUR = 6, C_HEAD = 12, C_NBIG = 13, C_NSMALL = 14, C_NTICK = 15, C_WORDS = 16 };
+
+TuneR resolve_tune(const lstc_tune *t) {
+    TuneR r{4, 256, 16, 0, 8};
+    if (t) {
+        if (t->table_scale) r.table_scale = t->table_scale;
+        if (t->small_max) r.small_max = t->small_max;
+        if (t->small_group) r.small_group = t->small_group;
+        if (t->grid_per_cu) r.grid_per_cu = t->grid_per_cu;
+        if (t->unroll) r.unroll = t->unroll;
+    }
+    r.table_scale = hmax(2u, hmin(8u, r.table_scale));
+    // a small row must fit its slice at the chosen load factor
+    r.small_max = hmax(64u, hmin(kSlice / r.table_scale, r.small_max));
+    r.small_group = hmax(4u, hmin(256u, r.small_group));
+    r.unroll = r.unroll >= 8 ? 8 : 4;
+    if (r.grid_per_cu == 0) r.grid_per_cu = 7;  // LDS-limited occupancy: 7 workgroups per CU
+    r.grid_per_cu = hmax(1u, hmin(16u, r.grid_per_cu));
+    return r;
+}
+
+lstc_status validate_graph(const HostGraph &h, const char *who) {
+    if (h.n_owned > h.n_rows) return fail(LSTC_ERR_INVALID, "%s: n_owned %u > n_rows %u", who, h.n_owned, h.n_rows);
+    if (h.n_rows == 0xFFFFFFFFu) return fail(LSTC_ERR_INVALID, "%s: n_rows must be < 2^32-1", who);
+    if (h.m >= 0xFFFFF000ull)
+        return fail(LSTC_ERR_INVALID, "%s: m = %llu arcs exceeds 2^32-4096", who, (unsigned long long)h.m);
+    if (h.n_owned && (!h.off || (h.m && !h.col))) return fail(LSTC_ERR_INVALID, "%s: NULL CSR arrays", who);
+    return LSTC_OK;
+}
+
+struct Bounds {
+    uint32_t hi[kBins];  // bin b holds d+ in (hi[b-1], hi[b]], bin 0 starts at 2
+};
+
+__device__ __forceinline__ GraphDev graph_at(const GraphSet &gs, uint32_t g) {
+    return gs.table ? gs.table[g] : gs.one;
+}
+
+__device__ __forceinline__ uint32_t table_log2_for(uint32_t keys, uint32_t scale, uint32_t max_log2) {
+    uint32_t l = 6;
+    while (l < max_log2 && (1u << l) < scale * keys) ++l;
+    return l;
+}
+
+__device__ __forceinline__ uint32_t hslot(uint32_t x, uint32_t shift) { return (x * 0x9E3779B1u) >> shift; }
+
+// ------------------------------------------------------------------ planning
+__global__ void k_plan_init(uint32_t *ctl, unsigned long long *counts, uint32_t G) {
+    for (uint32_t i = threadIdx.x; i < C_WORDS; i += blockDim.x) ctl[i] = 0;
+    for (uint32_t i = threadIdx.x; i < G; i += blockDim.x) counts[i] = 0;
+}
+
+// pass 0 counts rows per bin; pass 1 scatters (g << 32 | u) into the queue,
+// laid out bins 5,4,3 (big, largest first) then 2,1,0 (small, largest first).
+template <int PASS>
+__global__ __launch_bounds__(256) void k_plan_rows(GraphSet gs, Bounds bd, uint32_t *__restrict__ ctl,
+                                                   unsigned long long *__restrict__ queue) {
+    uint32_t bin_off[kBins];
+    if (PASS == 1) {
+        uint32_t acc = 0;
+        for (int b = kBins - 1; b >= 0; --b) { bin_off[b] = acc; acc += ctl[C_CNT + b]; }
+    }
+    const uint64_t total = gs.total_rows;
+    for (uint64_t r0 = (uint64_t)blockIdx.x * blockDim.x; r0 < total; r0 += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t r = r0 + threadIdx.x;
+        int b = -1;
+        uint32_t g = 0, u = 0;
+        if (r < total) {
+            if (gs.table) {  // last g with row_base[g] <= r
+                uint32_t lo = 0, hi = gs.G;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (gs.row_base[mid] <= r) lo = mid; else hi = mid;
+                }
+                g = lo;
+                u = (uint32_t)(r - gs.row_base[g]);
+            } else {
+                u = (uint32_t)r;
+            }
+            const uint64_t *off = graph_at(gs, g).off;
+            const uint64_t k = off[u + 1] - off[u];
+            if (k >= 2) {
+                b = kBins - 1;
+                for (int i = kBins - 2; i >= 0; --i)
+                    if (k <= bd.hi[i]) b = i;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kBins; ++i) {
+            const uint64_t m = ballot(b == i);
+            if (m == 0) continue;
+            const uint32_t first = (uint32_t)__builtin_ctzll(m);
+            uint32_t base = 0;
+            if (lane_id() == first) base = atomicAdd(ctl + (PASS == 0 ? C_CNT : C_CUR) + i, (uint32_t)__popcll(m));
+            base = read_lane(base, first);
+            if (PASS == 1 && b == i) queue[bin_off[i] + base + mask_rank(m)] = ((unsigned long long)g << 32) | u;
+        }
+    }
+}
+
+__global__ void k_plan_finish(uint32_t *ctl, uint32_t small_group) {
+    if (threadIdx.x == 0) {
+        const uint32_t n_big = ctl[C_CNT + 3] + ctl[C_CNT + 4] + ctl[C_CNT + 5];
+        const uint32_t n_small = ctl[C_CNT + 0] + ctl[C_CNT + 1] + ctl[C_CNT + 2];
+        ctl[C_NBIG] = n_big;
+        ctl[C_NSMALL] = n_small;
+        ctl[C_NTICK] = n_big + (n_small + small_group - 1) / small_group;
+        ctl[C_HEAD] = 0;
+    }
+}
+
+// ------------------------------------------------------------------ table build
+__device__ __forceinline__ void table_insert(uint32_t *tab, uint32_t tlog2, const uint32_t *__restrict__ col,
+                                             uint64_t b0, uint32_t nk, uint32_t tid, uint32_t nthreads) {
+    const uint32_t mask = (1u << tlog2) - 1, shift = 32 - tlog2;
+    for (uint32_t j = tid; j < nk; j += nthreads) {
+        const uint32_t key = col[b0 + j];
+        uint32_t h = hslot(key, shift);
+        while (atomicCAS(&tab[h], kEmpty, key) != kEmpty) h = (h + 1) & mask;
+    }
+}
+
+// ------------------------------------------------------------------ walk
+// Probes every entry of N+(w) for the segments j in [w0, K) step wstep (in
+// windows of 64) of row u (N+(u) at col[rb .. rb+K)) against tab.
+template <int UNROLL>
+__device__ __forceinline__ unsigned long long walk_row(const uint64_t *__restrict__ off,
+                                                       const uint32_t *__restrict__ col, uint32_t n_rows,
+                                                       uint64_t rb, uint32_t K, uint32_t w0, uint32_t wstep,
+                                                       const uint32_t *tab, uint32_t tlog2, uint64_t *s_base,
+                                                       uint32_t *s_end, uint32_t *flag, uint32_t lane) {
+    const uint32_t mask = (1u << tlog2) - 1, shift = 32 - tlog2;
+    uint32_t lane_hits = 0;
+    for (uint32_t j0 = w0; j0 < K; j0 += wstep) {
+        const uint32_t j = j0 + lane;
+        uint64_t s = 0;
+        uint32_t len = 0;
+        if (j < K) {
+            const uint32_t w = col[rb + j];
+            if (w < n_rows) {  // ids outside the CSR have no row
+                s = off[w];
+                len = (uint32_t)(off[w + 1] - s);
+            }
+        }
+        const uint32_t incl = wave_incl_scan(len, lane);
+        const uint32_t total = uniform(gather(incl, 63));
+        const uint64_t nzm = ballot(len != 0);
+        const uint32_t nseg = (uint32_t)__popcll(nzm);
+        if (len != 0) {
+            const uint32_t slot = mask_rank(nzm);
+            s_base[slot] = s - (uint64_t)(incl - len);
+            s_end[slot] = incl;
+        }
+        const uint64_t mybase = s_base[lane];
+        const uint32_t myend = lane < nseg ? s_end[lane] : kEmpty;
+        uint32_t cur = 0;
+        for (uint32_t B = 0; B < total; B += 64u * UNROLL) {
+            uint32_t x[UNROLL];
+            uint32_t h[UNROLL];
+            bool live[UNROLL];
+#pragma unroll
+            for (int k = 0; k < UNROLL; ++k) {
+                const uint32_t Bk = B + 64u * k;
+                x[k] = kEmpty;
+                live[k] = false;
+                if (Bk < total) {  // uniform
+                    const uint32_t cend = read_lane(myend, cur);
+                    const uint32_t t = Bk + lane;
+                    uint64_t base;
+                    if (cend >= Bk + 64) {
+                        base = read_lane64(mybase, cur);
+                        cur += (cend == Bk + 64) ? 1u : 0u;
+                    } else {
+                        const bool mk = lane >= cur && myend < Bk + 64;
+                        if (mk) flag[myend - Bk] = 1;
+                        const uint64_t M = ballot(flag[lane] != 0);
+                        if (mk) flag[myend - Bk] = 0;
+                        const uint32_t seg = cur + (uint32_t)__popcll(M & ((2ull << lane) - 1ull));
+                        base = gather64(mybase, seg);
+                        cur += (uint32_t)__popcll(ballot(lane >= cur && myend <= Bk + 64));
+                    }
+                    if (t < total) {
+                        x[k] = col[base + t];
+                        live[k] = true;
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < UNROLL; ++k) h[k] = hslot(x[k], shift);
+            // lockstep linear probing: one LDS round trip per step for all chunks.
+            // Hits are counted per lane inside the divergent probe (a lane that has
+            // finished is inactive there) and reduced across the wave per row.
+            for (;;) {
+                bool more = false;
+#pragma unroll
+                for (int k = 0; k < UNROLL; ++k) {
+                    if (live[k]) {
+                        const uint32_t v = tab[h[k]];
+                        const bool hit = v == x[k];
+                        lane_hits += hit ? 1u : 0u;
+                        live[k] = !(hit || v == kEmpty);
+                        h[k] = (h[k] + 1) & mask;
+                        more |= live[k];
+                    }
+                }
+                if (ballot(more) == 0) break;
+            }
+        }
+    }
+    return wave_sum64((unsigned long long)lane_hits);
+}
+
+// ------------------------------------------------------------------ persistent count kernel
+template <int UNROLL>
+__global__ __launch_bounds__(kWaves * 64) void k_count(GraphSet gs, const unsigned long long *__restrict__ queue,
+                                                       uint32_t *__restrict__ ctl,
+                                                       unsigned long long *__restrict__ counts, uint32_t scale,
+                                                       uint32_t small_group) {
+    __shared__ uint32_t s_tab[kBigCap];
+    __shared__ uint64_t s_base[kWaves][64];
+    __shared__ uint32_t s_end[kWaves][64];
+    __shared__ uint32_t s_flag[kWaves][64];
+    __shared__ uint32_t s_ticket;
+    __shared__ unsigned long long s_acc[kWaves];
+    __shared__ uint32_t s_g[kWaves];
+
+    const uint32_t tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    s_flag[wv][lane] = 0;
+    const uint32_t n_big = ctl[C_NBIG], n_small = ctl[C_NSMALL], n_tickets = ctl[C_NTICK];
+    uint32_t cur_g = kEmpty;
+    unsigned long long acc = 0;
+
+    for (;;) {
+        if (tid == 0) s_ticket = atomicAdd(ctl + C_HEAD, 1u);
+        __syncthreads();
+        const uint32_t t = s_ticket;
+        __syncthreads();
+        if (t >= n_tickets) break;
+        if (t < n_big) {
+            // big row: the whole workgroup shares one table
+            const unsigned long long it = queue[t];
+            const uint32_t g = (uint32_t)(it >> 32), u = (uint32_t)it;
+            const GraphDev G = graph_at(gs, g);
+            const uint64_t rb = G.off[u];
+            const uint32_t K = (uint32_t)(G.off[u + 1] - rb);
+            unsigned long long hits = 0;
+            for (uint32_t t0 = 0; t0 < K; t0 += kBigTile) {
+                const uint32_t tk = min(K - t0, kBigTile);
+                const uint32_t tl = table_log2_for(tk, scale, kBigLog2);
+                for (uint32_t i = tid; i < (1u << tl); i += kWaves * 64) s_tab[i] = kEmpty;
+                __syncthreads();
+                table_insert(s_tab, tl, G.col, rb + t0, tk, tid, kWaves * 64);
+                __syncthreads();
+                hits += walk_row<UNROLL>(G.off, G.col, G.n_rows, rb, K, wv * 64, kWaves * 64, s_tab, tl,
+                                         s_base[wv], s_end[wv], s_flag[wv], lane);
+                __syncthreads();
+            }
+            if (g != cur_g) {
+                if (cur_g != kEmpty && acc != 0 && lane == 0) atomicAdd(counts + cur_g, acc);
+                cur_g = g;
+                acc = 0;
+            }
+            acc += hits;
+            if (G.per_row && lane == 0 && hits) atomicAdd(G.per_row + u, hits);
+        } else {
+            // small rows: each wave takes every kWaves-th row of the group
+            const uint32_t first = n_big + (t - n_big) * small_group;
+            const uint32_t last = min(first + small_group, n_big + n_small);
+            uint32_t *tab = s_tab + wv * kSlice;
+            for (uint32_t q = first + wv; q < last; q += kWaves) {
+                const unsigned long long it = queue[q];
+                const uint32_t g = (uint32_t)(it >> 32), u = (uint32_t)it;
+                const GraphDev G = graph_at(gs, g);
+                const uint64_t rb = G.off[u];
+                const uint32_t K = (uint32_t)(G.off[u + 1] - rb);
+                const uint32_t tl = table_log2_for(K, scale, kSliceLog2);
+                for (uint32_t i = lane; i < (1u << tl); i += 64) tab[i] = kEmpty;
+                table_insert(tab, tl, G.col, rb, K, lane, 64);
+                const unsigned long long hits = walk_row<UNROLL>(G.off, G.col, G.n_rows, rb, K, 0, 64, tab, tl,
+                                                                 s_base[wv], s_end[wv], s_flag[wv], lane);
+                if (g != cur_g) {
+                    if (cur_g != kEmpty && acc != 0 && lane == 0) atomicAdd(counts + cur_g, acc);
+                    cur_g = g;
+                    acc = 0;
+                }
+                acc += hits;
+                if (G.per_row && lane == 0) G.per_row[u] = hits;
+            }
+        }
+    }
+    // one atomic per graph per workgroup for what the waves still hold
+    if (lane == 0) { s_acc[wv] = acc; s_g[wv] = cur_g; }
+    __syncthreads();
+    if (tid == 0) {
+        for (int i = 0; i < kWaves; ++i) {
+            if (s_g[i] == kEmpty) continue;
+            unsigned long long sum = s_acc[i];
+            for (int j = i + 1; j < kWaves; ++j)
+                if (s_g[j] == s_g[i]) { sum += s_acc[j]; s_g[j] = kEmpty; }
+            if (sum) atomicAdd(counts + s_g[i], sum);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ host side
+size_t count_ws_bytes(uint32_t G, uint64_t total_rows) {
+    (void)G;
+    Carver c(nullptr, 0);
+    c.take<uint32_t>(64);
+    c.take<unsigned long long>(total_rows ? total_rows : 1);
+    return c.need();
+}
+
+lstc_status count_launch(const HostGraph *hosts, uint32_t G, const GraphDev *d_table, const uint64_t *d_row_base,
+                         unsigned long long *d_counts, const lstc_tune *tune, void *ws, size_t ws_bytes,
+                         hipStream_t st) {
+    if (G == 0) return LSTC_OK;
+    if (!d_counts) return fail(LSTC_ERR_INVALID, "count: NULL counts");
+    if (G > 1 && (!d_table || !d_row_base)) return fail(LSTC_ERR_INVALID, "count: batch needs device descriptors");
+    const TuneR tr = resolve_tune(tune);
+    uint64_t total = 0;
+    for (uint32_t g = 0; g < G; ++g) total += hosts[g].n_owned;
+    if (total >= 0xFFFFFFFFull) return fail(LSTC_ERR_INVALID, "count: more than 2^32-1 owned rows in one launch");
+    Carver c(ws, ws_bytes);
+    uint32_t *ctl = c.take<uint32_t>(64);
+    unsigned long long *queue = c.take<unsigned long long>(total ? total : 1);
+    if (!ws || !c.ok()) return fail(LSTC_ERR_WORKSPACE, "count: workspace %zu < %zu bytes", ws_bytes, c.need());
+
+    GraphSet gs{};
+    gs.G = G;
+    gs.total_rows = total;
+    if (G == 1) {
+        gs.one = GraphDev{hosts[0].off, hosts[0].col, (unsigned long long *)hosts[0].per_row, hosts[0].n_rows,
+                          hosts[0].n_owned};
+    } else {
+        gs.table = d_table;
+        gs.row_base = d_row_base;
+    }
+    for (uint32_t g = 0; g < G; ++g)
+        if (hosts[g].per_row) LSTC_TRY(fill_u64(hosts[g].per_row, 0, hosts[g].n_owned, st));
+    hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(256), 0, st, ctl, d_counts, G);
+    LSTC_LAUNCHED("k_plan_init");
+    const int cus = device_cus();
+    Bounds bd{{16, 64, tr.small_max, 1024, 2048, 0xFFFFFFFFu}};
+    if (total) {
+        const uint32_t grid = (uint32_t)hmin<uint64_t>((total + 255) / 256, (uint64_t)cus * 8);
+        hipLaunchKernelGGL(k_plan_rows<0>, dim3(grid), dim3(256), 0, st, gs, bd, ctl, queue);
+        LSTC_LAUNCHED("k_plan_rows<0>");
+        hipLaunchKernelGGL(k_plan_rows<1>, dim3(grid), dim3(256), 0, st, gs, bd, ctl, queue);
+        LSTC_LAUNCHED("k_plan_rows<1>");
+    }
+    hipLaunchKernelGGL(k_plan_finish, dim3(1), dim3(64), 0, st, ctl, tr.small_group);
+    LSTC_LAUNCHED("k_plan_finish");
+    if (total) {
+        const uint32_t grid = (uint32_t)cus * tr.grid_per_cu;
+        if (tr.unroll == 8)
+            hipLaunchKernelGGL(k_count<8>, dim3(grid), dim3(kWaves * 64), 0, st, gs, queue, ctl, d_counts,
+                               tr.table_scale, tr.small_group);
+        else
+            hipLaunchKernelGGL(k_count<4>, dim3(grid), dim3(kWaves * 64), 0, st, gs, queue, ctl, d_counts,
+                               tr.table_scale, tr.small_group);
+        LSTC_LAUNCHED("k_count");
+    }
+    return LSTC_OK;
+}
+
+}  // namespace lstc

@@ -89,6 +89,13 @@ size_t radix_sort_ws_bytes(uint64_t n);
 lstc_status radix_sort_u64(uint64_t *keys, uint64_t *alt, uint64_t n, int key_bits,
                            uint64_t **result, void *ws, size_t ws_bytes, hipStream_t s);
 
+// (a) clean + orient without synchronising (prep.hip); sizes stay on the device:
+// (*d_sizes)[1] = vertices after cleaning, (*d_sizes)[2] = undirected edges.
+lstc_status clean_orient_launch(const uint32_t *d_src, const uint32_t *d_dst, uint64_t m_raw, uint32_t n,
+                                int compact, uint64_t *d_sym_off, uint32_t *d_sym_col, uint64_t *d_off,
+                                uint32_t *d_col, void *d_ws, size_t ws_bytes, hipStream_t st,
+                                const uint64_t **d_sizes, const uint32_t **d_bad);
+
 // Fill kernels (used instead of hipMemsetAsync so callers may capture them in graphs).
 lstc_status fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s);
 lstc_status fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);

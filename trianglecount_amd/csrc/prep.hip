@@ -60,10 +60,28 @@ __global__ void k_unique_scatter(const uint64_t *__restrict__ k, uint64_t nk,
     GRID_STRIDE(i, nk) if (f[i]) uk[pos[i]] = k[i];
 }
 
-__global__ void k_degree(const uint64_t *__restrict__ uk, const uint64_t *__restrict__ d_m2,
-                         uint32_t b, uint32_t *__restrict__ deg) {
+// Row boundaries of the sorted symmetric arc list: first[s], last[s] (exclusive).
+// Vertices without arcs keep first == last == 0 (pre-filled).  No atomics: a hub
+// row is one run of the sorted list, so its degree is last - first.
+__global__ void k_row_bounds(const uint64_t *__restrict__ uk, const uint64_t *__restrict__ d_m2, uint32_t b,
+                             uint64_t *__restrict__ first, uint64_t *__restrict__ last) {
     const uint64_t m2 = *d_m2;
-    GRID_STRIDE(i, m2) atomicAdd(deg + (uk[i] >> b), 1u);
+    GRID_STRIDE(i, m2) {
+        const uint64_t s = uk[i] >> b;
+        if (i == 0 || (uk[i - 1] >> b) != s) first[s] = i;
+        if (i + 1 == m2 || (uk[i + 1] >> b) != s) last[s] = i + 1;
+    }
+}
+
+__global__ void k_degree(const uint64_t *__restrict__ first, const uint64_t *__restrict__ last, uint32_t n,
+                         uint32_t *__restrict__ deg) {
+    GRID_STRIDE(v, n) deg[v] = (uint32_t)(last[v] - first[v]);
+}
+
+// kept arcs per row from the exclusive scan of keep flags at the row boundaries
+__global__ void k_out_degree(const uint64_t *__restrict__ first, const uint64_t *__restrict__ last,
+                             const uint64_t *__restrict__ pos, uint32_t n, uint32_t *__restrict__ dout) {
+    GRID_STRIDE(v, n) dout[v] = (uint32_t)(pos[last[v]] - pos[first[v]]);
 }
 
 __global__ void k_nonzero(const uint32_t *__restrict__ deg, uint32_t n, int compact,
@@ -94,18 +112,17 @@ __global__ void k_sym_cols(const uint64_t *__restrict__ uk, const uint64_t *__re
     GRID_STRIDE(i, m2) sym_col[i] = nid[uk[i] & mask];
 }
 
+// keep flags for [0, nk]; entries from m2 on (stale from the unique pass) become 0
 __global__ void k_orient_flags(const uint64_t *__restrict__ uk, const uint64_t *__restrict__ d_m2,
                                uint64_t nk, uint32_t b, uint64_t mask, const uint32_t *__restrict__ deg,
-                               uint32_t *__restrict__ f, uint32_t *__restrict__ dout) {
+                               uint32_t *__restrict__ f) {
     const uint64_t m2 = *d_m2;
-    GRID_STRIDE(i, nk) {
+    GRID_STRIDE(i, nk + 1) {
         if (i >= m2) { f[i] = 0u; continue; }
         const uint64_t k = uk[i];
         const uint32_t s = (uint32_t)(k >> b), d = (uint32_t)(k & mask);
         const uint32_t ds = deg[s], dd = deg[d];
-        const bool keep = ds < dd || (ds == dd && d < s);
-        f[i] = keep ? 1u : 0u;
-        if (keep) atomicAdd(dout + s, 1u);
+        f[i] = (ds < dd || (ds == dd && d < s)) ? 1u : 0u;
     }
 }
 
@@ -129,7 +146,7 @@ struct PrepLayout {
     uint64_t *keys, *alt, *pos;
     uint32_t *flags;
     uint32_t *deg, *keepv, *nid, *dout;
-    uint64_t *degscan, *nidscan, *doscan;
+    uint64_t *degscan, *nidscan, *doscan, *first, *last;
     uint64_t *scal;  // [0]=m2 [1]=n_out [2]=m [3]=unused
     uint32_t *bad;
     void *sort_ws;
@@ -154,11 +171,13 @@ static PrepLayout prep_layout(void *ws, size_t ws_bytes, uint32_t n, uint64_t m_
     L.degscan = c.take<uint64_t>((size_t)n + 1);
     L.nidscan = c.take<uint64_t>((size_t)n + 1);
     L.doscan = c.take<uint64_t>((size_t)n + 1);
+    L.first = c.take<uint64_t>((size_t)n + 1);
+    L.last = c.take<uint64_t>((size_t)n + 1);
     L.scal = c.take<uint64_t>(8);
     L.bad = c.take<uint32_t>(4);
     L.sort_bytes = radix_sort_ws_bytes(nk);
     L.sort_ws = c.take<char>(L.sort_bytes);
-    L.scan_bytes = scan_ws_bytes(nk > n ? nk : n);
+    L.scan_bytes = scan_ws_bytes(nk + 1 > n ? nk + 1 : n);
     L.scan_ws = c.take<char>(L.scan_bytes);
     L.bytes = c.need();
     return L;
@@ -174,14 +193,15 @@ extern "C" lstc_status lstc_clean_orient_workspace(uint32_t n, uint64_t m_raw, s
     return LSTC_OK;
 }
 
-extern "C" lstc_status lstc_clean_orient(const uint32_t *d_src, const uint32_t *d_dst, uint64_t m_raw,
-                                         uint32_t n, int compact, uint64_t *d_sym_off,
-                                         uint32_t *d_sym_col, uint64_t *d_off, uint32_t *d_col,
-                                         uint32_t *n_out, uint64_t *m_out, void *d_ws,
-                                         size_t ws_bytes, lstc_stream s) {
-    hipStream_t st = (hipStream_t)s;
+// Enqueues the whole pipeline without synchronising.  On return *d_sizes points
+// at device words {m2, n_out, m, -} and *d_bad at the out-of-range flag; both are
+// valid once the stream reaches this point.
+lstc_status lstc::clean_orient_launch(const uint32_t *d_src, const uint32_t *d_dst, uint64_t m_raw, uint32_t n,
+                                      int compact, uint64_t *d_sym_off, uint32_t *d_sym_col, uint64_t *d_off,
+                                      uint32_t *d_col, void *d_ws, size_t ws_bytes, hipStream_t st,
+                                      const uint64_t **d_sizes, const uint32_t **d_bad) {
     if (n == 0xFFFFFFFFu) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: n must be < 2^32-1");
-    if (!d_off || !n_out || !m_out) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: NULL output");
+    if (!d_off) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: NULL output");
     if ((d_sym_off == nullptr) != (d_sym_col == nullptr))
         return fail(LSTC_ERR_INVALID, "lstc_clean_orient: give both symmetric outputs or neither");
     if (m_raw && (!d_src || !d_dst || !d_col)) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: NULL edge arrays");
@@ -209,7 +229,11 @@ extern "C" lstc_status lstc_clean_orient(const uint32_t *d_src, const uint32_t *
         LSTC_TRY(scan_exclusive_u32_to_u64(L.flags, L.pos, nk, L.scal + 0, L.scan_ws, L.scan_bytes, st));
         hipLaunchKernelGGL(k_unique_scatter, dim3(grid_k), dim3(256), 0, st, sorted, nk, L.flags, L.pos, uk);
         LSTC_LAUNCHED("k_unique_scatter");
-        hipLaunchKernelGGL(k_degree, dim3(grid_k), dim3(256), 0, st, uk, L.scal + 0, b, L.deg);
+        LSTC_TRY(fill_u64(L.first, 0, (uint64_t)n + 1, st));
+        LSTC_TRY(fill_u64(L.last, 0, (uint64_t)n + 1, st));
+        hipLaunchKernelGGL(k_row_bounds, dim3(grid_k), dim3(256), 0, st, uk, L.scal + 0, b, L.first, L.last);
+        LSTC_LAUNCHED("k_row_bounds");
+        hipLaunchKernelGGL(k_degree, dim3(grid_n), dim3(256), 0, st, L.first, L.last, n, L.deg);
         LSTC_LAUNCHED("k_degree");
         hipLaunchKernelGGL(k_nonzero, dim3(grid_n), dim3(256), 0, st, L.deg, n, compact, L.keepv);
         LSTC_LAUNCHED("k_nonzero");
@@ -222,11 +246,13 @@ extern "C" lstc_status lstc_clean_orient(const uint32_t *d_src, const uint32_t *
             hipLaunchKernelGGL(k_sym_cols, dim3(grid_k), dim3(256), 0, st, uk, L.scal + 0, mask, L.nid, d_sym_col);
             LSTC_LAUNCHED("k_sym_cols");
         }
-        // k_orient_flags also zeroes the flags in [m2, nk) left over from the unique pass
+        // k_orient_flags also zeroes the flags in [m2, nk] left over from the unique pass
         hipLaunchKernelGGL(k_orient_flags, dim3(grid_k), dim3(256), 0, st, uk, L.scal + 0, nk, b, mask, L.deg,
-                           L.flags, L.dout);
+                           L.flags);
         LSTC_LAUNCHED("k_orient_flags");
-        LSTC_TRY(scan_exclusive_u32_to_u64(L.flags, L.pos, nk, L.scal + 2, L.scan_ws, L.scan_bytes, st));
+        LSTC_TRY(scan_exclusive_u32_to_u64(L.flags, L.pos, nk + 1, L.scal + 2, L.scan_ws, L.scan_bytes, st));
+        hipLaunchKernelGGL(k_out_degree, dim3(grid_n), dim3(256), 0, st, L.first, L.last, L.pos, n, L.dout);
+        LSTC_LAUNCHED("k_out_degree");
         hipLaunchKernelGGL(k_orient_scatter, dim3(grid_k), dim3(256), 0, st, uk, L.scal + 0, mask, L.flags,
                            L.pos, L.nid, d_col);
         LSTC_LAUNCHED("k_orient_scatter");
@@ -247,10 +273,26 @@ extern "C" lstc_status lstc_clean_orient(const uint32_t *d_src, const uint32_t *
                            L.scal + 1, L.scal + 2, d_off);
         LSTC_LAUNCHED("k_orient_offsets");
     }
+    if (d_sizes) *d_sizes = L.scal;
+    if (d_bad) *d_bad = L.bad;
+    return LSTC_OK;
+}
+
+extern "C" lstc_status lstc_clean_orient(const uint32_t *d_src, const uint32_t *d_dst, uint64_t m_raw,
+                                         uint32_t n, int compact, uint64_t *d_sym_off,
+                                         uint32_t *d_sym_col, uint64_t *d_off, uint32_t *d_col,
+                                         uint32_t *n_out, uint64_t *m_out, void *d_ws,
+                                         size_t ws_bytes, lstc_stream s) {
+    hipStream_t st = (hipStream_t)s;
+    if (!n_out || !m_out) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: NULL size outputs");
+    const uint64_t *d_sizes = nullptr;
+    const uint32_t *d_bad = nullptr;
+    LSTC_TRY(clean_orient_launch(d_src, d_dst, m_raw, n, compact, d_sym_off, d_sym_col, d_off, d_col, d_ws,
+                                 ws_bytes, st, &d_sizes, &d_bad));
     uint64_t h[4];
     uint32_t bad = 0;
-    LSTC_HIP(hipMemcpyAsync(h, L.scal, sizeof(h), hipMemcpyDeviceToHost, st));
-    LSTC_HIP(hipMemcpyAsync(&bad, L.bad, sizeof(bad), hipMemcpyDeviceToHost, st));
+    LSTC_HIP(hipMemcpyAsync(h, d_sizes, sizeof(h), hipMemcpyDeviceToHost, st));
+    LSTC_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, st));
     LSTC_HIP(hipStreamSynchronize(st));
     if (bad) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: vertex id >= n in input");
     *n_out = (uint32_t)h[1];

@@ -1,0 +1,535 @@
+// server.hip -- continuously batched count serving with an HBM budget.
+//
+// The reference has no serving path: every graph is a separate process run
+// (README.md:57-91).  Here a server owns one device slab of `budget` bytes and a
+// worker thread that runs the loop
+//   admit   : FIFO requests whose working set fits the free slab space get
+//             memory and their preparation (seeded generation + clean/orient)
+//             enqueued on the prep stream;
+//   collect : prepared requests (prep event done) release their transient
+//             memory and wait for counting; their CSR stays resident;
+//   launch  : when the count stream is idle, every prepared request is counted
+//             by ONE persistent batched launch (count_engine.hip), results are
+//             copied to pinned host memory behind an event;
+//   finish  : when that event completes, counts and latencies are published,
+//             resident memory is freed and waiters are woken.
+// Requests that arrive while a batch is on the GPU are prepared concurrently on
+// the prep stream and join the next launch, so preparation overlaps counting
+// and the batch grows with the load (continuous batching).  Nothing on the
+// device waits on host flags: every kernel runs to completion on its own.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "count_engine.h"
+
+namespace lstc {
+
+using Clock = std::chrono::steady_clock;
+
+// First-fit sub-allocator over one device slab (offsets aligned to 256 B).
+class SlabPool {
+   public:
+    void init(char *base, size_t bytes) {
+        base_ = base;
+        free_.clear();
+        free_[0] = bytes;
+        total_ = bytes;
+        used_ = 0;
+    }
+    char *alloc(size_t bytes) {
+        bytes = (bytes + 255) & ~size_t(255);
+        if (bytes == 0) bytes = 256;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                const size_t off = it->first, rest = it->second - bytes;
+                free_.erase(it);
+                if (rest) free_[off + bytes] = rest;
+                used_ += bytes;
+                sizes_[off] = bytes;
+                return base_ + off;
+            }
+        }
+        return nullptr;
+    }
+    void release(char *p) {
+        if (!p) return;
+        const size_t off = (size_t)(p - base_);
+        auto s = sizes_.find(off);
+        if (s == sizes_.end()) return;
+        size_t bytes = s->second;
+        sizes_.erase(s);
+        used_ -= bytes;
+        size_t start = off;
+        auto next = free_.lower_bound(off);
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                start = prev->first;
+                bytes += prev->second;
+                free_.erase(prev);
+            }
+        }
+        next = free_.lower_bound(start + bytes);
+        if (next != free_.end() && next->first == start + bytes) {
+            bytes += next->second;
+            free_.erase(next);
+        }
+        free_[start] = bytes;
+    }
+    size_t total() const { return total_; }
+    size_t used() const { return used_; }
+
+   private:
+    char *base_ = nullptr;
+    std::map<size_t, size_t> free_;
+    std::unordered_map<size_t, size_t> sizes_;
+    size_t total_ = 0, used_ = 0;
+};
+
+struct Req {
+    uint64_t id;
+    lstc_request spec;
+    Clock::time_point t_submit;
+    // device memory
+    char *transient = nullptr, *resident = nullptr;
+    uint32_t *src = nullptr, *dst = nullptr;
+    uint64_t *off = nullptr;
+    uint32_t *col = nullptr;
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    uint32_t n_rows = 0;
+    uint64_t m_cap = 0;
+    hipEvent_t prep_done = nullptr;
+    const uint32_t *d_bad = nullptr;
+};
+
+struct Result {
+    lstc_status status;
+    uint64_t count;
+    uint64_t edges;
+    double latency_s;
+};
+
+// edges[g] = off[n_rows] of every graph of a batch (undirected edges after cleaning)
+__global__ void k_gather_edges(const GraphDev *__restrict__ desc, uint32_t G, unsigned long long *__restrict__ out) {
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x)
+        out[g] = desc[g].off[desc[g].n_rows];
+}
+
+static size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+
+}  // namespace lstc
+
+using namespace lstc;
+
+struct lstc_server {
+    int device = 0;
+    size_t budget = 0;
+    uint32_t max_batch = 0;
+    lstc_tune tune{};
+    bool has_tune = false;
+    char *slab = nullptr;
+    SlabPool pool;             // request working sets (generation, cleaning, resident CSRs)
+    char *reserve = nullptr;   // batch descriptors, counts and work queue (outside the pool)
+    size_t reserve_bytes = 0;
+    hipStream_t prep_stream = nullptr, count_stream = nullptr;
+    // pinned host staging for descriptors and results of the batch in flight
+    GraphDev *h_desc = nullptr;
+    uint64_t *h_base = nullptr;
+    unsigned long long *h_counts = nullptr;
+    unsigned long long *h_edges = nullptr;
+    hipEvent_t batch_done = nullptr;
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Req *> submitted;          // guarded by mu
+    std::unordered_map<uint64_t, Result> results;  // guarded by mu
+    std::vector<double> latencies;        // guarded by mu
+    uint64_t next_id = 1, n_submitted = 0, n_rejected = 0, n_completed = 0, n_batches = 0, edges_done = 0;
+    uint64_t triangles_done = 0;
+    uint64_t max_batch_seen = 0;
+    bool stop = false;
+    lstc_status fatal = LSTC_OK;
+    std::thread worker;
+
+    // worker-private state
+    std::deque<Req *> pending, preparing, ready;
+    std::vector<Req *> in_flight;
+};
+
+namespace lstc {
+
+static size_t batch_head_bytes(uint32_t G);
+
+static size_t prep_bytes(uint32_t n, uint64_t m_raw, size_t *ws_out) {
+    size_t ws = 0;
+    lstc_clean_orient_workspace(n, m_raw, &ws);
+    *ws_out = ws;
+    return round256(8 * m_raw) + ws;  // raw src/dst + clean workspace
+}
+static size_t resident_bytes(uint32_t n, uint64_t m_raw) {
+    return round256(8 * ((size_t)n + 1)) + round256(4 * (m_raw ? m_raw : 1));
+}
+static uint32_t request_rows(const lstc_request &r) {
+    if (r.kind == LSTC_REQ_RMAT) return 1u << r.scale;
+    if (r.kind == LSTC_REQ_ER) return r.n;
+    return r.n_rows;
+}
+
+// Admit the head of `pending` while memory allows; returns false on a HIP error.
+static lstc_status admit(lstc_server *S) {
+    while (!S->pending.empty()) {
+        Req *r = S->pending.front();
+        if (r->spec.kind == LSTC_REQ_CSR) {  // caller-resident CSR: nothing to prepare
+            r->off = (uint64_t *)r->spec.off;
+            r->col = (uint32_t *)r->spec.col;
+            r->n_rows = r->spec.n_rows;
+            r->m_cap = r->spec.m;
+            S->pending.pop_front();
+            S->ready.push_back(r);
+            continue;
+        }
+        const uint32_t n = request_rows(r->spec);
+        size_t ws = 0;
+        const size_t tb = prep_bytes(n, r->spec.m_raw, &ws), rb = resident_bytes(n, r->spec.m_raw);
+        char *t = S->pool.alloc(tb);
+        if (!t) break;
+        char *res = S->pool.alloc(rb);
+        if (!res) { S->pool.release(t); break; }
+        S->pending.pop_front();
+        r->transient = t;
+        r->resident = res;
+        r->src = (uint32_t *)t;
+        r->dst = (uint32_t *)(t + 4 * r->spec.m_raw);
+        r->ws = t + round256(8 * r->spec.m_raw);
+        r->ws_bytes = ws;
+        r->off = (uint64_t *)res;
+        r->col = (uint32_t *)(res + round256(8 * ((size_t)n + 1)));
+        r->n_rows = n;
+        r->m_cap = r->spec.m_raw;
+        hipStream_t st = S->prep_stream;
+        lstc_status rc;
+        if (r->spec.kind == LSTC_REQ_RMAT)
+            rc = lstc_gen_rmat(r->spec.scale, r->spec.m_raw, r->spec.a, r->spec.b, r->spec.c, r->spec.seed, r->src,
+                               r->dst, st);
+        else
+            rc = lstc_gen_er(n, r->spec.m_raw, r->spec.seed, r->src, r->dst, st);
+        if (rc != LSTC_OK) return rc;
+        // compact = 0 keeps n rows, so the count needs no host-side sizes
+        LSTC_TRY(clean_orient_launch(r->src, r->dst, r->spec.m_raw, n, 0, nullptr, nullptr, r->off, r->col, r->ws,
+                                     r->ws_bytes, st, nullptr, &r->d_bad));
+        LSTC_HIP(hipEventCreateWithFlags(&r->prep_done, hipEventDisableTiming));
+        LSTC_HIP(hipEventRecord(r->prep_done, st));
+        S->preparing.push_back(r);
+    }
+    return LSTC_OK;
+}
+
+static lstc_status collect_prepared(lstc_server *S) {
+    while (!S->preparing.empty()) {
+        Req *r = S->preparing.front();
+        hipError_t e = hipEventQuery(r->prep_done);
+        if (e == hipErrorNotReady) break;
+        if (e != hipSuccess) return hip_fail(e, "server: prep event");
+        (void)hipEventDestroy(r->prep_done);
+        r->prep_done = nullptr;
+        S->pool.release(r->transient);
+        r->transient = nullptr;
+        S->preparing.pop_front();
+        S->ready.push_back(r);
+    }
+    return LSTC_OK;
+}
+
+size_t batch_head_bytes(uint32_t G) {
+    Carver c(nullptr, 0);
+    c.take<GraphDev>(G ? G : 1);
+    c.take<uint64_t>((size_t)G + 1);
+    c.take<unsigned long long>(G ? G : 1);
+    c.take<unsigned long long>(G ? G : 1);
+    return c.need();
+}
+
+static lstc_status launch_batch(lstc_server *S) {
+    if (!S->in_flight.empty() || S->ready.empty()) return LSTC_OK;
+    // every ready request joins, as long as the work queue fits the reserve
+    std::vector<Req *> batch;
+    uint64_t rows = 0;
+    while (!S->ready.empty() && batch.size() < S->max_batch) {
+        Req *r = S->ready.front();
+        const uint32_t G1 = (uint32_t)batch.size() + 1;
+        if (!batch.empty() && batch_head_bytes(G1) + count_ws_bytes(G1, rows + r->n_rows) > S->reserve_bytes) break;
+        batch.push_back(r);
+        rows += r->n_rows;
+        S->ready.pop_front();
+    }
+    const uint32_t G = (uint32_t)batch.size();
+    std::vector<HostGraph> hosts(G);
+    uint64_t acc = 0;
+    for (uint32_t g = 0; g < G; ++g) {
+        Req *r = batch[g];
+        hosts[g] = HostGraph{r->off, r->col, nullptr, r->n_rows, r->n_rows, r->m_cap};
+        S->h_desc[g] = GraphDev{r->off, r->col, nullptr, r->n_rows, r->n_rows};
+        S->h_base[g] = acc;
+        acc += r->n_rows;
+    }
+    S->h_base[G] = acc;
+    const size_t head = batch_head_bytes(G);
+    Carver d(S->reserve, S->reserve_bytes);
+    GraphDev *d_desc = d.take<GraphDev>(G);
+    uint64_t *d_base = d.take<uint64_t>((size_t)G + 1);
+    unsigned long long *d_counts = d.take<unsigned long long>(G);
+    unsigned long long *d_edges = d.take<unsigned long long>(G);
+    hipStream_t st = S->count_stream;
+    LSTC_HIP(hipMemcpyAsync(d_desc, S->h_desc, sizeof(GraphDev) * G, hipMemcpyHostToDevice, st));
+    LSTC_HIP(hipMemcpyAsync(d_base, S->h_base, sizeof(uint64_t) * (G + 1), hipMemcpyHostToDevice, st));
+    LSTC_TRY(count_launch(hosts.data(), G, G > 1 ? d_desc : nullptr, G > 1 ? d_base : nullptr, d_counts,
+                          S->has_tune ? &S->tune : nullptr, S->reserve + head, S->reserve_bytes - head, st));
+    hipLaunchKernelGGL(k_gather_edges, dim3(1), dim3(256), 0, st, d_desc, G, d_edges);
+    LSTC_LAUNCHED("k_gather_edges");
+    LSTC_HIP(hipMemcpyAsync(S->h_counts, d_counts, sizeof(unsigned long long) * G, hipMemcpyDeviceToHost, st));
+    LSTC_HIP(hipMemcpyAsync(S->h_edges, d_edges, sizeof(unsigned long long) * G, hipMemcpyDeviceToHost, st));
+    LSTC_HIP(hipEventRecord(S->batch_done, st));
+    S->in_flight = batch;
+    return LSTC_OK;
+}
+
+static lstc_status finish_batch(lstc_server *S) {
+    if (S->in_flight.empty()) return LSTC_OK;
+    hipError_t e = hipEventQuery(S->batch_done);
+    if (e == hipErrorNotReady) return LSTC_OK;
+    if (e != hipSuccess) return hip_fail(e, "server: batch event");
+    const auto now = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (size_t g = 0; g < S->in_flight.size(); ++g) {
+            Req *r = S->in_flight[g];
+            const double lat = std::chrono::duration<double>(now - r->t_submit).count();
+            S->results[r->id] = Result{LSTC_OK, (uint64_t)S->h_counts[g], (uint64_t)S->h_edges[g], lat};
+            S->latencies.push_back(lat);
+            S->n_completed++;
+            S->edges_done += S->h_edges[g];
+            S->triangles_done += S->h_counts[g];
+        }
+        S->n_batches++;
+        S->max_batch_seen = std::max<uint64_t>(S->max_batch_seen, S->in_flight.size());
+    }
+    for (Req *r : S->in_flight) {
+        S->pool.release(r->resident);
+        delete r;
+    }
+    S->in_flight.clear();
+    S->cv_done.notify_all();
+    return LSTC_OK;
+}
+
+static void fail_all(lstc_server *S, lstc_status st) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->fatal = st;
+    auto drop = [&](Req *r) {
+        S->results[r->id] = Result{st, 0, 0, 0.0};
+        delete r;
+    };
+    for (Req *r : S->pending) drop(r);
+    for (Req *r : S->preparing) drop(r);
+    for (Req *r : S->ready) drop(r);
+    for (Req *r : S->in_flight) drop(r);
+    for (Req *r : S->submitted) drop(r);
+    S->pending.clear(); S->preparing.clear(); S->ready.clear(); S->in_flight.clear(); S->submitted.clear();
+    S->cv_done.notify_all();
+}
+
+static void worker_main(lstc_server *S) {
+    if (hipSetDevice(S->device) != hipSuccess) { fail_all(S, LSTC_ERR_HIP); return; }
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            const bool idle = S->pending.empty() && S->preparing.empty() && S->ready.empty() && S->in_flight.empty();
+            if (idle && S->submitted.empty()) {
+                if (S->stop) break;
+                S->cv_work.wait_for(lk, std::chrono::milliseconds(2));
+            }
+            while (!S->submitted.empty()) {
+                S->pending.push_back(S->submitted.front());
+                S->submitted.pop_front();
+            }
+        }
+        lstc_status rc = finish_batch(S);
+        if (rc == LSTC_OK) rc = collect_prepared(S);
+        if (rc == LSTC_OK) rc = admit(S);
+        if (rc == LSTC_OK) rc = launch_batch(S);
+        if (rc != LSTC_OK) { fail_all(S, rc); break; }
+        const bool busy = !S->in_flight.empty() || !S->preparing.empty();
+        if (busy) std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
+}
+
+}  // namespace lstc
+
+extern "C" lstc_status lstc_server_create(size_t hbm_budget_bytes, uint32_t max_batch, const lstc_tune *tune,
+                                          lstc_server **out) {
+    if (!out) return fail(LSTC_ERR_INVALID, "lstc_server_create: NULL out");
+    *out = nullptr;
+    if (hbm_budget_bytes < (1u << 20)) return fail(LSTC_ERR_INVALID, "lstc_server_create: budget below 1 MiB");
+    if (max_batch == 0) max_batch = 1024;
+    lstc_server *S = new lstc_server();
+    S->budget = hbm_budget_bytes;
+    S->max_batch = max_batch;
+    if (tune) { S->tune = *tune; S->has_tune = true; }
+    hipError_t e = hipGetDevice(&S->device);
+    if (e == hipSuccess) e = hipMalloc((void **)&S->slab, hbm_budget_bytes);
+    if (e != hipSuccess) {
+        delete S;
+        return fail(LSTC_ERR_OOM, "lstc_server_create: cannot allocate %zu bytes of HBM", hbm_budget_bytes);
+    }
+    // batch work queues live in a fixed reserve so resident CSRs can never starve the count
+    S->reserve_bytes = hmin<size_t>(hmax<size_t>(hbm_budget_bytes / 8, (size_t)16 << 20), (size_t)4 << 30);
+    S->reserve_bytes = hmin<size_t>(S->reserve_bytes, hbm_budget_bytes / 2);
+    S->reserve = S->slab + (hbm_budget_bytes - S->reserve_bytes);
+    S->pool.init(S->slab, hbm_budget_bytes - S->reserve_bytes);
+    e = hipStreamCreateWithFlags(&S->prep_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->count_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_desc, sizeof(GraphDev) * max_batch);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_base, sizeof(uint64_t) * (max_batch + 1));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_counts, sizeof(unsigned long long) * max_batch);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_edges, sizeof(unsigned long long) * max_batch);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&S->batch_done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        lstc_server_destroy(S);
+        return hip_fail(e, "lstc_server_create");
+    }
+    S->worker = std::thread(worker_main, S);
+    *out = S;
+    return LSTC_OK;
+}
+
+extern "C" lstc_status lstc_server_submit(lstc_server *S, const lstc_request *req, uint64_t *id) {
+    if (!S || !req || !id) return fail(LSTC_ERR_INVALID, "lstc_server_submit: NULL argument");
+    const lstc_request &r = *req;
+    if (r.kind != LSTC_REQ_RMAT && r.kind != LSTC_REQ_ER && r.kind != LSTC_REQ_CSR)
+        return fail(LSTC_ERR_INVALID, "lstc_server_submit: unknown request kind %u", r.kind);
+    if (r.kind == LSTC_REQ_RMAT && (r.scale == 0 || r.scale > 31))
+        return fail(LSTC_ERR_INVALID, "lstc_server_submit: RMAT scale %u not in 1..31", r.scale);
+    if (r.kind == LSTC_REQ_ER && (r.n == 0 || r.n == 0xFFFFFFFFu))
+        return fail(LSTC_ERR_INVALID, "lstc_server_submit: bad ER vertex count");
+    if (r.kind == LSTC_REQ_CSR) {
+        HostGraph h{r.off, r.col, nullptr, r.n_rows, r.n_rows, r.m};
+        LSTC_TRY(validate_graph(h, "lstc_server_submit"));
+        if (batch_head_bytes(1) + count_ws_bytes(1, r.n_rows) > S->reserve_bytes)
+            return fail(LSTC_ERR_OVER_BUDGET, "lstc_server_submit: CSR rows exceed the batch reserve");
+    } else {
+        if (r.m_raw >= 0x7FFFF000ull) return fail(LSTC_ERR_INVALID, "lstc_server_submit: too many edges");
+        // the working set of one request must fit the budget on its own
+        const uint32_t n = request_rows(r);
+        size_t ws = 0;
+        const size_t need = prep_bytes(n, r.m_raw, &ws) + resident_bytes(n, r.m_raw);
+        const size_t qneed = batch_head_bytes(1) + count_ws_bytes(1, n);
+        if (need > S->pool.total() || qneed > S->reserve_bytes) {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->n_rejected++;
+            return fail(LSTC_ERR_OVER_BUDGET, "lstc_server_submit: request needs %zu + %zu bytes, budget is %zu",
+                        need, qneed, S->budget);
+        }
+    }
+    Req *q = new Req();
+    q->spec = r;
+    q->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->fatal != LSTC_OK) { delete q; return fail(S->fatal, "lstc_server_submit: server failed"); }
+        q->id = S->next_id++;
+        S->n_submitted++;
+        S->submitted.push_back(q);
+        *id = q->id;
+    }
+    S->cv_work.notify_one();
+    return LSTC_OK;
+}
+
+static lstc_status take_result(lstc_server *S, std::unordered_map<uint64_t, Result>::iterator it, lstc_result *out) {
+    if (out) {
+        out->count = it->second.count;
+        out->edges = it->second.edges;
+        out->latency_s = it->second.latency_s;
+    }
+    const lstc_status st = it->second.status;
+    S->results.erase(it);
+    return st;
+}
+
+extern "C" lstc_status lstc_server_poll(lstc_server *S, uint64_t id, lstc_result *out) {
+    if (!S) return fail(LSTC_ERR_INVALID, "lstc_server_poll: NULL server");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->results.find(id);
+    if (it == S->results.end()) return LSTC_ERR_NOT_READY;
+    return take_result(S, it, out);
+}
+
+extern "C" lstc_status lstc_server_wait(lstc_server *S, uint64_t id, double timeout_s, lstc_result *out) {
+    if (!S) return fail(LSTC_ERR_INVALID, "lstc_server_wait: NULL server");
+    std::unique_lock<std::mutex> lk(S->mu);
+    const auto deadline = Clock::now() + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(timeout_s));
+    for (;;) {
+        auto it = S->results.find(id);
+        if (it != S->results.end()) return take_result(S, it, out);
+        if (S->cv_done.wait_until(lk, deadline) == std::cv_status::timeout && Clock::now() >= deadline)
+            return fail(LSTC_ERR_NOT_READY, "lstc_server_wait: timed out");
+    }
+}
+
+extern "C" lstc_status lstc_server_get_stats(lstc_server *S, lstc_server_stats *out) {
+    if (!S || !out) return fail(LSTC_ERR_INVALID, "lstc_server_get_stats: NULL argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    out->submitted = S->n_submitted;
+    out->completed = S->n_completed;
+    out->rejected = S->n_rejected;
+    out->batches = S->n_batches;
+    out->max_batch_graphs = S->max_batch_seen;
+    out->pool_bytes = S->pool.total();
+    out->pool_used_bytes = S->pool.used();
+    out->edges = S->edges_done;
+    out->triangles = S->triangles_done;
+    std::vector<double> l = S->latencies;
+    std::sort(l.begin(), l.end());
+    auto pct = [&](double p) {
+        if (l.empty()) return 0.0;
+        size_t i = (size_t)(p * (double)(l.size() - 1) + 0.5);
+        return l[std::min(i, l.size() - 1)];
+    };
+    out->p50_s = pct(0.50);
+    out->p99_s = pct(0.99);
+    out->max_s = l.empty() ? 0.0 : l.back();
+    double s = 0;
+    for (double x : l) s += x;
+    out->mean_s = l.empty() ? 0.0 : s / (double)l.size();
+    return LSTC_OK;
+}
+
+extern "C" lstc_status lstc_server_destroy(lstc_server *S) {
+    if (!S) return LSTC_OK;
+    if (S->worker.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->stop = true;
+        }
+        S->cv_work.notify_all();
+        S->worker.join();
+    }
+    if (S->prep_stream) { (void)hipStreamSynchronize(S->prep_stream); (void)hipStreamDestroy(S->prep_stream); }
+    if (S->count_stream) { (void)hipStreamSynchronize(S->count_stream); (void)hipStreamDestroy(S->count_stream); }
+    if (S->batch_done) (void)hipEventDestroy(S->batch_done);
+    if (S->h_desc) (void)hipHostFree(S->h_desc);
+    if (S->h_base) (void)hipHostFree(S->h_base);
+    if (S->h_counts) (void)hipHostFree(S->h_counts);
+    if (S->h_edges) (void)hipHostFree(S->h_edges);
+    if (S->slab) (void)hipFree(S->slab);
+    delete S;
+    return LSTC_OK;
+}
