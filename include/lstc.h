@@ -72,14 +72,18 @@ lstc_status lstc_partition_random(uint32_t n, uint32_t k, uint64_t seed, uint32_
  * Convert).  Drops self loops and repeats, symmetrises, optionally (compact=1)
  * removes isolated vertices with an order-preserving relabel, and orients every
  * edge s->d iff deg(s) < deg(d) or (deg equal and d < s).
- * Outputs: optional symmetric CSR (d_sym_off[n+1], d_sym_col[2*m_raw]) and the
- * oriented CSR (d_off[n+1], d_col[m_raw]); *n_out / *m_out receive the vertex
- * and undirected edge counts (the stream is synchronised).                    */
+ * Outputs: optional symmetric CSR (d_sym_off[n+1], d_sym_col[2*m_raw]), the
+ * oriented CSR (d_off[n+1], d_col[m_raw]) and optional "target-side in-lists"
+ * (d_in_off[n+1], d_in_col[m_raw]): for every w, the sources u of arcs u->w with
+ * d+(u) <= d+(w), ascending.  Those are the arcs lstc_count_split counts from
+ * w's side.  *n_out / *m_out receive the vertex and undirected edge counts (the
+ * stream is synchronised).                                                    */
 lstc_status lstc_clean_orient_workspace(uint32_t n, uint64_t m_raw, size_t *bytes);
 lstc_status lstc_clean_orient(const uint32_t *d_src, const uint32_t *d_dst, uint64_t m_raw,
                               uint32_t n, int compact, uint64_t *d_sym_off, uint32_t *d_sym_col,
-                              uint64_t *d_off, uint32_t *d_col, uint32_t *n_out, uint64_t *m_out,
-                              void *d_ws, size_t ws_bytes, lstc_stream s);
+                              uint64_t *d_off, uint32_t *d_col, uint64_t *d_in_off, uint32_t *d_in_col,
+                              uint32_t *n_out, uint64_t *m_out, void *d_ws, size_t ws_bytes,
+                              lstc_stream s);
 
 /* ------------------------------------------------------------------ (b) halo subgraphs
  * Replaces create_subgraph_with_halo (preprocess_GPU/Halo_cleanned_preprocess/
@@ -104,13 +108,13 @@ lstc_status lstc_halo_extract(const uint64_t *d_sym_off, const uint32_t *d_sym_c
 /* ------------------------------------------------------------------ (c) counting
  * Tuning knobs (picked by lstc_autotune; zero-initialised = defaults).       */
 typedef struct lstc_tune {
-    uint32_t table_scale;  /* LDS hash capacity >= table_scale * keys (2..8), 0 = 4     */
+    uint32_t table_scale;  /* LDS hash capacity >= table_scale * keys (2..8), 0 = 8     */
     uint32_t small_max;    /* switch point: rows with d+ <= small_max get a wave-private
                               1024-slot table, larger rows a workgroup-shared 4096-slot
-                              one (64..1024/table_scale), 0 = 256                       */
+                              one (64..1024/table_scale), 0 = 256 (clamped)             */
     uint32_t small_group;  /* small rows per work ticket (4..256), 0 = 16              */
     uint32_t grid_per_cu;  /* persistent workgroups per CU (1..16), 0 = 7              */
-    uint32_t unroll;       /* 64-entry chunks in flight per wave (4 or 8), 0 = 8       */
+    uint32_t unroll;       /* 64-entry chunks in flight per wave (4 or 8), 0 = 4       */
     uint32_t reserved[3];
 } lstc_tune;
 
@@ -129,6 +133,14 @@ lstc_status lstc_count(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_
                        uint32_t n_owned, uint64_t m, uint64_t *d_count, uint64_t *d_per_row,
                        const lstc_tune *tune, void *d_ws, size_t ws_bytes, lstc_stream s);
 
+/* Same count of a whole graph (all rows owned) that also has the target-side
+ * in-lists from lstc_clean_orient: every arc (u, w) streams the shorter of
+ * N+(u) and N+(w) through an LDS table of the other, about halving the entries
+ * streamed on power-law graphs.  Workspace: lstc_count_workspace(n, n, m).   */
+lstc_status lstc_count_split(const uint64_t *d_off, const uint32_t *d_col, const uint64_t *d_in_off,
+                             const uint32_t *d_in_col, uint32_t n_rows, uint64_t m, uint64_t *d_count,
+                             const lstc_tune *tune, void *d_ws, size_t ws_bytes, lstc_stream s);
+
 /* ------------------------------------------------------------------ batching
  * Many graphs (e.g. the k halo parts of a partition, or unrelated graphs of
  * uneven size) counted by ONE persistent launch over one degree-binned ticket
@@ -136,13 +148,15 @@ lstc_status lstc_count(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_
  * reference's per-part loop of launch + synchronise + reduce + copy
  * (src/WithPartition/without_sorting_tc.cu:244-465).                          */
 typedef struct lstc_graph_desc {
-    const uint64_t *off;  /* device, n_rows + 1 */
-    const uint32_t *col;  /* device, m          */
-    uint64_t *per_row;    /* device, n_owned, or NULL */
+    const uint64_t *off;     /* device, n_rows + 1 */
+    const uint32_t *col;     /* device, m          */
+    uint64_t *per_row;       /* device, n_owned, or NULL */
+    const uint64_t *in_off;  /* target-side in-lists (split count) or NULL; used   */
+    const uint32_t *in_col;  /* only when n_owned == n_rows and per_row is NULL    */
     uint64_t m;
     uint32_t n_rows;
     uint32_t n_owned;
-} lstc_graph_desc;
+} lstc_graph_desc;          /* at most 2047 graphs per launch */
 
 lstc_status lstc_count_batch_workspace(const lstc_graph_desc *graphs, uint32_t G, const lstc_tune *tune,
                                        size_t *bytes);
@@ -159,6 +173,20 @@ lstc_status lstc_batch_plan_create(const lstc_graph_desc *graphs, uint32_t G, co
 lstc_status lstc_batch_plan_run(lstc_batch_plan *plan, uint64_t *d_counts, lstc_stream s);
 lstc_status lstc_batch_plan_launch_graph(lstc_batch_plan *plan, uint64_t *d_counts, lstc_stream s);
 lstc_status lstc_batch_plan_destroy(lstc_batch_plan *plan);
+
+/* ------------------------------------------------------------------ autotuning
+ * Replaces the reference's compile-time N_THREADS_PER_BLOCK / SHARED_MEM sweep
+ * (src/WithoutPartition/ShareKernelTCV62_update_v1.cu:8-9, src/profiling/occup_*.txt).
+ * lstc_autotune times candidate lstc_tune values on the caller's batch
+ * (coordinate descent, hipEvents, `reps` launches each) and returns the fastest;
+ * every candidate must reproduce the default counts exactly.  It allocates its
+ * own scratch and synchronises the stream.  The cache is a TSV of
+ * (device, shape class, knobs, ms); the last matching line wins.              */
+lstc_status lstc_shape_class(const lstc_graph_desc *graphs, uint32_t G, uint32_t *shape_class);
+lstc_status lstc_autotune(const lstc_graph_desc *graphs, uint32_t G, uint32_t reps, lstc_tune *best,
+                          double *best_ms, lstc_stream s);
+lstc_status lstc_tune_cache_lookup(const char *path, uint32_t shape_class, lstc_tune *out);
+lstc_status lstc_tune_cache_store(const char *path, uint32_t shape_class, const lstc_tune *tune, double ms);
 
 /* ------------------------------------------------------------------ serving
  * No reference counterpart (the reference runs one process per graph).  A server

@@ -4,6 +4,7 @@ Every GPU result must equal the oracle's uint64 exactly.
 """
 import numpy as np
 import pytest
+import torch
 
 import graphs as G
 
@@ -85,3 +86,28 @@ def test_invalid_arguments_raise(lstc, oracle):
     dg = _upload(lstc, g)
     with pytest.raises(lstc.LstcError):
         lstc.count(dg, n_owned=dg.n + 1)
+
+
+@pytest.mark.parametrize("name,builder", [
+    ("K3", lambda O: G.complete(3)),
+    ("K1200", lambda O: G.complete(1200)),
+    ("K3000_tiled", lambda O: G.complete(3000)),
+    ("wheel", lambda O: G.wheel(300)),
+    ("star", lambda O: G.star(9000)),
+    ("chung_lu", lambda O: O.gen_chung_lu(60000, 2.1, 50, 2e4, seed=4)),
+    ("rmat_s15", lambda O: O.gen_rmat(15, 16, seed=8)),
+])
+def test_split_count_matches_oracle(lstc, oracle, name, builder):
+    n, s, d = builder(oracle)
+    ds = torch.from_numpy(np.ascontiguousarray(s, np.uint32).view(np.int32)).cuda()
+    dd = torch.from_numpy(np.ascontiguousarray(d, np.uint32).view(np.int32)).cuda()
+    g = lstc.clean_orient(n, ds, dd, in_lists=True)
+    _, og = oracle.clean_orient(n, s, d)
+    # complete graphs: closed form n(n-1)(n-2)/6 (the merge oracle is slow at K3000)
+    want = n * (n - 1) * (n - 2) // 6 if name.startswith("K") else oracle.count(og)
+    assert lstc.count(g) == want, name
+    for t in (lstc.Tune(table_scale=2), lstc.Tune(table_scale=4, small_max=64), lstc.Tune(unroll=8)):
+        assert lstc.count(g, tune=t) == want, (name, t.as_dict())
+    # a batch mixing split and plain graphs
+    plain = lstc.Graph.from_numpy(og.off, og.col)
+    assert lstc.count_batch([g, plain, g]) == [want, want, want]

@@ -42,6 +42,7 @@ class Tune(C.Structure):
 class GraphDesc(C.Structure):
     """lstc_graph_desc (include/lstc.h)."""
     _fields_ = [("off", C.c_void_p), ("col", C.c_void_p), ("per_row", C.c_void_p),
+                ("in_off", C.c_void_p), ("in_col", C.c_void_p),
                 ("m", C.c_uint64), ("n_rows", C.c_uint32), ("n_owned", C.c_uint32)]
 
 
@@ -86,18 +87,23 @@ SIGNATURES = {
     "lstc_gen_grid": (ST, [U32, U32, DBL, U64, VP, VP, VP]),
     "lstc_partition_random": (ST, [U32, U32, U64, VP, VP]),
     "lstc_clean_orient_workspace": (ST, [U32, U64, PSZ]),
-    "lstc_clean_orient": (ST, [VP, VP, U64, U32, I32, VP, VP, VP, VP, PU32, PU64, VP, SZ, VP]),
+    "lstc_clean_orient": (ST, [VP, VP, U64, U32, I32, VP, VP, VP, VP, VP, VP, PU32, PU64, VP, SZ, VP]),
     "lstc_halo_workspace": (ST, [U32, PSZ]),
     "lstc_halo_sizes": (ST, [VP, VP, U32, VP, U32, I32, PU32, PU32, PU64, VP, SZ, VP]),
     "lstc_halo_extract": (ST, [VP, VP, U32, VP, U32, I32, VP, VP, VP, U32, U64, VP, SZ, VP]),
     "lstc_count_workspace": (ST, [U32, U32, U64, C.POINTER(Tune), PSZ]),
     "lstc_count": (ST, [VP, VP, U32, U32, U64, VP, VP, C.POINTER(Tune), VP, SZ, VP]),
+    "lstc_count_split": (ST, [VP, VP, VP, VP, U32, U64, VP, C.POINTER(Tune), VP, SZ, VP]),
     "lstc_count_batch_workspace": (ST, [C.POINTER(GraphDesc), U32, C.POINTER(Tune), PSZ]),
     "lstc_count_batch": (ST, [C.POINTER(GraphDesc), U32, VP, C.POINTER(Tune), VP, SZ, VP]),
     "lstc_batch_plan_create": (ST, [C.POINTER(GraphDesc), U32, C.POINTER(Tune), C.POINTER(VP)]),
     "lstc_batch_plan_run": (ST, [VP, VP, VP]),
     "lstc_batch_plan_launch_graph": (ST, [VP, VP, VP]),
     "lstc_batch_plan_destroy": (ST, [VP]),
+    "lstc_shape_class": (ST, [C.POINTER(GraphDesc), U32, PU32]),
+    "lstc_autotune": (ST, [C.POINTER(GraphDesc), U32, U32, C.POINTER(Tune), C.POINTER(C.c_double), VP]),
+    "lstc_tune_cache_lookup": (ST, [C.c_char_p, U32, C.POINTER(Tune)]),
+    "lstc_tune_cache_store": (ST, [C.c_char_p, U32, C.POINTER(Tune), DBL]),
     "lstc_server_create": (ST, [SZ, U32, C.POINTER(Tune), C.POINTER(VP)]),
     "lstc_server_submit": (ST, [VP, C.POINTER(Request), PU64]),
     "lstc_server_poll": (ST, [VP, U64, C.POINTER(Result)]),

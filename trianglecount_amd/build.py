@@ -64,7 +64,32 @@ def build(jobs: int | None = None) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     shutil.move(tmp, LIB)
+    build_cli(jobs)
     return LIB
+
+
+CLI = os.path.join(PKG, "cli")
+BIN = os.path.join(PKG, "_bin")
+
+
+def _build_tool(src: str) -> str:
+    name = os.path.splitext(os.path.basename(src))[0]
+    out = os.path.join(BIN, name)
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", f"-I{INCLUDE}", f"-I{CLI}", src, "-o", out + ".tmp",
+           f"-L{LIBDIR}", "-llstc", "-Wl,-rpath,$ORIGIN/../_lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
+    shutil.move(out + ".tmp", out)
+    return out
+
+
+def build_cli(jobs: int | None = None) -> list:
+    """Host tools mirroring the reference executables (lstc_tc, lstc_part_tc, lstc_halo, lstc_preprocess)."""
+    os.makedirs(BIN, exist_ok=True)
+    srcs = sorted(os.path.join(CLI, f) for f in os.listdir(CLI) if f.endswith(".cpp"))
+    with cf.ThreadPoolExecutor(jobs or min(len(srcs), os.cpu_count() or 4)) as ex:
+        return list(ex.map(_build_tool, srcs))
 
 
 if __name__ == "__main__":

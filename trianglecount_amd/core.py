@@ -81,6 +81,8 @@ class Graph:
     sym_col: torch.Tensor | None = None
     n_owned: int | None = None
     meta: dict = field(default_factory=dict)
+    in_off: torch.Tensor | None = None  # target-side in-lists (split count), optional
+    in_col: torch.Tensor | None = None
 
     @property
     def owned(self) -> int:
@@ -168,8 +170,12 @@ def partition_random(n: int, k: int, seed: int = 1, stream=None) -> torch.Tensor
 
 # ------------------------------------------------------------------ (a) clean + orient
 def clean_orient(n: int, src: torch.Tensor, dst: torch.Tensor, compact: bool = True,
-                 keep_symmetric: bool = False, ws: Workspace | None = None, stream=None) -> Graph:
-    """Drop loops/repeats, symmetrise, optionally drop isolated vertices, orient by (degree, id)."""
+                 keep_symmetric: bool = False, in_lists: bool = False, ws: Workspace | None = None,
+                 stream=None) -> Graph:
+    """Drop loops/repeats, symmetrise, optionally drop isolated vertices, orient by (degree, id).
+
+    ``in_lists`` also emits the target-side in-lists used by the split count.
+    """
     L = load()
     m_raw = int(src.numel())
     if int(dst.numel()) != m_raw:
@@ -181,21 +187,28 @@ def clean_orient(n: int, src: torch.Tensor, dst: torch.Tensor, compact: bool = T
     col = empty_u32(m_raw)
     sym_off = empty_u64(n + 1) if keep_symmetric else None
     sym_col = empty_u32(2 * m_raw) if keep_symmetric else None
+    in_off = empty_u64(n + 1) if in_lists else None
+    in_col = empty_u32(m_raw) if in_lists else None
     n_out, m_out = C.c_uint32(), C.c_uint64()
     check(L.lstc_clean_orient(_ptr(src), _ptr(dst), m_raw, n, int(bool(compact)), _ptr(sym_off),
-                              _ptr(sym_col), _ptr(off), _ptr(col), C.byref(n_out), C.byref(m_out),
-                              wp, wb, C.c_void_p(stream_handle(stream))), "lstc_clean_orient")
+                              _ptr(sym_col), _ptr(off), _ptr(col), _ptr(in_off), _ptr(in_col),
+                              C.byref(n_out), C.byref(m_out), wp, wb, C.c_void_p(stream_handle(stream))),
+          "lstc_clean_orient")
     g = Graph(n_out.value, m_out.value, off, col, sym_off, sym_col)
+    if in_lists:
+        g.in_off, g.in_col = in_off, in_col
     return g
 
 
 # ------------------------------------------------------------------ (c) count
 def count(g: Graph, n_owned: int | None = None, per_row: bool = False, tune: Tune | None = None,
           ws: Workspace | None = None, stream=None, out: torch.Tensor | None = None,
-          sync: bool = True):
+          sync: bool = True, split: bool = True):
     """Exact count over owned rows; returns int (and the per-row tensor if asked).
 
-    With ``sync=False`` the device result tensor is returned without waiting.
+    A whole graph carrying in-lists is counted with lstc_count_split unless
+    ``split`` is False.  With ``sync=False`` the device result tensor is
+    returned without waiting.
     """
     L = load()
     no = g.owned if n_owned is None else int(n_owned)
@@ -205,8 +218,12 @@ def count(g: Graph, n_owned: int | None = None, per_row: bool = False, tune: Tun
     wp, wb = (ws or _default_ws).get(nb.value)
     res = out if out is not None else empty_u64(1)
     pr = empty_u64(no) if per_row else None
-    check(L.lstc_count(_ptr(g.off), _ptr(g.col), g.n, no, g.m, _ptr(res), _ptr(pr), tp, wp, wb,
-                       C.c_void_p(stream_handle(stream))), "lstc_count")
+    if g.in_off is not None and not per_row and no == g.n and split:
+        check(L.lstc_count_split(_ptr(g.off), _ptr(g.col), _ptr(g.in_off), _ptr(g.in_col), g.n, g.m,
+                                 _ptr(res), tp, wp, wb, C.c_void_p(stream_handle(stream))), "lstc_count_split")
+    else:
+        check(L.lstc_count(_ptr(g.off), _ptr(g.col), g.n, no, g.m, _ptr(res), _ptr(pr), tp, wp, wb,
+                           C.c_void_p(stream_handle(stream))), "lstc_count")
     if not sync:
         return (res, pr) if per_row else res
     total = int(res[0].item()) & 0xFFFFFFFFFFFFFFFF
@@ -253,6 +270,9 @@ def _descs(graphs, per_row=None):
         arr[i].off = g.off.data_ptr()
         arr[i].col = g.col.data_ptr()
         arr[i].per_row = per_row[i].data_ptr() if per_row is not None else None
+        if g.in_off is not None:
+            arr[i].in_off = g.in_off.data_ptr()
+            arr[i].in_col = g.in_col.data_ptr()
         arr[i].m = g.m
         arr[i].n_rows = g.n
         arr[i].n_owned = g.owned
@@ -385,3 +405,29 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------ autotuning
+def shape_class(graphs: list) -> int:
+    sc = C.c_uint32()
+    check(load().lstc_shape_class(_descs(graphs), len(graphs), C.byref(sc)), "lstc_shape_class")
+    return sc.value
+
+
+def autotune(graphs: list, reps: int = 3, cache: str | None = None, stream=None):
+    """Pick lstc_tune for this batch (cached per device and shape class when ``cache`` is a path).
+
+    Returns (tune, ms_per_launch or None when served from the cache, from_cache).
+    """
+    L = load()
+    sc = shape_class(graphs)
+    t = Tune()
+    if cache is not None:
+        if L.lstc_tune_cache_lookup(cache.encode(), sc, C.byref(t)) == 0:
+            return t, None, True
+    ms = C.c_double()
+    check(L.lstc_autotune(_descs(graphs), len(graphs), reps, C.byref(t), C.byref(ms),
+                          C.c_void_p(stream_handle(stream))), "lstc_autotune")
+    if cache is not None:
+        check(L.lstc_tune_cache_store(cache.encode(), sc, C.byref(t), ms.value), "lstc_tune_cache_store")
+    return t, ms.value, False

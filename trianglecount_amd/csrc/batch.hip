@@ -21,12 +21,13 @@ static lstc_status to_hosts(const lstc_graph_desc *graphs, uint32_t G, std::vect
     hosts.resize(G);
     devs.resize(G);
     base.resize((size_t)G + 1);
+    if (G > kMaxGraphs) return fail(LSTC_ERR_INVALID, "batch: at most %u graphs per launch", kMaxGraphs);
     uint64_t acc = 0;
     for (uint32_t g = 0; g < G; ++g) {
         const lstc_graph_desc &d = graphs[g];
-        hosts[g] = HostGraph{d.off, d.col, d.per_row, d.n_rows, d.n_owned, d.m};
+        hosts[g] = HostGraph{d.off, d.col, d.per_row, d.n_rows, d.n_owned, d.m, d.in_off, d.in_col};
         LSTC_TRY(validate_graph(hosts[g], "batch"));
-        devs[g] = GraphDev{d.off, d.col, (unsigned long long *)d.per_row, d.n_rows, d.n_owned};
+        devs[g] = to_dev(hosts[g]);
         base[g] = acc;
         acc += d.n_owned;
     }
@@ -37,6 +38,12 @@ static lstc_status to_hosts(const lstc_graph_desc *graphs, uint32_t G, std::vect
 static uint64_t owned_total(const lstc_graph_desc *graphs, uint32_t G) {
     uint64_t t = 0;
     for (uint32_t g = 0; g < G; ++g) t += graphs[g].n_owned;
+    return t;
+}
+
+static uint64_t arcs_total(const lstc_graph_desc *graphs, uint32_t G) {
+    uint64_t t = 0;
+    for (uint32_t g = 0; g < G; ++g) t += graphs[g].m;
     return t;
 }
 
@@ -73,7 +80,7 @@ extern "C" lstc_status lstc_count_batch_workspace(const lstc_graph_desc *graphs,
     (void)tune;
     if (!bytes) return fail(LSTC_ERR_INVALID, "lstc_count_batch_workspace: NULL bytes");
     if (G && !graphs) return fail(LSTC_ERR_INVALID, "lstc_count_batch_workspace: NULL graph array");
-    *bytes = table_bytes(G) + count_ws_bytes(G, owned_total(graphs, G));
+    *bytes = table_bytes(G) + count_ws_bytes(G, owned_total(graphs, G), arcs_total(graphs, G));
     return LSTC_OK;
 }
 
@@ -117,7 +124,7 @@ extern "C" lstc_status lstc_batch_plan_create(const lstc_graph_desc *graphs, uin
     p->G = G;
     if (tune) { p->tune = *tune; p->has_tune = true; }
     const size_t tb = table_bytes(G);
-    p->ws_bytes = count_ws_bytes(G, base[G]);
+    p->ws_bytes = count_ws_bytes(G, base[G], arcs_total(graphs, G));
     p->dbytes = tb + p->ws_bytes;
     hipError_t e = hipMalloc(&p->dmem, p->dbytes);
     if (e != hipSuccess) { delete p; return fail(LSTC_ERR_OOM, "lstc_batch_plan_create: hipMalloc(%zu) failed", p->dbytes); }

@@ -1,4 +1,4 @@
-// count.hip -- lstc_count: exact triangle count of one oriented CSR.
+// count.hip -- lstc_count / lstc_count_split: exact triangle count of one oriented CSR.
 //
 // Semantics follow Find_Triangle + cub::DeviceReduce::Sum of
 // src/WithoutPartition/ShareKernelTCV62_update_v1.cu:43-191,264,271-277 and the
@@ -11,9 +11,9 @@ using namespace lstc;
 
 extern "C" lstc_status lstc_count_workspace(uint32_t n_rows, uint32_t n_owned, uint64_t m,
                                             const lstc_tune *tune, size_t *bytes) {
-    (void)n_rows; (void)m; (void)tune;
+    (void)n_rows; (void)tune;
     if (!bytes) return fail(LSTC_ERR_INVALID, "lstc_count_workspace: bytes is NULL");
-    *bytes = count_ws_bytes(1, n_owned);
+    *bytes = count_ws_bytes(1, n_owned, m);
     return LSTC_OK;
 }
 
@@ -23,6 +23,17 @@ extern "C" lstc_status lstc_count(const uint64_t *d_off, const uint32_t *d_col, 
     if (!d_count) return fail(LSTC_ERR_INVALID, "lstc_count: NULL output");
     HostGraph h{d_off, d_col, d_per_row, n_rows, n_owned, m};
     LSTC_TRY(validate_graph(h, "lstc_count"));
+    return count_launch(&h, 1, nullptr, nullptr, (unsigned long long *)d_count, tune, d_ws, ws_bytes,
+                        (hipStream_t)s);
+}
+
+extern "C" lstc_status lstc_count_split(const uint64_t *d_off, const uint32_t *d_col, const uint64_t *d_in_off,
+                                        const uint32_t *d_in_col, uint32_t n_rows, uint64_t m, uint64_t *d_count,
+                                        const lstc_tune *tune, void *d_ws, size_t ws_bytes, lstc_stream s) {
+    if (!d_count) return fail(LSTC_ERR_INVALID, "lstc_count_split: NULL output");
+    if (!d_in_off || !d_in_col) return fail(LSTC_ERR_INVALID, "lstc_count_split: NULL in-lists");
+    HostGraph h{d_off, d_col, nullptr, n_rows, n_rows, m, d_in_off, d_in_col};
+    LSTC_TRY(validate_graph(h, "lstc_count_split"));
     return count_launch(&h, 1, nullptr, nullptr, (unsigned long long *)d_count, tune, d_ws, ws_bytes,
                         (hipStream_t)s);
 }

@@ -10,6 +10,8 @@ struct GraphDev {
     const uint64_t *off;
     const uint32_t *col;
     unsigned long long *per_row;  // may be null
+    const uint64_t *in_off;       // target-side in-lists (split count), may be null
+    const uint32_t *in_col;
     uint32_t n_rows;
     uint32_t n_owned;
 };
@@ -31,7 +33,19 @@ struct HostGraph {
     uint64_t *per_row;
     uint32_t n_rows, n_owned;
     uint64_t m;
+    const uint64_t *in_off = nullptr;  // optional target-side in-lists
+    const uint32_t *in_col = nullptr;
 };
+
+// Split counting (in-lists present) applies only to whole graphs without per-row output.
+inline bool split_ok(const HostGraph &h) {
+    return h.in_off && h.in_col && !h.per_row && h.n_owned == h.n_rows;
+}
+inline GraphDev to_dev(const HostGraph &h) {
+    const bool sp = split_ok(h);
+    return GraphDev{h.off, h.col, (unsigned long long *)h.per_row, sp ? h.in_off : nullptr, sp ? h.in_col : nullptr,
+                    h.n_rows, h.n_owned};
+}
 
 // Resolved tuning (defaults filled in, ranges clamped).
 struct TuneR {
@@ -45,9 +59,11 @@ TuneR resolve_tune(const lstc_tune *t);
 
 lstc_status validate_graph(const HostGraph &h, const char *who);
 
-// Workspace for a launch over G graphs with total_rows owned rows (excludes the
-// batch descriptor table, which the caller places).
-size_t count_ws_bytes(uint32_t G, uint64_t total_rows);
+// Workspace for a launch over G graphs with total_rows owned rows and total_m
+// arcs (excludes the batch descriptor table, which the caller places).
+size_t count_ws_bytes(uint32_t G, uint64_t total_rows, uint64_t total_m);
+// Largest batch one launch can address (graph ids are packed into work items).
+constexpr uint32_t kMaxGraphs = 2047;
 
 // hosts[G] describe the graphs (used for sizes, per-row zeroing and, when G == 1,
 // as the inline descriptor).  For G > 1, d_table[G] and d_row_base[G+1] must

@@ -4,26 +4,37 @@
 // (src/WithoutPartition/ShareKernelTCV62_update_v1.cu:43-191 summed as at
 // :271-277; owned rows only as in src/WithPartition/without_sorting_tc.cu:400).
 //
-// Plan (k_plan_rows): owned rows with d+(u) >= 2 of every graph of the launch
-// are binned by d+ into one queue, largest bin first.  Rows above small_max are
-// "big" tickets (one row each); smaller rows form "small" tickets of
-// small_group rows.
+// Every arc (u, w) contributes |N+(u) n N+(w)|, found by hashing one list into
+// LDS and streaming the other through it.  A work item hashes one row r:
+//   kind A (source side): N+(r) hashed; the lists N+(w), w in N+(r), streamed.
+//   kind B (target side): N+(r) hashed; the lists N+(u) of in-neighbours u
+//            (from the graph's in-lists) streamed.
+// With in-lists present ("split" counting) kind A keeps only arcs with
+// d+(w) < d+(r) and kind B covers the rest (the in-lists hold exactly the arcs
+// u -> w with d+(u) <= d+(w)), so each arc streams its shorter list.  Without
+// in-lists every arc is a kind A arc.
+//
+// Plan (k_plan_rows): owned rows with d+ >= 2 of every graph of the launch emit
+// one kind A item and ceil(in-degree / kSegChunk) kind B items, binned by the
+// hashed row's d+ into one queue, largest bin first.  Items of rows above
+// small_max are "big" tickets (one item each); smaller ones form "small"
+// tickets of small_group items.
 //
 // Count (k_count, one persistent launch): each workgroup of kWaves waves takes
 // tickets from an atomic head.
-//   big row   : the whole workgroup builds one LDS hash table of N+(u) (up to
-//               4096 slots, load <= 1/2; rows above 2048 keys are tiled) and the
-//               waves split the 64-segment windows of N+(u) round-robin;
-//   small row : one wave, a private 1024-slot slice of the same LDS.
-// Walk of one window: one segment per lane (w, start, length), inclusive scan
+//   big item  : the whole workgroup builds one LDS hash table (up to 4096 slots,
+//               load <= 1/2; rows above 2048 keys are tiled) and the waves split
+//               the 64-segment windows round-robin;
+//   small item: one wave, a private 1024-slot slice of the same LDS.
+// Walk of one window: one segment per lane (id, start, length), inclusive scan
 // of lengths, empty segments compacted out; then the flat sequence of entries
 // is consumed 64 per chunk.  A chunk inside one segment takes its base from the
 // current segment (scalar); otherwise each lane finds its segment from a
 // boundary mask (LDS flags + ballot + popcount).  UNROLL chunks are addressed
-// and loaded before the first probe, then all of them are probed in lockstep,
-// so each wave keeps UNROLL global loads in flight.  Hits are counted as
-// popcount(ballot(hit)) into a scalar; the workgroup issues one uint64 atomic
-// per graph it counted.
+// and loaded (global loads, never FLAT) before the first probe, then probed in
+// lockstep, so each wave keeps UNROLL loads in flight.  Hits are counted per
+// lane and reduced across the wave per item; the workgroup issues one uint64
+// atomic per graph it counted.
 #include "count_engine.h"
 #include "wave.h"
 
@@ -32,17 +43,32 @@ namespace lstc {
 static constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 static constexpr int kWaves = 4;
 static constexpr uint32_t kSliceLog2 = 10;
-static constexpr uint32_t kSlice = 1u << kSliceLog2;  // wave-private slice (small rows)
+static constexpr uint32_t kSlice = 1u << kSliceLog2;  // wave-private slice (small items)
 static constexpr uint32_t kBigLog2 = 12;
-static constexpr uint32_t kBigCap = 1u << kBigLog2;   // shared table (big rows)
+static constexpr uint32_t kBigCap = 1u << kBigLog2;   // shared table (big items)
 static constexpr uint32_t kBigTile = kBigCap / 2;     // keys per table fill of a big row
+static constexpr uint32_t kSegChunk = 4096;           // in-neighbour segments per kind B item
 static constexpr int kBins = 6;
 
 // ctl (uint32) layout
 enum { C_CNT = 0, C_CUR = 6, C_HEAD = 12, C_NBIG = 13, C_NSMALL = 14, C_NTICK = 15, C_WORDS = 16 };
 
+// work item: row (bits 0-31) | chunk (32-51) | kind (52) | graph (53-63).
+// A row's in-degree is below 2^32 (m < 2^32), so its chunk index fits 20 bits.
+__device__ __forceinline__ unsigned long long make_item(uint32_t g, uint32_t kind, uint32_t chunk, uint32_t r) {
+    return ((unsigned long long)g << 53) | ((unsigned long long)kind << 52) | ((unsigned long long)chunk << 32) | r;
+}
+struct Item {
+    uint32_t row, chunk, kind, g;
+};
+__device__ __forceinline__ Item decode(unsigned long long it) {
+    return Item{(uint32_t)it, (uint32_t)(it >> 32) & 0xFFFFFu, (uint32_t)(it >> 52) & 1u, (uint32_t)(it >> 53)};
+}
+
 TuneR resolve_tune(const lstc_tune *t) {
-    TuneR r{4, 256, 16, 0, 8};
+    // defaults: the fastest point of a (table_scale, small_max, grid_per_cu, unroll)
+    // sweep at RMAT scales 21 and 23 on MI355X
+    TuneR r{8, 256, 16, 0, 4};
     if (t) {
         if (t->table_scale) r.table_scale = t->table_scale;
         if (t->small_max) r.small_max = t->small_max;
@@ -66,6 +92,8 @@ lstc_status validate_graph(const HostGraph &h, const char *who) {
     if (h.m >= 0xFFFFF000ull)
         return fail(LSTC_ERR_INVALID, "%s: m = %llu arcs exceeds 2^32-4096", who, (unsigned long long)h.m);
     if (h.n_owned && (!h.off || (h.m && !h.col))) return fail(LSTC_ERR_INVALID, "%s: NULL CSR arrays", who);
+    if ((h.in_off == nullptr) != (h.in_col == nullptr))
+        return fail(LSTC_ERR_INVALID, "%s: give both in-list arrays or neither", who);
     return LSTC_OK;
 }
 
@@ -74,7 +102,17 @@ struct Bounds {
 };
 
 __device__ __forceinline__ GraphDev graph_at(const GraphSet &gs, uint32_t g) {
-    return gs.table ? gs.table[g] : gs.one;
+    if (!gs.table) return gs.one;
+    const LSTC_GLOBAL_AS GraphDev *q = (const LSTC_GLOBAL_AS GraphDev *)(gs.table + g);
+    GraphDev d;
+    d.off = q->off;
+    d.col = q->col;
+    d.per_row = q->per_row;
+    d.in_off = q->in_off;
+    d.in_col = q->in_col;
+    d.n_rows = q->n_rows;
+    d.n_owned = q->n_owned;
+    return d;
 }
 
 __device__ __forceinline__ uint32_t table_log2_for(uint32_t keys, uint32_t scale, uint32_t max_log2) {
@@ -91,11 +129,12 @@ __global__ void k_plan_init(uint32_t *ctl, unsigned long long *counts, uint32_t 
     for (uint32_t i = threadIdx.x; i < G; i += blockDim.x) counts[i] = 0;
 }
 
-// pass 0 counts rows per bin; pass 1 scatters (g << 32 | u) into the queue,
-// laid out bins 5,4,3 (big, largest first) then 2,1,0 (small, largest first).
+// pass 0 counts items per bin; pass 1 writes them, laid out bins 5,4,3 (big,
+// largest first) then 2,1,0 (small, largest first).
 template <int PASS>
 __global__ __launch_bounds__(256) void k_plan_rows(GraphSet gs, Bounds bd, uint32_t *__restrict__ ctl,
                                                    unsigned long long *__restrict__ queue) {
+    const uint32_t lane = lane_id();
     uint32_t bin_off[kBins];
     if (PASS == 1) {
         uint32_t acc = 0;
@@ -105,36 +144,46 @@ __global__ __launch_bounds__(256) void k_plan_rows(GraphSet gs, Bounds bd, uint3
     for (uint64_t r0 = (uint64_t)blockIdx.x * blockDim.x; r0 < total; r0 += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t r = r0 + threadIdx.x;
         int b = -1;
-        uint32_t g = 0, u = 0;
+        uint32_t g = 0, u = 0, n_b = 0;
         if (r < total) {
             if (gs.table) {  // last g with row_base[g] <= r
                 uint32_t lo = 0, hi = gs.G;
                 while (hi - lo > 1) {
                     const uint32_t mid = (lo + hi) >> 1;
-                    if (gs.row_base[mid] <= r) lo = mid; else hi = mid;
+                    if (gload(gs.row_base + mid) <= r) lo = mid; else hi = mid;
                 }
                 g = lo;
-                u = (uint32_t)(r - gs.row_base[g]);
+                u = (uint32_t)(r - gload(gs.row_base + g));
             } else {
                 u = (uint32_t)r;
             }
-            const uint64_t *off = graph_at(gs, g).off;
-            const uint64_t k = off[u + 1] - off[u];
+            const GraphDev G = graph_at(gs, g);
+            const uint64_t k = gload(G.off + u + 1) - gload(G.off + u);
             if (k >= 2) {
                 b = kBins - 1;
                 for (int i = kBins - 2; i >= 0; --i)
                     if (k <= bd.hi[i]) b = i;
+                if (G.in_off) {
+                    const uint64_t kb = gload(G.in_off + u + 1) - gload(G.in_off + u);
+                    n_b = (uint32_t)((kb + kSegChunk - 1) / kSegChunk);
+                }
             }
         }
+        const uint32_t n_items = (b >= 0) ? 1u + n_b : 0u;
 #pragma unroll
         for (int i = 0; i < kBins; ++i) {
-            const uint64_t m = ballot(b == i);
-            if (m == 0) continue;
-            const uint32_t first = (uint32_t)__builtin_ctzll(m);
+            if (ballot(b == i) == 0) continue;
+            const uint32_t c = (b == i) ? n_items : 0u;
+            const uint32_t incl = wave_incl_scan(c, lane);
+            const uint32_t tot = read_lane(incl, 63);
             uint32_t base = 0;
-            if (lane_id() == first) base = atomicAdd(ctl + (PASS == 0 ? C_CNT : C_CUR) + i, (uint32_t)__popcll(m));
-            base = read_lane(base, first);
-            if (PASS == 1 && b == i) queue[bin_off[i] + base + mask_rank(m)] = ((unsigned long long)g << 32) | u;
+            if (lane == 0) base = atomicAdd(ctl + (PASS == 0 ? C_CNT : C_CUR) + i, tot);
+            base = read_lane(base, 0);
+            if (PASS == 1 && b == i) {
+                unsigned long long *q = queue + bin_off[i] + base + (incl - c);
+                q[0] = make_item(g, 0, 0, u);
+                for (uint32_t j = 0; j < n_b; ++j) q[1 + j] = make_item(g, 1, j, u);
+            }
         }
     }
 }
@@ -155,32 +204,34 @@ __device__ __forceinline__ void table_insert(uint32_t *tab, uint32_t tlog2, cons
                                              uint64_t b0, uint32_t nk, uint32_t tid, uint32_t nthreads) {
     const uint32_t mask = (1u << tlog2) - 1, shift = 32 - tlog2;
     for (uint32_t j = tid; j < nk; j += nthreads) {
-        const uint32_t key = col[b0 + j];
+        const uint32_t key = gload(col + b0 + j);
         uint32_t h = hslot(key, shift);
         while (atomicCAS(&tab[h], kEmpty, key) != kEmpty) h = (h + 1) & mask;
     }
 }
 
 // ------------------------------------------------------------------ walk
-// Probes every entry of N+(w) for the segments j in [w0, K) step wstep (in
-// windows of 64) of row u (N+(u) at col[rb .. rb+K)) against tab.
+// Streams the lists N+(s) of the segment ids s = seg[j], j in [w0, nseg) step
+// wstep (windows of 64), through tab.  A segment whose list is not shorter than
+// `below` is skipped (below = 0: none skipped); ids >= n_rows have no list.
 template <int UNROLL>
-__device__ __forceinline__ unsigned long long walk_row(const uint64_t *__restrict__ off,
-                                                       const uint32_t *__restrict__ col, uint32_t n_rows,
-                                                       uint64_t rb, uint32_t K, uint32_t w0, uint32_t wstep,
-                                                       const uint32_t *tab, uint32_t tlog2, uint64_t *s_base,
-                                                       uint32_t *s_end, uint32_t *flag, uint32_t lane) {
+__device__ __forceinline__ unsigned long long walk(const uint64_t *__restrict__ off, const uint32_t *__restrict__ col,
+                                                   uint32_t n_rows, const uint32_t *__restrict__ seg, uint32_t nseg_all,
+                                                   uint32_t below, uint32_t w0, uint32_t wstep, const uint32_t *tab,
+                                                   uint32_t tlog2, uint64_t *s_base, uint32_t *s_end, uint32_t *flag,
+                                                   uint32_t lane) {
     const uint32_t mask = (1u << tlog2) - 1, shift = 32 - tlog2;
     uint32_t lane_hits = 0;
-    for (uint32_t j0 = w0; j0 < K; j0 += wstep) {
+    for (uint32_t j0 = w0; j0 < nseg_all; j0 += wstep) {
         const uint32_t j = j0 + lane;
         uint64_t s = 0;
         uint32_t len = 0;
-        if (j < K) {
-            const uint32_t w = col[rb + j];
-            if (w < n_rows) {  // ids outside the CSR have no row
-                s = off[w];
-                len = (uint32_t)(off[w + 1] - s);
+        if (j < nseg_all) {
+            const uint32_t w = gload(seg + j);
+            if (w < n_rows) {
+                s = gload(off + w);
+                len = (uint32_t)(gload(off + w + 1) - s);
+                if (below && len >= below) len = 0;  // that arc is counted from the other side
             }
         }
         const uint32_t incl = wave_incl_scan(len, lane);
@@ -216,12 +267,12 @@ __device__ __forceinline__ unsigned long long walk_row(const uint64_t *__restric
                         if (mk) flag[myend - Bk] = 1;
                         const uint64_t M = ballot(flag[lane] != 0);
                         if (mk) flag[myend - Bk] = 0;
-                        const uint32_t seg = cur + (uint32_t)__popcll(M & ((2ull << lane) - 1ull));
-                        base = gather64(mybase, seg);
+                        const uint32_t sg = cur + (uint32_t)__popcll(M & ((2ull << lane) - 1ull));
+                        base = gather64(mybase, sg);
                         cur += (uint32_t)__popcll(ballot(lane >= cur && myend <= Bk + 64));
                     }
                     if (t < total) {
-                        x[k] = col[base + t];
+                        x[k] = gload(col + base + t);
                         live[k] = true;
                     }
                 }
@@ -230,7 +281,7 @@ __device__ __forceinline__ unsigned long long walk_row(const uint64_t *__restric
             for (int k = 0; k < UNROLL; ++k) h[k] = hslot(x[k], shift);
             // lockstep linear probing: one LDS round trip per step for all chunks.
             // Hits are counted per lane inside the divergent probe (a lane that has
-            // finished is inactive there) and reduced across the wave per row.
+            // finished is inactive there) and reduced across the wave per item.
             for (;;) {
                 bool more = false;
 #pragma unroll
@@ -249,6 +300,23 @@ __device__ __forceinline__ unsigned long long walk_row(const uint64_t *__restric
         }
     }
     return wave_sum64((unsigned long long)lane_hits);
+}
+
+// The segment source of a work item: kind A walks N+(r) with the split filter,
+// kind B walks one chunk of r's in-list unfiltered.
+struct Segs {
+    const uint32_t *ids;
+    uint32_t n;
+    uint32_t below;
+};
+
+__device__ __forceinline__ Segs item_segs(const GraphDev &G, uint32_t kind, uint32_t chunk, uint32_t r, uint64_t rb,
+                                          uint32_t K) {
+    if (kind == 0) return Segs{G.col + rb, K, G.in_off ? K : 0u};
+    const uint64_t ib = gload(G.in_off + r), ie = gload(G.in_off + r + 1);
+    const uint64_t c0 = ib + (uint64_t)chunk * kSegChunk;
+    const uint64_t c1 = (c0 + kSegChunk < ie) ? c0 + kSegChunk : ie;
+    return Segs{G.in_col + c0, (uint32_t)(c1 - c0), 0u};
 }
 
 // ------------------------------------------------------------------ persistent count kernel
@@ -278,12 +346,13 @@ __global__ __launch_bounds__(kWaves * 64) void k_count(GraphSet gs, const unsign
         __syncthreads();
         if (t >= n_tickets) break;
         if (t < n_big) {
-            // big row: the whole workgroup shares one table
-            const unsigned long long it = queue[t];
-            const uint32_t g = (uint32_t)(it >> 32), u = (uint32_t)it;
+            // big item: the whole workgroup shares one table
+            const Item itm = decode(gload(queue + t));
+            const uint32_t u = itm.row, chunk = itm.chunk, kind = itm.kind, g = itm.g;
             const GraphDev G = graph_at(gs, g);
-            const uint64_t rb = G.off[u];
-            const uint32_t K = (uint32_t)(G.off[u + 1] - rb);
+            const uint64_t rb = gload(G.off + u);
+            const uint32_t K = (uint32_t)(gload(G.off + u + 1) - rb);
+            const Segs sg = item_segs(G, kind, chunk, u, rb, K);
             unsigned long long hits = 0;
             for (uint32_t t0 = 0; t0 < K; t0 += kBigTile) {
                 const uint32_t tk = min(K - t0, kBigTile);
@@ -292,8 +361,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_count(GraphSet gs, const unsign
                 __syncthreads();
                 table_insert(s_tab, tl, G.col, rb + t0, tk, tid, kWaves * 64);
                 __syncthreads();
-                hits += walk_row<UNROLL>(G.off, G.col, G.n_rows, rb, K, wv * 64, kWaves * 64, s_tab, tl,
-                                         s_base[wv], s_end[wv], s_flag[wv], lane);
+                hits += walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.n, sg.below, wv * 64, kWaves * 64, s_tab, tl,
+                                     s_base[wv], s_end[wv], s_flag[wv], lane);
                 __syncthreads();
             }
             if (g != cur_g) {
@@ -304,21 +373,22 @@ __global__ __launch_bounds__(kWaves * 64) void k_count(GraphSet gs, const unsign
             acc += hits;
             if (G.per_row && lane == 0 && hits) atomicAdd(G.per_row + u, hits);
         } else {
-            // small rows: each wave takes every kWaves-th row of the group
+            // small items: each wave takes every kWaves-th item of the group
             const uint32_t first = n_big + (t - n_big) * small_group;
             const uint32_t last = min(first + small_group, n_big + n_small);
             uint32_t *tab = s_tab + wv * kSlice;
             for (uint32_t q = first + wv; q < last; q += kWaves) {
-                const unsigned long long it = queue[q];
-                const uint32_t g = (uint32_t)(it >> 32), u = (uint32_t)it;
+                const Item itm = decode(gload(queue + q));
+                const uint32_t u = itm.row, chunk = itm.chunk, kind = itm.kind, g = itm.g;
                 const GraphDev G = graph_at(gs, g);
-                const uint64_t rb = G.off[u];
-                const uint32_t K = (uint32_t)(G.off[u + 1] - rb);
+                const uint64_t rb = gload(G.off + u);
+                const uint32_t K = (uint32_t)(gload(G.off + u + 1) - rb);
+                const Segs sg = item_segs(G, kind, chunk, u, rb, K);
                 const uint32_t tl = table_log2_for(K, scale, kSliceLog2);
                 for (uint32_t i = lane; i < (1u << tl); i += 64) tab[i] = kEmpty;
                 table_insert(tab, tl, G.col, rb, K, lane, 64);
-                const unsigned long long hits = walk_row<UNROLL>(G.off, G.col, G.n_rows, rb, K, 0, 64, tab, tl,
-                                                                 s_base[wv], s_end[wv], s_flag[wv], lane);
+                const unsigned long long hits = walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.n, sg.below, 0, 64,
+                                                             tab, tl, s_base[wv], s_end[wv], s_flag[wv], lane);
                 if (g != cur_g) {
                     if (cur_g != kEmpty && acc != 0 && lane == 0) atomicAdd(counts + cur_g, acc);
                     cur_g = g;
@@ -344,11 +414,14 @@ __global__ __launch_bounds__(kWaves * 64) void k_count(GraphSet gs, const unsign
 }
 
 // ------------------------------------------------------------------ host side
-size_t count_ws_bytes(uint32_t G, uint64_t total_rows) {
+// queue entries: one kind A item per row plus at most rows + arcs / kSegChunk kind B items
+static uint64_t queue_cap(uint64_t total_rows, uint64_t total_m) { return 2 * total_rows + total_m / kSegChunk + 1; }
+
+size_t count_ws_bytes(uint32_t G, uint64_t total_rows, uint64_t total_m) {
     (void)G;
     Carver c(nullptr, 0);
     c.take<uint32_t>(64);
-    c.take<unsigned long long>(total_rows ? total_rows : 1);
+    c.take<unsigned long long>(queue_cap(total_rows, total_m));
     return c.need();
 }
 
@@ -356,23 +429,26 @@ lstc_status count_launch(const HostGraph *hosts, uint32_t G, const GraphDev *d_t
                          unsigned long long *d_counts, const lstc_tune *tune, void *ws, size_t ws_bytes,
                          hipStream_t st) {
     if (G == 0) return LSTC_OK;
+    if (G > kMaxGraphs) return fail(LSTC_ERR_INVALID, "count: at most %u graphs per launch", kMaxGraphs);
     if (!d_counts) return fail(LSTC_ERR_INVALID, "count: NULL counts");
     if (G > 1 && (!d_table || !d_row_base)) return fail(LSTC_ERR_INVALID, "count: batch needs device descriptors");
     const TuneR tr = resolve_tune(tune);
-    uint64_t total = 0;
-    for (uint32_t g = 0; g < G; ++g) total += hosts[g].n_owned;
+    uint64_t total = 0, total_m = 0;
+    for (uint32_t g = 0; g < G; ++g) {
+        total += hosts[g].n_owned;
+        total_m += hosts[g].m;
+    }
     if (total >= 0xFFFFFFFFull) return fail(LSTC_ERR_INVALID, "count: more than 2^32-1 owned rows in one launch");
     Carver c(ws, ws_bytes);
     uint32_t *ctl = c.take<uint32_t>(64);
-    unsigned long long *queue = c.take<unsigned long long>(total ? total : 1);
+    unsigned long long *queue = c.take<unsigned long long>(queue_cap(total, total_m));
     if (!ws || !c.ok()) return fail(LSTC_ERR_WORKSPACE, "count: workspace %zu < %zu bytes", ws_bytes, c.need());
 
     GraphSet gs{};
     gs.G = G;
     gs.total_rows = total;
     if (G == 1) {
-        gs.one = GraphDev{hosts[0].off, hosts[0].col, (unsigned long long *)hosts[0].per_row, hosts[0].n_rows,
-                          hosts[0].n_owned};
+        gs.one = to_dev(hosts[0]);
     } else {
         gs.table = d_table;
         gs.row_base = d_row_base;

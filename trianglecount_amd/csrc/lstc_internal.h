@@ -93,7 +93,8 @@ lstc_status radix_sort_u64(uint64_t *keys, uint64_t *alt, uint64_t n, int key_bi
 // (*d_sizes)[1] = vertices after cleaning, (*d_sizes)[2] = undirected edges.
 lstc_status clean_orient_launch(const uint32_t *d_src, const uint32_t *d_dst, uint64_t m_raw, uint32_t n,
                                 int compact, uint64_t *d_sym_off, uint32_t *d_sym_col, uint64_t *d_off,
-                                uint32_t *d_col, void *d_ws, size_t ws_bytes, hipStream_t st,
+                                uint32_t *d_col, uint64_t *d_in_off, uint32_t *d_in_col, void *d_ws,
+                                size_t ws_bytes, hipStream_t st,
                                 const uint64_t **d_sizes, const uint32_t **d_bad);
 
 // Fill kernels (used instead of hipMemsetAsync so callers may capture them in graphs).

@@ -126,6 +126,22 @@ __global__ void k_orient_flags(const uint64_t *__restrict__ uk, const uint64_t *
     }
 }
 
+// In-arcs d -> s (the reverse keys of row s) with d+(d) <= d+(s): the arcs the
+// count handles from the target side.  Flags for [0, nk]; zero from m2 on.
+__global__ void k_in_flags(const uint64_t *__restrict__ uk, const uint64_t *__restrict__ d_m2, uint64_t nk,
+                           uint32_t b, uint64_t mask, const uint32_t *__restrict__ deg,
+                           const uint32_t *__restrict__ dout, uint32_t *__restrict__ f) {
+    const uint64_t m2 = *d_m2;
+    GRID_STRIDE(i, nk + 1) {
+        if (i >= m2) { f[i] = 0u; continue; }
+        const uint64_t k = uk[i];
+        const uint32_t s = (uint32_t)(k >> b), d = (uint32_t)(k & mask);
+        const uint32_t ds = deg[s], dd = deg[d];
+        const bool out_arc = ds < dd || (ds == dd && d < s);
+        f[i] = (!out_arc && dout[d] <= dout[s]) ? 1u : 0u;
+    }
+}
+
 __global__ void k_orient_scatter(const uint64_t *__restrict__ uk, const uint64_t *__restrict__ d_m2,
                                  uint64_t mask, const uint32_t *__restrict__ f,
                                  const uint64_t *__restrict__ pos, const uint32_t *__restrict__ nid,
@@ -145,8 +161,8 @@ __global__ void k_orient_offsets(const uint32_t *__restrict__ keepv, const uint6
 struct PrepLayout {
     uint64_t *keys, *alt, *pos;
     uint32_t *flags;
-    uint32_t *deg, *keepv, *nid, *dout;
-    uint64_t *degscan, *nidscan, *doscan, *first, *last;
+    uint32_t *deg, *keepv, *nid, *dout, *din;
+    uint64_t *degscan, *nidscan, *doscan, *first, *last, *discan;
     uint64_t *scal;  // [0]=m2 [1]=n_out [2]=m [3]=unused
     uint32_t *bad;
     void *sort_ws;
@@ -168,6 +184,8 @@ static PrepLayout prep_layout(void *ws, size_t ws_bytes, uint32_t n, uint64_t m_
     L.keepv = c.take<uint32_t>((size_t)n + 1);
     L.nid = c.take<uint32_t>((size_t)n + 1);
     L.dout = c.take<uint32_t>((size_t)n + 1);
+    L.din = c.take<uint32_t>((size_t)n + 1);
+    L.discan = c.take<uint64_t>((size_t)n + 1);
     L.degscan = c.take<uint64_t>((size_t)n + 1);
     L.nidscan = c.take<uint64_t>((size_t)n + 1);
     L.doscan = c.take<uint64_t>((size_t)n + 1);
@@ -198,7 +216,8 @@ extern "C" lstc_status lstc_clean_orient_workspace(uint32_t n, uint64_t m_raw, s
 // valid once the stream reaches this point.
 lstc_status lstc::clean_orient_launch(const uint32_t *d_src, const uint32_t *d_dst, uint64_t m_raw, uint32_t n,
                                       int compact, uint64_t *d_sym_off, uint32_t *d_sym_col, uint64_t *d_off,
-                                      uint32_t *d_col, void *d_ws, size_t ws_bytes, hipStream_t st,
+                                      uint32_t *d_col, uint64_t *d_in_off, uint32_t *d_in_col, void *d_ws,
+                                      size_t ws_bytes, hipStream_t st,
                                       const uint64_t **d_sizes, const uint32_t **d_bad) {
     if (n == 0xFFFFFFFFu) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: n must be < 2^32-1");
     if (!d_off) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: NULL output");
@@ -260,6 +279,22 @@ lstc_status lstc::clean_orient_launch(const uint32_t *d_src, const uint32_t *d_d
         hipLaunchKernelGGL(k_orient_offsets, dim3(grid_n), dim3(256), 0, st, L.keepv, L.nidscan, L.doscan, n,
                            L.scal + 1, L.scal + 2, d_off);
         LSTC_LAUNCHED("k_orient_offsets");
+        if (d_in_off) {
+            // in-lists of the arcs counted from the target side (same sorted key list, no atomics)
+            hipLaunchKernelGGL(k_in_flags, dim3(grid_k), dim3(256), 0, st, uk, L.scal + 0, nk, b, mask, L.deg,
+                               L.dout, L.flags);
+            LSTC_LAUNCHED("k_in_flags");
+            LSTC_TRY(scan_exclusive_u32_to_u64(L.flags, L.pos, nk + 1, L.scal + 3, L.scan_ws, L.scan_bytes, st));
+            hipLaunchKernelGGL(k_orient_scatter, dim3(grid_k), dim3(256), 0, st, uk, L.scal + 0, mask, L.flags,
+                               L.pos, L.nid, d_in_col);
+            LSTC_LAUNCHED("k_orient_scatter(in)");
+            hipLaunchKernelGGL(k_out_degree, dim3(grid_n), dim3(256), 0, st, L.first, L.last, L.pos, n, L.din);
+            LSTC_LAUNCHED("k_out_degree(in)");
+            LSTC_TRY(scan_exclusive_u32_to_u64(L.din, L.discan, n, nullptr, L.scan_ws, L.scan_bytes, st));
+            hipLaunchKernelGGL(k_orient_offsets, dim3(grid_n), dim3(256), 0, st, L.keepv, L.nidscan, L.discan, n,
+                               L.scal + 1, L.scal + 3, d_in_off);
+            LSTC_LAUNCHED("k_orient_offsets(in)");
+        }
     } else {
         // no edges: every vertex is isolated
         hipLaunchKernelGGL(k_nonzero, dim3(grid_n), dim3(256), 0, st, L.deg, n, compact, L.keepv);
@@ -272,6 +307,11 @@ lstc_status lstc::clean_orient_launch(const uint32_t *d_src, const uint32_t *d_d
         hipLaunchKernelGGL(k_orient_offsets, dim3(grid_n), dim3(256), 0, st, L.keepv, L.nidscan, L.doscan, n,
                            L.scal + 1, L.scal + 2, d_off);
         LSTC_LAUNCHED("k_orient_offsets");
+        if (d_in_off) {
+            hipLaunchKernelGGL(k_orient_offsets, dim3(grid_n), dim3(256), 0, st, L.keepv, L.nidscan, L.doscan, n,
+                               L.scal + 1, L.scal + 3, d_in_off);
+            LSTC_LAUNCHED("k_orient_offsets(in)");
+        }
     }
     if (d_sizes) *d_sizes = L.scal;
     if (d_bad) *d_bad = L.bad;
@@ -281,14 +321,16 @@ lstc_status lstc::clean_orient_launch(const uint32_t *d_src, const uint32_t *d_d
 extern "C" lstc_status lstc_clean_orient(const uint32_t *d_src, const uint32_t *d_dst, uint64_t m_raw,
                                          uint32_t n, int compact, uint64_t *d_sym_off,
                                          uint32_t *d_sym_col, uint64_t *d_off, uint32_t *d_col,
-                                         uint32_t *n_out, uint64_t *m_out, void *d_ws,
-                                         size_t ws_bytes, lstc_stream s) {
+                                         uint64_t *d_in_off, uint32_t *d_in_col, uint32_t *n_out,
+                                         uint64_t *m_out, void *d_ws, size_t ws_bytes, lstc_stream s) {
     hipStream_t st = (hipStream_t)s;
     if (!n_out || !m_out) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: NULL size outputs");
     const uint64_t *d_sizes = nullptr;
     const uint32_t *d_bad = nullptr;
-    LSTC_TRY(clean_orient_launch(d_src, d_dst, m_raw, n, compact, d_sym_off, d_sym_col, d_off, d_col, d_ws,
-                                 ws_bytes, st, &d_sizes, &d_bad));
+    if ((d_in_off == nullptr) != (d_in_col == nullptr))
+        return fail(LSTC_ERR_INVALID, "lstc_clean_orient: give both in-list outputs or neither");
+    LSTC_TRY(clean_orient_launch(d_src, d_dst, m_raw, n, compact, d_sym_off, d_sym_col, d_off, d_col, d_in_off,
+                                 d_in_col, d_ws, ws_bytes, st, &d_sizes, &d_bad));
     uint64_t h[4];
     uint32_t bad = 0;
     LSTC_HIP(hipMemcpyAsync(h, d_sizes, sizeof(h), hipMemcpyDeviceToHost, st));

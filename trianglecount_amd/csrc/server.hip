@@ -102,6 +102,8 @@ struct Req {
     uint32_t *src = nullptr, *dst = nullptr;
     uint64_t *off = nullptr;
     uint32_t *col = nullptr;
+    uint64_t *in_off = nullptr;  // target-side in-lists (null for caller CSRs)
+    uint32_t *in_col = nullptr;
     void *ws = nullptr;
     size_t ws_bytes = 0;
     uint32_t n_rows = 0;
@@ -174,8 +176,9 @@ static size_t prep_bytes(uint32_t n, uint64_t m_raw, size_t *ws_out) {
     *ws_out = ws;
     return round256(8 * m_raw) + ws;  // raw src/dst + clean workspace
 }
+// resident per request: oriented CSR plus its target-side in-lists (split count)
 static size_t resident_bytes(uint32_t n, uint64_t m_raw) {
-    return round256(8 * ((size_t)n + 1)) + round256(4 * (m_raw ? m_raw : 1));
+    return 2 * (round256(8 * ((size_t)n + 1)) + round256(4 * (m_raw ? m_raw : 1)));
 }
 static uint32_t request_rows(const lstc_request &r) {
     if (r.kind == LSTC_REQ_RMAT) return 1u << r.scale;
@@ -210,8 +213,11 @@ static lstc_status admit(lstc_server *S) {
         r->dst = (uint32_t *)(t + 4 * r->spec.m_raw);
         r->ws = t + round256(8 * r->spec.m_raw);
         r->ws_bytes = ws;
+        const size_t half = round256(8 * ((size_t)n + 1)) + round256(4 * (r->spec.m_raw ? r->spec.m_raw : 1));
         r->off = (uint64_t *)res;
         r->col = (uint32_t *)(res + round256(8 * ((size_t)n + 1)));
+        r->in_off = (uint64_t *)(res + half);
+        r->in_col = (uint32_t *)(res + half + round256(8 * ((size_t)n + 1)));
         r->n_rows = n;
         r->m_cap = r->spec.m_raw;
         hipStream_t st = S->prep_stream;
@@ -223,8 +229,8 @@ static lstc_status admit(lstc_server *S) {
             rc = lstc_gen_er(n, r->spec.m_raw, r->spec.seed, r->src, r->dst, st);
         if (rc != LSTC_OK) return rc;
         // compact = 0 keeps n rows, so the count needs no host-side sizes
-        LSTC_TRY(clean_orient_launch(r->src, r->dst, r->spec.m_raw, n, 0, nullptr, nullptr, r->off, r->col, r->ws,
-                                     r->ws_bytes, st, nullptr, &r->d_bad));
+        LSTC_TRY(clean_orient_launch(r->src, r->dst, r->spec.m_raw, n, 0, nullptr, nullptr, r->off, r->col, r->in_off,
+                                     r->in_col, r->ws, r->ws_bytes, st, nullptr, &r->d_bad));
         LSTC_HIP(hipEventCreateWithFlags(&r->prep_done, hipEventDisableTiming));
         LSTC_HIP(hipEventRecord(r->prep_done, st));
         S->preparing.push_back(r);
@@ -261,13 +267,16 @@ static lstc_status launch_batch(lstc_server *S) {
     if (!S->in_flight.empty() || S->ready.empty()) return LSTC_OK;
     // every ready request joins, as long as the work queue fits the reserve
     std::vector<Req *> batch;
-    uint64_t rows = 0;
+    uint64_t rows = 0, arcs = 0;
     while (!S->ready.empty() && batch.size() < S->max_batch) {
         Req *r = S->ready.front();
         const uint32_t G1 = (uint32_t)batch.size() + 1;
-        if (!batch.empty() && batch_head_bytes(G1) + count_ws_bytes(G1, rows + r->n_rows) > S->reserve_bytes) break;
+        if (!batch.empty() &&
+            batch_head_bytes(G1) + count_ws_bytes(G1, rows + r->n_rows, arcs + r->m_cap) > S->reserve_bytes)
+            break;
         batch.push_back(r);
         rows += r->n_rows;
+        arcs += r->m_cap;
         S->ready.pop_front();
     }
     const uint32_t G = (uint32_t)batch.size();
@@ -275,8 +284,8 @@ static lstc_status launch_batch(lstc_server *S) {
     uint64_t acc = 0;
     for (uint32_t g = 0; g < G; ++g) {
         Req *r = batch[g];
-        hosts[g] = HostGraph{r->off, r->col, nullptr, r->n_rows, r->n_rows, r->m_cap};
-        S->h_desc[g] = GraphDev{r->off, r->col, nullptr, r->n_rows, r->n_rows};
+        hosts[g] = HostGraph{r->off, r->col, nullptr, r->n_rows, r->n_rows, r->m_cap, r->in_off, r->in_col};
+        S->h_desc[g] = to_dev(hosts[g]);
         S->h_base[g] = acc;
         acc += r->n_rows;
     }
@@ -422,7 +431,7 @@ extern "C" lstc_status lstc_server_submit(lstc_server *S, const lstc_request *re
     if (r.kind == LSTC_REQ_CSR) {
         HostGraph h{r.off, r.col, nullptr, r.n_rows, r.n_rows, r.m};
         LSTC_TRY(validate_graph(h, "lstc_server_submit"));
-        if (batch_head_bytes(1) + count_ws_bytes(1, r.n_rows) > S->reserve_bytes)
+        if (batch_head_bytes(1) + count_ws_bytes(1, r.n_rows, r.m) > S->reserve_bytes)
             return fail(LSTC_ERR_OVER_BUDGET, "lstc_server_submit: CSR rows exceed the batch reserve");
     } else {
         if (r.m_raw >= 0x7FFFF000ull) return fail(LSTC_ERR_INVALID, "lstc_server_submit: too many edges");
@@ -430,7 +439,7 @@ extern "C" lstc_status lstc_server_submit(lstc_server *S, const lstc_request *re
         const uint32_t n = request_rows(r);
         size_t ws = 0;
         const size_t need = prep_bytes(n, r.m_raw, &ws) + resident_bytes(n, r.m_raw);
-        const size_t qneed = batch_head_bytes(1) + count_ws_bytes(1, n);
+        const size_t qneed = batch_head_bytes(1) + count_ws_bytes(1, n, r.m_raw);
         if (need > S->pool.total() || qneed > S->reserve_bytes) {
             std::lock_guard<std::mutex> lk(S->mu);
             S->n_rejected++;

@@ -5,6 +5,16 @@
 
 namespace lstc {
 
+// Loads through an explicit global (address space 1) pointer.  Pointers that
+// reach a kernel inside a struct are generic to the compiler, which then emits
+// FLAT loads; those share the LDS counter, so every LDS wait also drains the
+// outstanding global loads.  Global loads keep the two queues independent.
+#define LSTC_GLOBAL_AS __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ T gload(const T *p) {
+    return *(const LSTC_GLOBAL_AS T *)p;
+}
+
 __device__ __forceinline__ uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
