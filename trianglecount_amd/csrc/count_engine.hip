@@ -28,13 +28,13 @@
 //   small item: one wave, a private 1024-slot slice of the same LDS.
 // Walk of one window: one segment per lane (id, start, length), inclusive scan
 // of lengths, empty segments compacted out; then the flat sequence of entries
-// is consumed 64 per chunk.  A chunk inside one segment takes its base from the
-// current segment (scalar); otherwise each lane finds its segment from a
-// boundary mask (LDS flags + ballot + popcount).  UNROLL chunks are addressed
-// and loaded (global loads, never FLAT) before the first probe, then probed in
-// lockstep, so each wave keeps UNROLL loads in flight.  Hits are counted per
-// lane and reduced across the wave per item; the workgroup issues one uint64
-// atomic per graph it counted.
+// is consumed in rounds of UNROLL chunks of 64.  The segment ends inside a
+// round are marked in LDS once, then every chunk finds each lane's segment with
+// one LDS read + ballot + popcount (a chunk inside one segment takes its base
+// from a scalar).  Rounds are software-pipelined: round r+1 is addressed and
+// loaded (global loads, never FLAT) before round r is probed, lockstep over its
+// UNROLL chunks.  Hits are counted per lane and reduced across the wave per
+// item; the workgroup issues one uint64 atomic per graph it counted.
 #include "count_engine.h"
 #include "wave.h"
 
@@ -211,15 +211,86 @@ __device__ __forceinline__ void table_insert(uint32_t *tab, uint32_t tlog2, cons
 }
 
 // ------------------------------------------------------------------ walk
+// One round of a window: the entries [B, B + 64*UNROLL) of the window's flat
+// sequence (B < total), chunk k (64 entries) loaded into x[k].  Lane i of the
+// wave holds segment i: mybase = its column base minus its flat start, myend =
+// its exclusive flat end (kEmpty for lanes without a segment); ends ascend.
+// The ends that fall inside the round are marked in `flag` (one byte per
+// position, left all zero again), so every chunk finds each lane's segment with
+// one LDS read + ballot + popcount, independently of the other chunks.
+// Positions at or past `total` load col[0] (in bounds, since total > 0 means the
+// graph has arcs) and are never probed: every load is unconditional, so its
+// result register is never copied and the loads stay in flight until probed.
+template <int UNROLL>
+__device__ __forceinline__ void issue_round(const uint32_t *__restrict__ col, uint32_t B, uint32_t total,
+                                            uint64_t mybase, uint32_t myend, uint8_t *flag, uint32_t lane,
+                                            uint32_t (&x)[UNROLL]) {
+    constexpr uint32_t RB = 64u * UNROLL;
+    // segments that end at or before B lie wholly before this round
+    uint32_t sg = (uint32_t)__popcll(ballot(myend <= B));
+    // an end e inside the round: position e already belongs to the next segment
+    const bool mk = myend > B && myend < B + RB;
+    if (mk) flag[myend - B] = 1;
+#pragma unroll
+    for (int k = 0; k < UNROLL; ++k) {
+        const uint64_t M = ballot(flag[64u * k + lane] != 0);
+        uint64_t base;
+        if (M == 0)
+            base = read_lane64(mybase, sg & 63u);  // the chunk lies inside one segment
+        else
+            base = gather64(mybase, sg + (uint32_t)__popcll(M & ((2ull << lane) - 1ull)));
+        sg += (uint32_t)__popcll(M);
+        const uint32_t t = B + 64u * k + lane;
+        const uint32_t *p = t < total ? col + base + t : col;
+        x[k] = gload(p);
+    }
+    if (mk) flag[myend - B] = 0;
+}
+
+// Probes the live entries of one round through tab (linear probing, lockstep
+// over the UNROLL chunks: one LDS round trip per step for all of them).  Hits
+// are counted per lane inside the divergent probe (a lane that has finished is
+// inactive there) and reduced across the wave per item.
+template <int UNROLL>
+__device__ __forceinline__ void probe_round(const uint32_t (&x)[UNROLL], uint32_t B, uint32_t total, const uint32_t *tab,
+                                            uint32_t mask, uint32_t shift, uint32_t lane, uint32_t &lane_hits) {
+    uint32_t h[UNROLL];
+    bool live[UNROLL];
+#pragma unroll
+    for (int k = 0; k < UNROLL; ++k) {
+        live[k] = B + 64u * k + lane < total;
+        h[k] = hslot(x[k], shift);
+    }
+    for (;;) {
+        bool more = false;
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) {
+            if (live[k]) {
+                const uint32_t v = tab[h[k]];
+                const bool hit = v == x[k];
+                lane_hits += hit ? 1u : 0u;
+                live[k] = !(hit || v == kEmpty);
+                h[k] = (h[k] + 1) & mask;
+                more |= live[k];
+            }
+        }
+        if (ballot(more) == 0) break;
+    }
+}
+
 // Streams the lists N+(s) of the segment ids s = seg[j], j in [w0, nseg) step
 // wstep (windows of 64), through tab.  A segment whose list is not shorter than
 // `below` is skipped (below = 0: none skipped); ids >= n_rows have no list.
+// Rounds are software-pipelined over two buffers: the loads of round r+1 are
+// issued before round r is probed, so a wave keeps up to 2*UNROLL chunk loads
+// in flight.
 template <int UNROLL>
 __device__ __forceinline__ unsigned long long walk(const uint64_t *__restrict__ off, const uint32_t *__restrict__ col,
                                                    uint32_t n_rows, const uint32_t *__restrict__ seg, uint32_t nseg_all,
                                                    uint32_t below, uint32_t w0, uint32_t wstep, const uint32_t *tab,
-                                                   uint32_t tlog2, uint64_t *s_base, uint32_t *s_end, uint32_t *flag,
+                                                   uint32_t tlog2, uint64_t *s_base, uint32_t *s_end, uint8_t *flag,
                                                    uint32_t lane) {
+    constexpr uint32_t RB = 64u * UNROLL;
     const uint32_t mask = (1u << tlog2) - 1, shift = 32 - tlog2;
     uint32_t lane_hits = 0;
     for (uint32_t j0 = w0; j0 < nseg_all; j0 += wstep) {
@@ -236,6 +307,7 @@ __device__ __forceinline__ unsigned long long walk(const uint64_t *__restrict__ 
         }
         const uint32_t incl = wave_incl_scan(len, lane);
         const uint32_t total = uniform(gather(incl, 63));
+        if (total == 0) continue;
         const uint64_t nzm = ballot(len != 0);
         const uint32_t nseg = (uint32_t)__popcll(nzm);
         if (len != 0) {
@@ -245,57 +317,25 @@ __device__ __forceinline__ unsigned long long walk(const uint64_t *__restrict__ 
         }
         const uint64_t mybase = s_base[lane];
         const uint32_t myend = lane < nseg ? s_end[lane] : kEmpty;
-        uint32_t cur = 0;
-        for (uint32_t B = 0; B < total; B += 64u * UNROLL) {
-            uint32_t x[UNROLL];
-            uint32_t h[UNROLL];
-            bool live[UNROLL];
-#pragma unroll
-            for (int k = 0; k < UNROLL; ++k) {
-                const uint32_t Bk = B + 64u * k;
-                x[k] = kEmpty;
-                live[k] = false;
-                if (Bk < total) {  // uniform
-                    const uint32_t cend = read_lane(myend, cur);
-                    const uint32_t t = Bk + lane;
-                    uint64_t base;
-                    if (cend >= Bk + 64) {
-                        base = read_lane64(mybase, cur);
-                        cur += (cend == Bk + 64) ? 1u : 0u;
-                    } else {
-                        const bool mk = lane >= cur && myend < Bk + 64;
-                        if (mk) flag[myend - Bk] = 1;
-                        const uint64_t M = ballot(flag[lane] != 0);
-                        if (mk) flag[myend - Bk] = 0;
-                        const uint32_t sg = cur + (uint32_t)__popcll(M & ((2ull << lane) - 1ull));
-                        base = gather64(mybase, sg);
-                        cur += (uint32_t)__popcll(ballot(lane >= cur && myend <= Bk + 64));
-                    }
-                    if (t < total) {
-                        x[k] = gload(col + base + t);
-                        live[k] = true;
-                    }
-                }
+        uint32_t xa[UNROLL], xb[UNROLL];
+        issue_round<UNROLL>(col, 0, total, mybase, myend, flag, lane, xa);
+        // The probe is written out in both arms of each branch: a join between
+        // an issue and the probe would make the wait for the probed buffer also
+        // wait for the loads just issued.
+        for (uint32_t B = 0;; B += 2 * RB) {
+            if (B + RB < total) {
+                issue_round<UNROLL>(col, B + RB, total, mybase, myend, flag, lane, xb);
+                probe_round<UNROLL>(xa, B, total, tab, mask, shift, lane, lane_hits);
+            } else {
+                probe_round<UNROLL>(xa, B, total, tab, mask, shift, lane, lane_hits);
+                break;
             }
-#pragma unroll
-            for (int k = 0; k < UNROLL; ++k) h[k] = hslot(x[k], shift);
-            // lockstep linear probing: one LDS round trip per step for all chunks.
-            // Hits are counted per lane inside the divergent probe (a lane that has
-            // finished is inactive there) and reduced across the wave per item.
-            for (;;) {
-                bool more = false;
-#pragma unroll
-                for (int k = 0; k < UNROLL; ++k) {
-                    if (live[k]) {
-                        const uint32_t v = tab[h[k]];
-                        const bool hit = v == x[k];
-                        lane_hits += hit ? 1u : 0u;
-                        live[k] = !(hit || v == kEmpty);
-                        h[k] = (h[k] + 1) & mask;
-                        more |= live[k];
-                    }
-                }
-                if (ballot(more) == 0) break;
+            if (B + 2 * RB < total) {
+                issue_round<UNROLL>(col, B + 2 * RB, total, mybase, myend, flag, lane, xa);
+                probe_round<UNROLL>(xb, B + RB, total, tab, mask, shift, lane, lane_hits);
+            } else {
+                probe_round<UNROLL>(xb, B + RB, total, tab, mask, shift, lane, lane_hits);
+                break;
             }
         }
     }
@@ -328,13 +368,13 @@ __global__ __launch_bounds__(kWaves * 64) void k_count(GraphSet gs, const unsign
     __shared__ uint32_t s_tab[kBigCap];
     __shared__ uint64_t s_base[kWaves][64];
     __shared__ uint32_t s_end[kWaves][64];
-    __shared__ uint32_t s_flag[kWaves][64];
+    __shared__ uint32_t s_flag[kWaves][16 * UNROLL];  // 64*UNROLL bytes per wave
     __shared__ uint32_t s_ticket;
     __shared__ unsigned long long s_acc[kWaves];
     __shared__ uint32_t s_g[kWaves];
 
     const uint32_t tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-    s_flag[wv][lane] = 0;
+    for (uint32_t i = lane; i < 16 * UNROLL; i += 64) s_flag[wv][i] = 0;
     const uint32_t n_big = ctl[C_NBIG], n_small = ctl[C_NSMALL], n_tickets = ctl[C_NTICK];
     uint32_t cur_g = kEmpty;
     unsigned long long acc = 0;
@@ -362,7 +402,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_count(GraphSet gs, const unsign
                 table_insert(s_tab, tl, G.col, rb + t0, tk, tid, kWaves * 64);
                 __syncthreads();
                 hits += walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.n, sg.below, wv * 64, kWaves * 64, s_tab, tl,
-                                     s_base[wv], s_end[wv], s_flag[wv], lane);
+                                     s_base[wv], s_end[wv], (uint8_t *)s_flag[wv], lane);
                 __syncthreads();
             }
             if (g != cur_g) {
@@ -388,7 +428,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_count(GraphSet gs, const unsign
                 for (uint32_t i = lane; i < (1u << tl); i += 64) tab[i] = kEmpty;
                 table_insert(tab, tl, G.col, rb, K, lane, 64);
                 const unsigned long long hits = walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.n, sg.below, 0, 64,
-                                                             tab, tl, s_base[wv], s_end[wv], s_flag[wv], lane);
+                                                             tab, tl, s_base[wv], s_end[wv], (uint8_t *)s_flag[wv], lane);
                 if (g != cur_g) {
                     if (cur_g != kEmpty && acc != 0 && lane == 0) atomicAdd(counts + cur_g, acc);
                     cur_g = g;
