@@ -41,6 +41,23 @@ def b_alg(st):
     return 8 * (st["n"] + 1) + 4 * st["m"] + 16 * st["m"] + 4 * (st["sum_out_sq"] + st["sum_in_out"])
 
 
+def committed_traffic(scale):
+    """HBM bytes per k_count launch from the committed rocprofv3 PMC summary of this workload, or None.
+
+    The summary (profiles/k_count_rmat_s<scale>_pmc.txt) holds FETCH_SIZE and WRITE_SIZE per
+    dispatch, each scaled by what the counter reads for a known-size copy kernel in the same run.
+    """
+    p = os.path.join(ROOT, "profiles", f"k_count_rmat_s{scale}_pmc.txt")
+    try:
+        with open(p) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def timed_counts(T, g, steps, warmup, stream, ws, split=True):
     import torch
     out = torch.zeros(1, dtype=torch.int64, device="cuda")
@@ -86,15 +103,10 @@ def cpu_baseline(g, target_s=15.0):
                       f"{edges} oriented edges) counted by the oracle merge counter in {dt:.1f} s"}
 
 
-def partitioned(T, g, k, seed, steps, ws):
-    """RMAT halo parts (k-way seeded random assignment) counted in one persistent launch."""
+def _replay_ms(T, graphs, steps):
+    """Device time of one persistent batched launch of ``graphs`` (hipGraph replay); returns (ms, counts)."""
     import torch
-    part = T.partition_random(g.n, k, seed=seed)
-    t0 = time.perf_counter()
-    parts = [T.halo_extract(g, part, p, mode=0, ws=ws) for p in range(k)]
-    torch.cuda.synchronize()
-    t_halo = time.perf_counter() - t0
-    plan = T.BatchPlan(parts)
+    plan = T.BatchPlan(graphs)
     plan.launch_graph()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -106,9 +118,36 @@ def partitioned(T, g, k, seed, steps, ws):
     ms = e0.elapsed_time(e1) / steps
     per = plan.results()
     plan.close()
+    return ms, per
+
+
+def partitioned(T, g, k, seed, steps, ws):
+    """RMAT halo parts (k-way seeded random assignment) counted in one persistent launch.
+
+    Each part carries owned-source in-lists (split counting); the same parts
+    without them (plain CSR counting) are timed beside.
+    """
+    import torch
+    part = T.partition_random(g.n, k, seed=seed)
+    t0 = time.perf_counter()
+    parts = [T.halo_extract(g, part, p, mode=0, ws=ws) for p in range(k)]
+    torch.cuda.synchronize()
+    t_halo = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for h in parts:
+        T.in_lists(h, ws=ws)
+    torch.cuda.synchronize()
+    t_inl = time.perf_counter() - t0
+    ms, per = _replay_ms(T, parts, steps)
+    for h in parts:
+        h.in_off = h.in_col = h.in_owned = None
+    ms_plain, per_plain = _replay_ms(T, parts, max(1, steps // 2))
+    if per_plain != per:
+        raise SystemExit(f"split part counts {per} differ from plain {per_plain}")
     return {"k": k, "mode": "P u N(P) (reference un-pruned halo)", "ms": round(ms, 3),
             "gteps": round(g.m / (ms / 1e3) / 1e9, 4), "sum_parts": sum(per),
-            "halo_extract_s": round(t_halo, 3), "part_rows": [p.n for p in parts],
+            "plain_csr_ms": round(ms_plain, 3), "halo_extract_s": round(t_halo, 3),
+            "in_lists_s": round(t_inl, 3), "part_rows": [p.n for p in parts],
             "part_owned": [p.owned for p in parts], "part_arcs": [p.m for p in parts]}
 
 
@@ -118,23 +157,24 @@ def mixed_workload(T, seed, ws):
     import torch
     graphs, names = [], []
     n, s, d = T.generate_grid(2000, 2000, 0.3, seed=seed)
-    graphs.append(T.clean_orient(n, s, d, ws=ws)); names.append("grid2000")
+    graphs.append(T.clean_orient(n, s, d, ws=ws, in_lists=True)); names.append("grid2000")
     del s, d
     n, s, d = T.generate_chung_lu(1_000_000, 2.1, 50.0, 5e4, seed=seed, ws=ws)
-    graphs.append(T.clean_orient(n, s, d, ws=ws)); names.append("chung_lu_1M")
+    graphs.append(T.clean_orient(n, s, d, ws=ws, in_lists=True)); names.append("chung_lu_1M")
     del s, d
     rng = np.random.default_rng(seed)
     for i in range(64):
         nn = int(round(10 ** rng.uniform(3, 5)))
         n, s, d = T.generate_er(nn, 20.0, seed=seed * 1000 + i)
-        graphs.append(T.clean_orient(n, s, d, ws=ws)); names.append(f"er{i}_{nn}")
+        graphs.append(T.clean_orient(n, s, d, ws=ws, in_lists=True)); names.append(f"er{i}_{nn}")
     torch.cuda.synchronize()
     return graphs, names
 
 
 def batched_vs_per_graph(T, graphs, reps, ws):
+    """Whole mixed workload: one launch + synchronise + copy-back per graph (default tune) vs
+    one persistent batched launch with the tune the autotuner picks for this batch."""
     import torch
-    # per-graph plain path: one launch + synchronise + copy-back per graph
     for g in graphs[:3]:
         T.count(g, ws=ws)
     torch.cuda.synchronize()
@@ -142,7 +182,8 @@ def batched_vs_per_graph(T, graphs, reps, ws):
     for _ in range(reps):
         per_graph = [T.count(g, ws=ws) for g in graphs]
     t_plain = (time.perf_counter() - t0) / reps
-    plan = T.BatchPlan(graphs)
+    tune, tune_ms, _cached = T.autotune(graphs, reps=3)
+    plan = T.BatchPlan(graphs, tune=tune)
     plan.launch_graph()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -154,7 +195,78 @@ def batched_vs_per_graph(T, graphs, reps, ws):
     return {"graphs": len(graphs), "edges": int(sum(g.m for g in graphs)),
             "per_graph_ms": round(t_plain * 1e3, 3), "batched_ms": round(t_batch * 1e3, 3),
             "speedup": round(t_plain / t_batch, 3), "identical": per_graph == batched,
-            "triangles": int(sum(batched))}
+            "triangles": int(sum(batched)), "tune": tune.as_dict()}
+
+
+def serving(T, n_req, rate, budget_gb, seed, verify):
+    """Serving loop: a Poisson stream of count requests through lstc_server.
+
+    Request i: raw edge count E_i log-uniform in [1e3, 5e6]; 70% RMAT (a/b/c/d
+    .57/.19/.19/.05, scale = max(4, round(log2(E_i / 16)))), 30% Erdos-Renyi
+    (n = E_i / 10, mean degree 20), each with its own seed.  Arrivals are a
+    Poisson process of `rate` requests/s; each request is submitted at its
+    arrival time and its latency runs from submission to the count being
+    available on the host.  The server admits requests against an HBM budget of
+    `budget_gb`, generates and cleans them on one stream and counts ready ones in
+    persistent batches on another.
+
+    verify: "sample" checks every request with E_i <= 2e5 plus every 50th larger
+    one against the CPU oracle (generate, clean, orient, merge count); "all"
+    checks all of them; "none" skips the check.
+    """
+    import concurrent.futures as cf
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    E = np.exp(rng.uniform(np.log(1e3), np.log(5e6), n_req)).astype(np.int64)
+    is_rmat = rng.uniform(size=n_req) < 0.7
+    arrive = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    specs = []
+    for i in range(n_req):
+        e, s = int(E[i]), seed * 1_000_003 + i
+        if is_rmat[i]:
+            specs.append(("rmat", max(4, int(round(np.log2(e / 16)))), e, s))
+        else:
+            specs.append(("er", max(16, e // 10), e, s))
+    srv = T.Server(int(budget_gb * (1 << 30)))
+    ids = []
+    t0 = time.perf_counter()
+    for i, (kind, a, e, s) in enumerate(specs):
+        while time.perf_counter() - t0 < arrive[i]:
+            pass
+        ids.append(srv.submit_rmat(a, e, s) if kind == "rmat" else srv.submit_er(a, e, s))
+    res = [srv.wait(r) for r in ids]
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r["latency_s"] for r in res])
+    edges = int(sum(r["edges"] for r in res))
+    out = {"requests": n_req, "rate_per_s": rate, "hbm_budget_gb": budget_gb,
+           "mix": "70% RMAT / 30% ER, raw edges log-uniform [1e3, 5e6]",
+           "raw_edges": int(E.sum()), "edges": edges, "wall_s": round(wall, 3),
+           "edges_per_s": round(edges / wall, 1), "gteps": round(edges / wall / 1e9, 4),
+           "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+           "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+           "mean_ms": round(float(lat.mean()) * 1e3, 3), "max_ms": round(float(lat.max()) * 1e3, 3),
+           "batches": int(st["batches"]), "max_batch_graphs": int(st["max_batch_graphs"]),
+           "rejected": int(st["rejected"])}
+    if verify != "none":
+        from oracle import oracle as O
+        pick = [i for i in range(n_req) if verify == "all" or E[i] <= 200_000 or i % 50 == 0]
+
+        def check(i):
+            kind, a, e, s = specs[i]
+            n, src, dst = O.gen_rmat(a, 16, seed=s, m_raw=e) if kind == "rmat" else O.gen_er(a, 20.0, seed=s, m_raw=e)
+            _, g = O.clean_orient(n, src, dst)
+            return O.count(g, threads=1) == res[i]["count"] and g.m == res[i]["edges"]
+
+        threads = min(16, os.cpu_count() or 1)
+        t1 = time.perf_counter()
+        with cf.ThreadPoolExecutor(threads) as ex:
+            ok = list(ex.map(check, pick))
+        out["cpu_verified"] = len(pick)
+        out["cpu_mismatches"] = int(len(ok) - sum(ok))
+        out["cpu_verify_s"] = round(time.perf_counter() - t1, 1)
+    return out
 
 
 def main():
@@ -168,6 +280,10 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the partitioned and mixed workloads")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the timed split count (for profiling: no plain, extra or CPU legs)")
+    ap.add_argument("--serve-requests", type=int, default=10000)
+    ap.add_argument("--serve-rate", type=float, default=1000.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--serve-verify", choices=["sample", "all", "none"], default="sample")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run an int32 elementwise copy of this many MiB (reads N, writes N bytes), "
                          "the known-size kernel that calibrates HBM byte counters")
@@ -231,6 +347,12 @@ def main():
         if not extra["mixed"]["identical"]:
             raise SystemExit("batched counts differ from per-graph counts")
         del mg
+        torch.cuda.empty_cache()
+        extra["serving"] = serving(T, args.serve_requests, args.serve_rate, args.serve_budget_gb, args.seed,
+                                   args.serve_verify)
+        log(f"serving: {extra['serving']}")
+        if extra["serving"].get("cpu_mismatches"):
+            raise SystemExit("served counts differ from the CPU oracle")
     cpu = None if args.no_cpu else cpu_baseline(g)
     if cpu:
         log(f"cpu baseline {cpu['value']} GTEPS on {cpu['cores']} threads")
@@ -247,12 +369,13 @@ def main():
                    "plain_csr_count_ms": ms_plain and round(ms_plain, 4),
                    "plain_csr_gteps": ms_plain and round(g.m / (ms_plain / 1e3) / 1e9, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": committed_traffic(args.scale)},
         "cpu_baseline": cpu,
     }
     if extra:
         res["partitioned_s23_k8"] = extra["partitioned"]
         res["batched_vs_per_graph"] = extra["mixed"]
+        res["serving"] = extra["serving"]
     print(json.dumps(res))
 
 

@@ -43,6 +43,13 @@ const char *lstc_status_string(lstc_status st);
 const char *lstc_last_error(void);
 int lstc_version(void); /* major*10000 + minor*100 + patch */
 
+/* Device self-test of the wave64 primitives (DPP prefix sum, reduction, lane
+ * permute, ballot rank) that the counting kernel's addressing is built on: runs
+ * them on fixed patterns on the current device and compares with host
+ * arithmetic.  *mismatches receives the number of wrong lanes (0 = pass).
+ * Diagnostic only; it has no counterpart in the reference.                    */
+lstc_status lstc_selftest(uint32_t *mismatches);
+
 /* ------------------------------------------------------------------ generators
  * Seeded, counter-based (SplitMix64) synthetic inputs written straight into
  * HBM.  The reference reads public datasets from disk instead (README.md:14-18).
@@ -115,7 +122,10 @@ typedef struct lstc_tune {
     uint32_t small_group;  /* small rows per work ticket (4..256), 0 = 16              */
     uint32_t grid_per_cu;  /* persistent workgroups per CU (1..16), 0 = 7              */
     uint32_t unroll;       /* 64-entry chunks in flight per wave (4 or 8), 0 = 4       */
-    uint32_t reserved[3];
+    uint32_t tiny_max;     /* graphs with in-lists: rows with 2 <= d+ <= tiny_max are
+                              counted one lane per arc, comparing the two lists (each
+                              at most tiny_max <= 8 entries) in registers; 1 = off, 0 = 8 */
+    uint32_t reserved[2];
 } lstc_tune;
 
 /* Exact count over owned rows [0, n_owned) of an oriented CSR: the sum over
@@ -133,13 +143,28 @@ lstc_status lstc_count(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_
                        uint32_t n_owned, uint64_t m, uint64_t *d_count, uint64_t *d_per_row,
                        const lstc_tune *tune, void *d_ws, size_t ws_bytes, lstc_stream s);
 
-/* Same count of a whole graph (all rows owned) that also has the target-side
- * in-lists from lstc_clean_orient: every arc (u, w) streams the shorter of
- * N+(u) and N+(w) through an LDS table of the other, about halving the entries
- * streamed on power-law graphs.  Workspace: lstc_count_workspace(n, n, m).   */
+/* Same count (over owned rows [0, n_owned)) of a graph that also has
+ * target-side in-lists, from lstc_clean_orient (whole graphs) or lstc_in_lists
+ * (any CSR, owned sources only): every arc (u, w) streams the shorter of N+(u)
+ * and N+(w) through an LDS table of the other, and rows with d+ <= tiny_max
+ * compare both lists lane-per-arc, which on power-law graphs streams well under
+ * half the entries of lstc_count.  Workspace: lstc_count_workspace(n_rows, ., m). */
 lstc_status lstc_count_split(const uint64_t *d_off, const uint32_t *d_col, const uint64_t *d_in_off,
-                             const uint32_t *d_in_col, uint32_t n_rows, uint64_t m, uint64_t *d_count,
-                             const lstc_tune *tune, void *d_ws, size_t ws_bytes, lstc_stream s);
+                             const uint32_t *d_in_col, uint32_t n_rows, uint32_t n_owned, uint64_t m,
+                             uint64_t *d_count, const lstc_tune *tune, void *d_ws, size_t ws_bytes,
+                             lstc_stream s);
+
+/* Target-side in-lists of an oriented CSR for split counting: for every row w,
+ * the owned sources u < n_owned of arcs u -> w with d+(u) <= d+(w) (row lengths
+ * of this CSR), ascending.  d_in_off[n_rows+1], d_in_col[m] (at most m entries);
+ * *n_entries receives the entry count (the stream is synchronised).  With
+ * n_owned = n_rows this equals the in-lists lstc_clean_orient emits.  Enables
+ * split counting of the halo subgraphs of a partition (owned rows first, as
+ * src/WithPartition/without_sorting_tc.cu:400 counts them).                  */
+lstc_status lstc_in_lists_workspace(uint32_t n_rows, uint32_t n_owned, uint64_t m, size_t *bytes);
+lstc_status lstc_in_lists(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows, uint32_t n_owned,
+                          uint64_t m, uint64_t *d_in_off, uint32_t *d_in_col, uint64_t *n_entries, void *d_ws,
+                          size_t ws_bytes, lstc_stream s);
 
 /* ------------------------------------------------------------------ batching
  * Many graphs (e.g. the k halo parts of a partition, or unrelated graphs of
@@ -152,7 +177,7 @@ typedef struct lstc_graph_desc {
     const uint32_t *col;     /* device, m          */
     uint64_t *per_row;       /* device, n_owned, or NULL */
     const uint64_t *in_off;  /* target-side in-lists (split count) or NULL; used   */
-    const uint32_t *in_col;  /* only when n_owned == n_rows and per_row is NULL    */
+    const uint32_t *in_col;  /* only when per_row is NULL                          */
     uint64_t m;
     uint32_t n_rows;
     uint32_t n_owned;

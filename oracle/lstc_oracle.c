@@ -2,7 +2,7 @@
  * lstc_oracle.c -- CPU restatement of the LSTC triangle-counting path.
  *
  * TEST INFRASTRUCTURE ONLY: only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py load this library, and only as the checker.
+ * cpu_baseline leg of bench.py load this library, and only to check results.
  * The product never links or calls it.
  *
  * Reference files restated (paths relative to the reference root):

@@ -2,8 +2,8 @@
  * lstc_oracle.h -- CPU restatement of the LSTC triangle-counting path.
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (trianglecount_amd/,
- * include/) links, loads or calls this code.  It is the checker used by
- * tests/, by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py.
+ * include/) links, loads or calls this code.  It checks results in tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py.
  *
  * Every function cites the reference file:line whose behaviour it restates.
  * Paths are relative to the reference repository root.

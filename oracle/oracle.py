@@ -1,7 +1,7 @@
 """CPU oracle for the LSTC triangle-counting path (test infrastructure only).
 
 Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` import this module, and only as the checker.  The product
+``bench.py`` import this module, and only to check results.  The product
 (``trianglecount_amd``) never imports it.
 
 Thin numpy/ctypes wrapper over ``oracle/_build/liblstc_oracle.so`` (built from

@@ -1,4 +1,4 @@
-"""Small graph builders with known triangle counts, and brute-force checkers.
+"""Small graph builders with known triangle counts, and brute-force counters.
 
 All builders return ``(n, src, dst)`` raw edge samples (uint32 numpy arrays);
 cleaning (loops, repeats, symmetrisation) is left to the code under test.
@@ -115,3 +115,18 @@ def brute_triangle_list(src, dst):
         if (a, b) in es and (a, c) in es and (b, c) in es:
             out.append((a, b, c))
     return out
+
+
+def owned_in_lists(off, col, n_owned):
+    """In-lists by definition: for w, owned sources u < n_owned of arcs u->w with d+(u) <= d+(w), ascending."""
+    n = len(off) - 1
+    d = np.diff(off.astype(np.int64))
+    src = np.repeat(np.arange(n, dtype=np.int64), d)
+    dst = col.astype(np.int64)
+    keep = (src < n_owned) & (d[src] <= d[dst])
+    s, t = src[keep], dst[keep]
+    order = np.lexsort((s, t))
+    s, t = s[order], t[order]
+    in_off = np.zeros(n + 1, np.uint64)
+    np.add.at(in_off, t + 1, 1)
+    return np.cumsum(in_off).astype(np.uint64), s.astype(np.uint32)

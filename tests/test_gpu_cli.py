@@ -64,10 +64,11 @@ def test_preprocess_writes_reference_formats(lstc, oracle, rmat_files):
 
 def test_tc_matches_oracle(lstc, oracle, rmat_files):
     wd, sym, g = rmat_files
-    out = _run([os.path.join(BIN, "lstc_tc"), "g_output.csr"], str(wd))
-    tri = [int(x) for x in re.findall(r"Triangle : (\d+)", out)]
-    assert tri == [oracle.count(g)] * 3
-    assert "Total avg of 3 RUNS" in out
+    for extra in ([], ["--plain"]):
+        out = _run([os.path.join(BIN, "lstc_tc"), "g_output.csr"] + extra, str(wd))
+        tri = [int(x) for x in re.findall(r"Triangle : (\d+)", out)]
+        assert tri == [oracle.count(g)] * 3
+        assert "Total avg of 3 RUNS" in out
 
 
 @pytest.mark.parametrize("prune", [False, True])
@@ -89,7 +90,7 @@ def test_halo_and_partitioned_count(lstc, oracle, rmat_files, prune):
         assert vals[pos: pos + ml] == h.csr.col.tolist()
         pos += ml
     want = oracle.count(g)
-    for extra in ([], ["--per-part"]):
+    for extra in ([], ["--per-part"], ["--plain"], ["--per-part", "--plain"]):
         out = _run([os.path.join(BIN, "lstc_part_tc"), "g_part.csr_node_parts_random_4_output.csr", "4"] + extra,
                    str(wd))
         assert int(re.search(r"Total Triangle : (\d+)", out).group(1)) == want

@@ -96,6 +96,9 @@ def test_invalid_arguments_raise(lstc, oracle):
     ("star", lambda O: G.star(9000)),
     ("chung_lu", lambda O: O.gen_chung_lu(60000, 2.1, 50, 2e4, seed=4)),
     ("rmat_s15", lambda O: O.gen_rmat(15, 16, seed=8)),
+    ("grid", lambda O: O.gen_grid(300, 200, 0.3, 3)),
+    ("er", lambda O: O.gen_er(30000, 12, seed=9)),
+    ("toy_edge_filter_cu", lambda O: G.toy_edge_filter_cu()),
 ])
 def test_split_count_matches_oracle(lstc, oracle, name, builder):
     n, s, d = builder(oracle)
@@ -106,8 +109,25 @@ def test_split_count_matches_oracle(lstc, oracle, name, builder):
     # complete graphs: closed form n(n-1)(n-2)/6 (the merge oracle is slow at K3000)
     want = n * (n - 1) * (n - 2) // 6 if name.startswith("K") else oracle.count(og)
     assert lstc.count(g) == want, name
-    for t in (lstc.Tune(table_scale=2), lstc.Tune(table_scale=4, small_max=64), lstc.Tune(unroll=8)):
+    for t in (lstc.Tune(table_scale=2), lstc.Tune(table_scale=4, small_max=64), lstc.Tune(unroll=8),
+              lstc.Tune(tiny_max=1), lstc.Tune(tiny_max=2), lstc.Tune(tiny_max=5, small_group=4)):
         assert lstc.count(g, tune=t) == want, (name, t.as_dict())
     # a batch mixing split and plain graphs
     plain = lstc.Graph.from_numpy(og.off, og.col)
     assert lstc.count_batch([g, plain, g]) == [want, want, want]
+
+
+@pytest.mark.parametrize("cut_frac", [0.0, 0.3, 1.0])
+def test_in_lists_of_any_csr(lstc, oracle, cut_frac):
+    """lstc_in_lists on an uploaded CSR with n_owned rows: definition, and split = plain count."""
+    _, og = oracle.clean_orient(*oracle.gen_rmat(13, 16, seed=12))
+    cut = int(og.n * cut_frac)
+    g = lstc.Graph.from_numpy(og.off, og.col, n_owned=cut)
+    lstc.in_lists(g)
+    want_off, want_col = G.owned_in_lists(og.off, og.col, cut)
+    assert np.array_equal(g.in_off.cpu().numpy().view(np.uint64), want_off)
+    assert np.array_equal(g.in_col[: len(want_col)].cpu().numpy().view(np.uint32), want_col)
+    want = oracle.count(og, n_owned=cut)
+    assert lstc.count(g) == want
+    # in-lists built for another n_owned are not used
+    assert lstc.count(g, n_owned=og.n) == oracle.count(og)

@@ -62,3 +62,36 @@ def test_halo_empty_part(lstc, oracle):
     assert h.n == 0 and h.m == 0 and h.n_owned == 0
     h = lstc.halo_extract(g, part, 0)
     assert h.n == 20 and h.n_owned == 20 and lstc.count(h) == 20 * 19 * 18 // 6
+
+
+@pytest.mark.parametrize("name,builder,k", [
+    ("rmat_s13", lambda O: O.gen_rmat(13, 16, seed=6), 8),
+    ("grid", lambda O: O.gen_grid(40, 40, 0.3, 4), 4),
+    ("chung_lu", lambda O: O.gen_chung_lu(20000, 2.1, 30, 4e3, seed=3), 3),
+])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_halo_split_parts_match_oracle(lstc, oracle, name, builder, k, mode):
+    """Halo parts with owned-source in-lists (lstc_in_lists) counted split, part by part and batched."""
+    n, s, d = builder(oracle)
+    g = _device_graph(lstc, n, s, d)
+    sym, og = oracle.clean_orient(n, s, d)
+    whole = oracle.count(og)
+    part = lstc.partition_random(g.n, k, seed=7)
+    opart = oracle.partition_random(sym.n, k, seed=7)
+    parts, acc = [], 0
+    for p in range(k):
+        h = lstc.halo_extract(g, part, p, mode=mode, split=True)
+        oh = oracle.halo_extract(sym, opart, p, mode)
+        want_off, want_col = G.owned_in_lists(oh.csr.off, oh.csr.col, oh.n_owned)
+        assert np.array_equal(h.in_off.cpu().numpy().view(np.uint64), want_off), (name, p)
+        assert h.meta["in_entries"] == len(want_col)
+        assert np.array_equal(h.in_col[: len(want_col)].cpu().numpy().view(np.uint32), want_col), (name, p)
+        want = oracle.count(oh.csr, n_owned=oh.n_owned)
+        assert lstc.count(h) == want, (name, p)
+        assert lstc.count(h, split=False) == want, (name, p)
+        for t in (lstc.Tune(tiny_max=1), lstc.Tune(tiny_max=4), lstc.Tune(unroll=8, table_scale=2)):
+            assert lstc.count(h, tune=t) == want, (name, p, t.as_dict())
+        parts.append(h)
+        acc += want
+    assert acc == whole
+    assert lstc.count_batch(parts) == [lstc.count(h) for h in parts]

@@ -94,21 +94,6 @@ def test_device_pipeline_rmat_s16(lstc, oracle):
     assert lstc.count(g) == oracle.count(og)
 
 
-def _in_lists_numpy(off, col):
-    """Target-side in-lists by definition: for w, sources u of arcs u->w with d+(u) <= d+(w), ascending."""
-    n = len(off) - 1
-    d = np.diff(off.astype(np.int64))
-    src = np.repeat(np.arange(n, dtype=np.int64), d)
-    dst = col.astype(np.int64)
-    keep = d[src] <= d[dst]
-    s, t = src[keep], dst[keep]
-    order = np.lexsort((s, t))
-    s, t = s[order], t[order]
-    in_off = np.zeros(n + 1, np.uint64)
-    np.add.at(in_off, t + 1, 1)
-    return np.cumsum(in_off).astype(np.uint64), s.astype(np.uint32)
-
-
 @pytest.mark.parametrize("name,builder", [
     ("toy_edge_filter_cu", lambda O: G.toy_edge_filter_cu()),
     ("K120", lambda O: G.complete(120)),
@@ -122,7 +107,7 @@ def test_in_lists_bit_exact(lstc, oracle, name, builder):
     dd = torch.from_numpy(np.ascontiguousarray(d, np.uint32).view(np.int32)).cuda()
     g = lstc.clean_orient(n, ds, dd, in_lists=True)
     _, og = oracle.clean_orient(n, s, d)
-    want_off, want_col = _in_lists_numpy(og.off, og.col)
+    want_off, want_col = G.owned_in_lists(og.off, og.col, og.n)
     got_off = g.in_off[: g.n + 1].cpu().numpy().view(np.uint64)
     got_col = g.in_col[: int(want_off[-1])].cpu().numpy().view(np.uint32)
     assert np.array_equal(got_off, want_off)

@@ -264,3 +264,23 @@ def test_chung_lu_shape(oracle):
     deg = np.bincount(np.concatenate([s, d]), minlength=n)
     assert abs(deg.mean() - 50) < 3
     assert 0.5 * 5e4 < deg.max() < 1.5 * 5e4
+
+
+@pytest.mark.parametrize("cut_frac", [0.0, 0.4, 1.0])
+def test_owned_in_lists_helper_by_brute_force(oracle, cut_frac):
+    """The in-list definition the GPU tests compare with, checked by plain loops."""
+    _, g = oracle.clean_orient(*G.random_multigraph(300, 2500, seed=4))
+    cut = int(g.n * cut_frac)
+    off, col = g.off.astype(np.int64), g.col.astype(np.int64)
+    dplus = [int(off[u + 1] - off[u]) for u in range(g.n)]
+    lists = [[] for _ in range(g.n)]
+    for u in range(cut):
+        for i in range(off[u], off[u + 1]):
+            w = int(col[i])
+            if dplus[u] <= dplus[w]:
+                lists[w].append(u)
+    want_off = np.cumsum([0] + [len(x) for x in lists]).astype(np.uint64)
+    want_col = np.array([u for x in lists for u in sorted(x)], np.uint32)
+    got_off, got_col = G.owned_in_lists(g.off, g.col, cut)
+    assert np.array_equal(got_off, want_off)
+    assert np.array_equal(got_col, want_col)

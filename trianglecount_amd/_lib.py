@@ -33,7 +33,7 @@ class Tune(C.Structure):
     """lstc_tune (include/lstc.h)."""
     _fields_ = [("table_scale", C.c_uint32), ("small_max", C.c_uint32),
                 ("small_group", C.c_uint32), ("grid_per_cu", C.c_uint32),
-                ("unroll", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("unroll", C.c_uint32), ("tiny_max", C.c_uint32), ("reserved", C.c_uint32 * 2)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "reserved"}
@@ -80,6 +80,7 @@ SIGNATURES = {
     "lstc_status_string": (C.c_char_p, [ST]),
     "lstc_last_error": (C.c_char_p, []),
     "lstc_version": (C.c_int, []),
+    "lstc_selftest": (ST, [PU32]),
     "lstc_gen_rmat": (ST, [U32, U64, DBL, DBL, DBL, U64, VP, VP, VP]),
     "lstc_gen_er": (ST, [U32, U64, U64, VP, VP, VP]),
     "lstc_gen_chung_lu_workspace": (ST, [U32, PSZ]),
@@ -93,7 +94,9 @@ SIGNATURES = {
     "lstc_halo_extract": (ST, [VP, VP, U32, VP, U32, I32, VP, VP, VP, U32, U64, VP, SZ, VP]),
     "lstc_count_workspace": (ST, [U32, U32, U64, C.POINTER(Tune), PSZ]),
     "lstc_count": (ST, [VP, VP, U32, U32, U64, VP, VP, C.POINTER(Tune), VP, SZ, VP]),
-    "lstc_count_split": (ST, [VP, VP, VP, VP, U32, U64, VP, C.POINTER(Tune), VP, SZ, VP]),
+    "lstc_count_split": (ST, [VP, VP, VP, VP, U32, U32, U64, VP, C.POINTER(Tune), VP, SZ, VP]),
+    "lstc_in_lists_workspace": (ST, [U32, U32, U64, PSZ]),
+    "lstc_in_lists": (ST, [VP, VP, U32, U32, U64, VP, VP, PU64, VP, SZ, VP]),
     "lstc_count_batch_workspace": (ST, [C.POINTER(GraphDesc), U32, C.POINTER(Tune), PSZ]),
     "lstc_count_batch": (ST, [C.POINTER(GraphDesc), U32, VP, C.POINTER(Tune), VP, SZ, VP]),
     "lstc_batch_plan_create": (ST, [C.POINTER(GraphDesc), U32, C.POINTER(Tune), C.POINTER(VP)]),

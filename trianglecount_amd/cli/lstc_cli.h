@@ -5,6 +5,7 @@
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 #include <sys/time.h>
 
 #include <vector>
@@ -67,5 +68,25 @@ struct Workspace {
         if (p) (void)hipFree(p);
     }
 };
+
+// Owned-source target-side in-lists of an uploaded CSR (lstc_in_lists), so the
+// graph is counted split; the caller frees *in_off and *in_col.
+inline void build_in_lists(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows, uint32_t n_owned,
+                           uint64_t m, uint64_t **in_off, uint32_t **in_col, Workspace &ws) {
+    CLI_HIP(hipMalloc((void **)in_off, sizeof(uint64_t) * ((size_t)n_rows + 1)));
+    CLI_HIP(hipMalloc((void **)in_col, sizeof(uint32_t) * (m ? m : 1)));
+    size_t need = 0;
+    CLI_CHECK(lstc_in_lists_workspace(n_rows, n_owned, m, &need));
+    void *w = ws.get(need);
+    uint64_t entries = 0;
+    CLI_CHECK(lstc_in_lists(d_off, d_col, n_rows, n_owned, m, *in_off, *in_col, &entries, w, ws.bytes, nullptr));
+}
+
+// True when argv holds the flag.
+inline bool has_flag(int argc, char **argv, int from, const char *flag) {
+    for (int i = from; i < argc; ++i)
+        if (strcmp(argv[i], flag) == 0) return true;
+    return false;
+}
 
 }  // namespace lstc_cli

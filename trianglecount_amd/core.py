@@ -83,6 +83,12 @@ class Graph:
     meta: dict = field(default_factory=dict)
     in_off: torch.Tensor | None = None  # target-side in-lists (split count), optional
     in_col: torch.Tensor | None = None
+    in_owned: int | None = None  # the n_owned the in-lists were built for (owned sources only)
+
+    def split_ready(self, n_owned: int | None = None) -> bool:
+        """True when the in-lists fit a count over the first ``n_owned`` rows."""
+        no = self.owned if n_owned is None else int(n_owned)
+        return self.in_off is not None and self.in_owned == no
 
     @property
     def owned(self) -> int:
@@ -196,7 +202,24 @@ def clean_orient(n: int, src: torch.Tensor, dst: torch.Tensor, compact: bool = T
           "lstc_clean_orient")
     g = Graph(n_out.value, m_out.value, off, col, sym_off, sym_col)
     if in_lists:
-        g.in_off, g.in_col = in_off, in_col
+        g.in_off, g.in_col, g.in_owned = in_off, in_col, g.n
+    return g
+
+
+def in_lists(g: Graph, ws: Workspace | None = None, stream=None) -> Graph:
+    """Attach target-side in-lists (owned sources only, lstc_in_lists) so ``g`` is counted split."""
+    L = load()
+    no = g.owned
+    nb = C.c_size_t()
+    check(L.lstc_in_lists_workspace(g.n, no, g.m, C.byref(nb)), "lstc_in_lists_workspace")
+    wp, wb = (ws or _default_ws).get(nb.value)
+    in_off = empty_u64(g.n + 1)
+    in_col = empty_u32(max(g.m, 1))
+    k = C.c_uint64()
+    check(L.lstc_in_lists(_ptr(g.off), _ptr(g.col), g.n, no, g.m, _ptr(in_off), _ptr(in_col), C.byref(k), wp, wb,
+                          C.c_void_p(stream_handle(stream))), "lstc_in_lists")
+    g.in_off, g.in_col, g.in_owned = in_off, in_col, no
+    g.meta["in_entries"] = k.value
     return g
 
 
@@ -206,9 +229,9 @@ def count(g: Graph, n_owned: int | None = None, per_row: bool = False, tune: Tun
           sync: bool = True, split: bool = True):
     """Exact count over owned rows; returns int (and the per-row tensor if asked).
 
-    A whole graph carrying in-lists is counted with lstc_count_split unless
-    ``split`` is False.  With ``sync=False`` the device result tensor is
-    returned without waiting.
+    A graph carrying in-lists built for this ``n_owned`` is counted with
+    lstc_count_split unless ``split`` is False.  With ``sync=False`` the device
+    result tensor is returned without waiting.
     """
     L = load()
     no = g.owned if n_owned is None else int(n_owned)
@@ -218,8 +241,8 @@ def count(g: Graph, n_owned: int | None = None, per_row: bool = False, tune: Tun
     wp, wb = (ws or _default_ws).get(nb.value)
     res = out if out is not None else empty_u64(1)
     pr = empty_u64(no) if per_row else None
-    if g.in_off is not None and not per_row and no == g.n and split:
-        check(L.lstc_count_split(_ptr(g.off), _ptr(g.col), _ptr(g.in_off), _ptr(g.in_col), g.n, g.m,
+    if g.split_ready(no) and not per_row and split:
+        check(L.lstc_count_split(_ptr(g.off), _ptr(g.col), _ptr(g.in_off), _ptr(g.in_col), g.n, no, g.m,
                                  _ptr(res), tp, wp, wb, C.c_void_p(stream_handle(stream))), "lstc_count_split")
     else:
         check(L.lstc_count(_ptr(g.off), _ptr(g.col), g.n, no, g.m, _ptr(res), _ptr(pr), tp, wp, wb,
@@ -235,11 +258,12 @@ def count(g: Graph, n_owned: int | None = None, per_row: bool = False, tune: Tun
 
 # ------------------------------------------------------------------ (b) halo
 def halo_extract(g: Graph, part: torch.Tensor, p: int, mode: int = 0, with_l2g: bool = False,
-                 ws: Workspace | None = None, stream=None) -> Graph:
+                 ws: Workspace | None = None, stream=None, split: bool = False) -> Graph:
     """Halo subgraph of part ``p`` from the symmetric CSR kept in ``g`` (keep_symmetric=True).
 
     Returns an oriented CSR whose first ``n_owned`` rows are the owned vertices;
-    ``meta['l2g']`` holds the local -> global map when asked.
+    ``meta['l2g']`` holds the local -> global map when asked.  ``split`` also
+    builds its in-lists (lstc_in_lists) so the part is counted split.
     """
     if g.sym_off is None:
         raise ValueError("halo_extract needs the symmetric CSR: clean_orient(..., keep_symmetric=True)")
@@ -259,6 +283,8 @@ def halo_extract(g: Graph, part: torch.Tensor, p: int, mode: int = 0, with_l2g: 
     h = Graph(nl.value, ml.value, off, col, n_owned=no.value)
     if with_l2g:
         h.meta["l2g"] = l2g[: nl.value]
+    if split:
+        in_lists(h, ws=ws, stream=stream)
     return h
 
 
@@ -270,7 +296,7 @@ def _descs(graphs, per_row=None):
         arr[i].off = g.off.data_ptr()
         arr[i].col = g.col.data_ptr()
         arr[i].per_row = per_row[i].data_ptr() if per_row is not None else None
-        if g.in_off is not None:
+        if g.split_ready():
             arr[i].in_off = g.in_off.data_ptr()
             arr[i].in_col = g.in_col.data_ptr()
         arr[i].m = g.m
@@ -405,6 +431,15 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------ diagnostics
+def selftest() -> int:
+    """Run lstc_selftest on the current device; returns the number of wrong lanes (0 = pass)."""
+    _dev()
+    bad = C.c_uint32()
+    check(load().lstc_selftest(C.byref(bad)), "lstc_selftest")
+    return bad.value
 
 
 # ------------------------------------------------------------------ autotuning

@@ -29,15 +29,18 @@ static lstc_status to_hosts(const lstc_graph_desc *graphs, uint32_t G, std::vect
         LSTC_TRY(validate_graph(hosts[g], "batch"));
         devs[g] = to_dev(hosts[g]);
         base[g] = acc;
-        acc += d.n_owned;
+        acc += plan_rows(hosts[g]);
     }
     base[G] = acc;
     return LSTC_OK;
 }
 
-static uint64_t owned_total(const lstc_graph_desc *graphs, uint32_t G) {
+static uint64_t planned_total(const lstc_graph_desc *graphs, uint32_t G) {
     uint64_t t = 0;
-    for (uint32_t g = 0; g < G; ++g) t += graphs[g].n_owned;
+    for (uint32_t g = 0; g < G; ++g) {
+        const lstc_graph_desc &d = graphs[g];
+        t += plan_rows(HostGraph{d.off, d.col, d.per_row, d.n_rows, d.n_owned, d.m, d.in_off, d.in_col});
+    }
     return t;
 }
 
@@ -80,7 +83,7 @@ extern "C" lstc_status lstc_count_batch_workspace(const lstc_graph_desc *graphs,
     (void)tune;
     if (!bytes) return fail(LSTC_ERR_INVALID, "lstc_count_batch_workspace: NULL bytes");
     if (G && !graphs) return fail(LSTC_ERR_INVALID, "lstc_count_batch_workspace: NULL graph array");
-    *bytes = table_bytes(G) + count_ws_bytes(G, owned_total(graphs, G), arcs_total(graphs, G));
+    *bytes = table_bytes(G) + count_ws_bytes(G, planned_total(graphs, G), arcs_total(graphs, G));
     return LSTC_OK;
 }
 

@@ -11,9 +11,9 @@ using namespace lstc;
 
 extern "C" lstc_status lstc_count_workspace(uint32_t n_rows, uint32_t n_owned, uint64_t m,
                                             const lstc_tune *tune, size_t *bytes) {
-    (void)n_rows; (void)tune;
+    (void)n_owned; (void)tune;
     if (!bytes) return fail(LSTC_ERR_INVALID, "lstc_count_workspace: bytes is NULL");
-    *bytes = count_ws_bytes(1, n_owned, m);
+    *bytes = count_ws_bytes(1, n_rows, m);  // split counting plans every row
     return LSTC_OK;
 }
 
@@ -28,11 +28,12 @@ extern "C" lstc_status lstc_count(const uint64_t *d_off, const uint32_t *d_col, 
 }
 
 extern "C" lstc_status lstc_count_split(const uint64_t *d_off, const uint32_t *d_col, const uint64_t *d_in_off,
-                                        const uint32_t *d_in_col, uint32_t n_rows, uint64_t m, uint64_t *d_count,
-                                        const lstc_tune *tune, void *d_ws, size_t ws_bytes, lstc_stream s) {
+                                        const uint32_t *d_in_col, uint32_t n_rows, uint32_t n_owned, uint64_t m,
+                                        uint64_t *d_count, const lstc_tune *tune, void *d_ws, size_t ws_bytes,
+                                        lstc_stream s) {
     if (!d_count) return fail(LSTC_ERR_INVALID, "lstc_count_split: NULL output");
     if (!d_in_off || !d_in_col) return fail(LSTC_ERR_INVALID, "lstc_count_split: NULL in-lists");
-    HostGraph h{d_off, d_col, nullptr, n_rows, n_rows, m, d_in_off, d_in_col};
+    HostGraph h{d_off, d_col, nullptr, n_rows, n_owned, m, d_in_off, d_in_col};
     LSTC_TRY(validate_graph(h, "lstc_count_split"));
     return count_launch(&h, 1, nullptr, nullptr, (unsigned long long *)d_count, tune, d_ws, ws_bytes,
                         (hipStream_t)s);

@@ -38,9 +38,13 @@ struct HostGraph {
 };
 
 // Split counting (in-lists present) applies only to whole graphs without per-row output.
-inline bool split_ok(const HostGraph &h) {
-    return h.in_off && h.in_col && !h.per_row && h.n_owned == h.n_rows;
-}
+// Split counting needs the in-lists and no per-row output (a row's count is
+// spread over both sides).  With n_owned < n_rows the in-lists must hold only
+// owned sources (lstc_in_lists does that).
+inline bool split_ok(const HostGraph &h) { return h.in_off && h.in_col && !h.per_row; }
+// Rows the planner visits: every row for split counting (in-list items of halo
+// rows), else the owned rows.
+inline uint64_t plan_rows(const HostGraph &h) { return split_ok(h) ? h.n_rows : h.n_owned; }
 inline GraphDev to_dev(const HostGraph &h) {
     const bool sp = split_ok(h);
     return GraphDev{h.off, h.col, (unsigned long long *)h.per_row, sp ? h.in_off : nullptr, sp ? h.in_col : nullptr,
@@ -54,6 +58,7 @@ struct TuneR {
     uint32_t small_group;   // small rows per ticket
     uint32_t grid_per_cu;   // persistent workgroups per CU
     uint32_t unroll;        // chunks in flight per wave (4 or 8)
+    uint32_t tiny_max;      // rows with 2 <= d+ <= tiny_max (graphs with in-lists): lane-per-arc path
 };
 TuneR resolve_tune(const lstc_tune *t);
 

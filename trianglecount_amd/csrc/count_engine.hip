@@ -48,10 +48,14 @@ static constexpr uint32_t kBigLog2 = 12;
 static constexpr uint32_t kBigCap = 1u << kBigLog2;   // shared table (big items)
 static constexpr uint32_t kBigTile = kBigCap / 2;     // keys per table fill of a big row
 static constexpr uint32_t kSegChunk = 4096;           // in-neighbour segments per kind B item
-static constexpr int kBins = 6;
+static constexpr int kBins = 6;                       // hash-path bins
+static constexpr int kTinyBin = kBins;                // the lane-per-arc rows
+static constexpr int kSlots = kBins + 1;              // queue regions
+static constexpr uint32_t kTinyMax = 8;               // largest d+ of the lane-per-arc path
 
 // ctl (uint32) layout
-enum { C_CNT = 0, C_CUR = 6, C_HEAD = 12, C_NBIG = 13, C_NSMALL = 14, C_NTICK = 15, C_WORDS = 16 };
+enum { C_CNT = 0, C_HEAD = 12, C_NBIG = 13, C_NSMALL = 14, C_NTICK = 15, C_NTINY = 16, C_TINY0 = 17, C_NSMALLT = 18,
+       C_WORDS = 19 };
 
 // work item: row (bits 0-31) | chunk (32-51) | kind (52) | graph (53-63).
 // A row's in-degree is below 2^32 (m < 2^32), so its chunk index fits 20 bits.
@@ -68,8 +72,9 @@ __device__ __forceinline__ Item decode(unsigned long long it) {
 TuneR resolve_tune(const lstc_tune *t) {
     // defaults: the fastest point of a (table_scale, small_max, grid_per_cu, unroll)
     // sweep at RMAT scales 21 and 23 on MI355X
-    TuneR r{8, 256, 16, 0, 4};
+    TuneR r{8, 256, 16, 0, 4, kTinyMax};
     if (t) {
+        if (t->tiny_max) r.tiny_max = t->tiny_max == 1 ? 0u : t->tiny_max;  // 1 = off
         if (t->table_scale) r.table_scale = t->table_scale;
         if (t->small_max) r.small_max = t->small_max;
         if (t->small_group) r.small_group = t->small_group;
@@ -83,6 +88,8 @@ TuneR resolve_tune(const lstc_tune *t) {
     r.unroll = r.unroll >= 8 ? 8 : 4;
     if (r.grid_per_cu == 0) r.grid_per_cu = 7;  // LDS-limited occupancy: 7 workgroups per CU
     r.grid_per_cu = hmax(1u, hmin(16u, r.grid_per_cu));
+    r.tiny_max = hmin(kTinyMax, r.tiny_max);
+    if (r.tiny_max < 2) r.tiny_max = 0;
     return r;
 }
 
@@ -99,10 +106,26 @@ lstc_status validate_graph(const HostGraph &h, const char *who) {
 
 struct Bounds {
     uint32_t hi[kBins];  // bin b holds d+ in (hi[b-1], hi[b]], bin 0 starts at 2
+    uint32_t tiny_max;   // rows of graphs with in-lists and 2 <= d+ <= tiny_max go to the tiny region
 };
 
+// UNIFORM: g is the same in every lane, so the entry is read through the
+// scalar cache into scalar registers.
+template <bool UNIFORM = false>
 __device__ __forceinline__ GraphDev graph_at(const GraphSet &gs, uint32_t g) {
     if (!gs.table) return gs.one;
+    if (UNIFORM) {
+        const LSTC_CONST_AS GraphDev *q = (const LSTC_CONST_AS GraphDev *)(gs.table + g);
+        GraphDev d;
+        d.off = q->off;
+        d.col = q->col;
+        d.per_row = q->per_row;
+        d.in_off = q->in_off;
+        d.in_col = q->in_col;
+        d.n_rows = q->n_rows;
+        d.n_owned = q->n_owned;
+        return d;
+    }
     const LSTC_GLOBAL_AS GraphDev *q = (const LSTC_GLOBAL_AS GraphDev *)(gs.table + g);
     GraphDev d;
     d.off = q->off;
@@ -121,6 +144,8 @@ __device__ __forceinline__ uint32_t table_log2_for(uint32_t keys, uint32_t scale
     return l;
 }
 
+__device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+
 __device__ __forceinline__ uint32_t hslot(uint32_t x, uint32_t shift) { return (x * 0x9E3779B1u) >> shift; }
 
 // ------------------------------------------------------------------ planning
@@ -129,23 +154,35 @@ __global__ void k_plan_init(uint32_t *ctl, unsigned long long *counts, uint32_t 
     for (uint32_t i = threadIdx.x; i < G; i += blockDim.x) counts[i] = 0;
 }
 
-// pass 0 counts items per bin; pass 1 writes them, laid out bins 5,4,3 (big,
-// largest first) then 2,1,0 (small, largest first).
+// The plan splits the owned rows of the launch (graph after graph) into
+// kPlanWaves contiguous ranges, one per wave of the plan grid.  Pass 0 counts
+// each range's items per bin into wcnt[slot][wave]; k_plan_scan turns that into
+// exclusive queue positions (slot-major: bins 5,4,3 = big, largest first, then
+// 2,1,0 = small); pass 1 writes the items, in row order within a bin and range.
+// No atomics: the queue is the same on every run.
+static constexpr uint32_t kPlanBlocks = 1024;
+static constexpr uint32_t kPlanWaves = kPlanBlocks * 4;
+static constexpr uint32_t kPlanEntries = kSlots * kPlanWaves;
+
+// queue region of a bin: hash bins 5..0 in slots 0..5, the tiny rows in slot 6
+__device__ __forceinline__ int slot_of(int b) { return b < kBins ? kBins - 1 - b : kTinyBin; }
+
 template <int PASS>
-__global__ __launch_bounds__(256) void k_plan_rows(GraphSet gs, Bounds bd, uint32_t *__restrict__ ctl,
+__global__ __launch_bounds__(256) void k_plan_rows(GraphSet gs, Bounds bd, uint32_t *__restrict__ wcnt,
                                                    unsigned long long *__restrict__ queue) {
     const uint32_t lane = lane_id();
-    uint32_t bin_off[kBins];
-    if (PASS == 1) {
-        uint32_t acc = 0;
-        for (int b = kBins - 1; b >= 0; --b) { bin_off[b] = acc; acc += ctl[C_CNT + b]; }
-    }
+    const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint64_t total = gs.total_rows;
-    for (uint64_t r0 = (uint64_t)blockIdx.x * blockDim.x; r0 < total; r0 += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t r = r0 + threadIdx.x;
+    const uint64_t chunk = (total + kPlanWaves - 1) / kPlanWaves;
+    const uint64_t r_begin = umin64(total, (uint64_t)wave * chunk), r_end = umin64(total, r_begin + chunk);
+    uint32_t run[kSlots];
+#pragma unroll
+    for (int i = 0; i < kSlots; ++i) run[i] = PASS == 1 ? wcnt[slot_of(i) * kPlanWaves + wave] : 0u;
+    for (uint64_t r0 = r_begin; r0 < r_end; r0 += 64) {
+        const uint64_t r = r0 + lane;
         int b = -1;
-        uint32_t g = 0, u = 0, n_b = 0;
-        if (r < total) {
+        uint32_t g = 0, u = 0, n_a = 0, n_b = 0, tiny_kind = 0;
+        if (r < r_end) {
             if (gs.table) {  // last g with row_base[g] <= r
                 uint32_t lo = 0, hi = gs.G;
                 while (hi - lo > 1) {
@@ -158,43 +195,102 @@ __global__ __launch_bounds__(256) void k_plan_rows(GraphSet gs, Bounds bd, uint3
                 u = (uint32_t)r;
             }
             const GraphDev G = graph_at(gs, g);
+            const bool owned = u < G.n_owned;  // rows past n_owned are visited only for split counting
             const uint64_t k = gload(G.off + u + 1) - gload(G.off + u);
-            if (k >= 2) {
-                b = kBins - 1;
-                for (int i = kBins - 2; i >= 0; --i)
-                    if (k <= bd.hi[i]) b = i;
-                if (G.in_off) {
-                    const uint64_t kb = gload(G.in_off + u + 1) - gload(G.in_off + u);
-                    n_b = (uint32_t)((kb + kSegChunk - 1) / kSegChunk);
+            const uint64_t kb = G.in_off ? gload(G.in_off + u + 1) - gload(G.in_off + u) : 0;
+            if (k >= 2 && G.in_off && k <= bd.tiny_max) {
+                // one entry; kind 1 = in side only (a halo row owns no out side)
+                if (owned || kb) {
+                    b = kTinyBin;
+                    tiny_kind = owned ? 0u : 1u;
+                }
+            } else if (k >= 2) {
+                n_a = owned ? 1u : 0u;
+                n_b = (uint32_t)((kb + kSegChunk - 1) / kSegChunk);
+                if (n_a + n_b) {
+                    b = kBins - 1;
+                    for (int i = kBins - 2; i >= 0; --i)
+                        if (k <= bd.hi[i]) b = i;
                 }
             }
         }
-        const uint32_t n_items = (b >= 0) ? 1u + n_b : 0u;
+        const uint32_t n_items = b == kTinyBin ? 1u : n_a + n_b;
 #pragma unroll
-        for (int i = 0; i < kBins; ++i) {
+        for (int i = 0; i < kSlots; ++i) {
             if (ballot(b == i) == 0) continue;
             const uint32_t c = (b == i) ? n_items : 0u;
-            const uint32_t incl = wave_incl_scan(c, lane);
-            const uint32_t tot = read_lane(incl, 63);
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(ctl + (PASS == 0 ? C_CNT : C_CUR) + i, tot);
-            base = read_lane(base, 0);
-            if (PASS == 1 && b == i) {
-                unsigned long long *q = queue + bin_off[i] + base + (incl - c);
-                q[0] = make_item(g, 0, 0, u);
-                for (uint32_t j = 0; j < n_b; ++j) q[1 + j] = make_item(g, 1, j, u);
+            if (PASS == 0) {
+                run[i] += c;
+            } else {
+                const uint32_t incl = wave_incl_scan(c, lane);
+                if (b == i) {
+                    unsigned long long *q = queue + run[i] + (incl - c);
+                    if (b == kTinyBin) {
+                        q[0] = make_item(g, tiny_kind, 0, u);
+                    } else {
+                        if (n_a) *q++ = make_item(g, 0, 0, u);
+                        for (uint32_t j = 0; j < n_b; ++j) q[j] = make_item(g, 1, j, u);
+                    }
+                }
+                run[i] += read_lane(incl, 63);
             }
+        }
+    }
+    if (PASS == 0) {
+#pragma unroll
+        for (int i = 0; i < kSlots; ++i) {
+            const uint32_t sum = wave_sum32(run[i]);
+            if (lane == 0) wcnt[slot_of(i) * kPlanWaves + wave] = sum;
         }
     }
 }
 
-__global__ void k_plan_finish(uint32_t *ctl, uint32_t small_group) {
-    if (threadIdx.x == 0) {
-        const uint32_t n_big = ctl[C_CNT + 3] + ctl[C_CNT + 4] + ctl[C_CNT + 5];
-        const uint32_t n_small = ctl[C_CNT + 0] + ctl[C_CNT + 1] + ctl[C_CNT + 2];
+// One workgroup of 1024: exclusive scan of wcnt in place (kPlanEntries values,
+// 24 per thread), then the ticket counts.  Item totals stay below 2^32
+// (checked on the host against the queue capacity).
+__global__ __launch_bounds__(1024) void k_plan_scan(uint32_t *__restrict__ wcnt, uint32_t *__restrict__ ctl,
+                                                    uint32_t small_group) {
+    constexpr uint32_t kPer = kPlanEntries / 1024;
+    static_assert(kPlanEntries % 1024 == 0, "plan entries per thread");
+    __shared__ uint32_t s_wave[16];
+    __shared__ uint32_t s_slot[kSlots + 1];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    uint32_t v[kPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kPer; ++i) {
+        v[i] = wcnt[tid * kPer + i];
+        sum += v[i];
+    }
+    const uint32_t incl = wave_incl_scan(sum, lane);
+    if (lane == 63) s_wave[wv] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wv; ++w) before += s_wave[w];
+    uint32_t run = before + incl - sum;
+#pragma unroll
+    for (uint32_t i = 0; i < kPer; ++i) {
+        const uint32_t e = tid * kPer + i;
+        if (e % kPlanWaves == 0) s_slot[e / kPlanWaves] = run;
+        wcnt[e] = run;
+        run += v[i];
+    }
+    if (tid == 1023) s_slot[kSlots] = run;
+    __syncthreads();
+    if (tid == 0) {
+        // slots 0..2 hold bins 5,4,3 (big), slots 3..5 bins 2,1,0 (small), slot 6 the tiny rows
+        const uint32_t n_big = s_slot[3] - s_slot[0];
+        const uint32_t n_small = s_slot[6] - s_slot[3];
+        const uint32_t n_tiny = s_slot[7] - s_slot[6];
+        for (int b = 0; b < kBins; ++b) ctl[C_CNT + b] = s_slot[kBins - b] - s_slot[kBins - 1 - b];
+        const uint32_t small_t = (n_small + small_group - 1) / small_group;
+        const uint32_t tiny_t = (n_tiny + kWaves * 64 - 1) / (kWaves * 64);
         ctl[C_NBIG] = n_big;
         ctl[C_NSMALL] = n_small;
-        ctl[C_NTICK] = n_big + (n_small + small_group - 1) / small_group;
+        ctl[C_NSMALLT] = small_t;
+        ctl[C_NTINY] = n_tiny;
+        ctl[C_TINY0] = s_slot[6];
+        ctl[C_NTICK] = n_big + small_t + tiny_t;
         ctl[C_HEAD] = 0;
     }
 }
@@ -213,7 +309,8 @@ __device__ __forceinline__ void table_insert(uint32_t *tab, uint32_t tlog2, cons
 // ------------------------------------------------------------------ walk
 // One round of a window: the entries [B, B + 64*UNROLL) of the window's flat
 // sequence (B < total), chunk k (64 entries) loaded into x[k].  Lane i of the
-// wave holds segment i: mybase = its column base minus its flat start, myend =
+// wave holds segment i: mybase = its column start minus its flat start (mod
+// 2^32: column indices are below 2^32 because m < 2^32 - 4096), myend =
 // its exclusive flat end (kEmpty for lanes without a segment); ends ascend.
 // The ends that fall inside the round are marked in `flag` (one byte per
 // position, left all zero again), so every chunk finds each lane's segment with
@@ -223,7 +320,7 @@ __device__ __forceinline__ void table_insert(uint32_t *tab, uint32_t tlog2, cons
 // result register is never copied and the loads stay in flight until probed.
 template <int UNROLL>
 __device__ __forceinline__ void issue_round(const uint32_t *__restrict__ col, uint32_t B, uint32_t total,
-                                            uint64_t mybase, uint32_t myend, uint8_t *flag, uint32_t lane,
+                                            uint32_t mybase, uint32_t myend, uint8_t *flag, uint32_t lane,
                                             uint32_t (&x)[UNROLL]) {
     constexpr uint32_t RB = 64u * UNROLL;
     // segments that end at or before B lie wholly before this round
@@ -234,14 +331,14 @@ __device__ __forceinline__ void issue_round(const uint32_t *__restrict__ col, ui
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) {
         const uint64_t M = ballot(flag[64u * k + lane] != 0);
-        uint64_t base;
+        uint32_t base;
         if (M == 0)
-            base = read_lane64(mybase, sg & 63u);  // the chunk lies inside one segment
+            base = read_lane(mybase, sg & 63u);  // the chunk lies inside one segment
         else
-            base = gather64(mybase, sg + (uint32_t)__popcll(M & ((2ull << lane) - 1ull)));
+            base = gather(mybase, sg + (uint32_t)__popcll(M & ((2ull << lane) - 1ull)));
         sg += (uint32_t)__popcll(M);
         const uint32_t t = B + 64u * k + lane;
-        const uint32_t *p = t < total ? col + base + t : col;
+        const uint32_t *p = t < total ? col + (uint32_t)(base + t) : col;
         x[k] = gload(p);
     }
     if (mk) flag[myend - B] = 0;
@@ -285,23 +382,23 @@ __device__ __forceinline__ void probe_round(const uint32_t (&x)[UNROLL], uint32_
 // issued before round r is probed, so a wave keeps up to 2*UNROLL chunk loads
 // in flight.
 template <int UNROLL>
-__device__ __forceinline__ unsigned long long walk(const uint64_t *__restrict__ off, const uint32_t *__restrict__ col,
+__device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const uint32_t *__restrict__ col,
                                                    uint32_t n_rows, const uint32_t *__restrict__ seg, uint32_t nseg_all,
                                                    uint32_t below, uint32_t w0, uint32_t wstep, const uint32_t *tab,
-                                                   uint32_t tlog2, uint64_t *s_base, uint32_t *s_end, uint8_t *flag,
+                                                   uint32_t tlog2, uint32_t *s_base, uint32_t *s_end, uint8_t *flag,
                                                    uint32_t lane) {
     constexpr uint32_t RB = 64u * UNROLL;
     const uint32_t mask = (1u << tlog2) - 1, shift = 32 - tlog2;
     uint32_t lane_hits = 0;
     for (uint32_t j0 = w0; j0 < nseg_all; j0 += wstep) {
         const uint32_t j = j0 + lane;
-        uint64_t s = 0;
-        uint32_t len = 0;
+        uint32_t s = 0, len = 0;
         if (j < nseg_all) {
             const uint32_t w = gload(seg + j);
             if (w < n_rows) {
-                s = gload(off + w);
-                len = (uint32_t)(gload(off + w + 1) - s);
+                const uint64_t s0 = gload(off + w);
+                s = (uint32_t)s0;
+                len = (uint32_t)(gload(off + w + 1) - s0);
                 if (below && len >= below) len = 0;  // that arc is counted from the other side
             }
         }
@@ -312,10 +409,10 @@ __device__ __forceinline__ unsigned long long walk(const uint64_t *__restrict__ 
         const uint32_t nseg = (uint32_t)__popcll(nzm);
         if (len != 0) {
             const uint32_t slot = mask_rank(nzm);
-            s_base[slot] = s - (uint64_t)(incl - len);
+            s_base[slot] = s - (incl - len);
             s_end[slot] = incl;
         }
-        const uint64_t mybase = s_base[lane];
+        const uint32_t mybase = s_base[lane];
         const uint32_t myend = lane < nseg ? s_end[lane] : kEmpty;
         uint32_t xa[UNROLL], xb[UNROLL];
         issue_round<UNROLL>(col, 0, total, mybase, myend, flag, lane, xa);
@@ -339,7 +436,8 @@ __device__ __forceinline__ unsigned long long walk(const uint64_t *__restrict__ 
             }
         }
     }
-    return wave_sum64((unsigned long long)lane_hits);
+    // an item streams at most m < 2^32 entries, so its hits fit 32 bits
+    return wave_sum32(lane_hits);
 }
 
 // The segment source of a work item: kind A walks N+(r) with the split filter,
@@ -353,10 +451,101 @@ struct Segs {
 __device__ __forceinline__ Segs item_segs(const GraphDev &G, uint32_t kind, uint32_t chunk, uint32_t r, uint64_t rb,
                                           uint32_t K) {
     if (kind == 0) return Segs{G.col + rb, K, G.in_off ? K : 0u};
-    const uint64_t ib = gload(G.in_off + r), ie = gload(G.in_off + r + 1);
+    const uint64_t ib = sload(G.in_off + r), ie = sload(G.in_off + r + 1);
     const uint64_t c0 = ib + (uint64_t)chunk * kSegChunk;
     const uint64_t c1 = (c0 + kSegChunk < ie) ? c0 + kSegChunk : ie;
     return Segs{G.in_col + c0, (uint32_t)(c1 - c0), 0u};
+}
+
+// ------------------------------------------------------------------ tiny rows
+// Rows with 2 <= d+ <= tiny_max of a graph with in-lists.  Under the split rule
+// every arc such a row r owns joins two lists of at most d+(r) entries:
+//   out side: arcs (r, x) with d+(x) < d+(r), from N+(r);
+//   in side : arcs (u, r) from r's in-list (d+(u) <= d+(r)).
+// So each arc is one lane's job: compare the two lists, at most kTinyMax^2
+// equality tests, no hash table and no probing.  The wave's lanes hold up to 64
+// rows of graph G (mine); their lists are staged once in the wave's LDS slice
+// (row of lane i at tab[i*kTinyMax ...]), then their out-sides and in-sides are
+// walked flat, 64 arcs per step, with the same end-marking as the hash walk.
+// Each lane loads the other list of its arc into registers.
+__device__ __forceinline__ uint32_t tiny_walk(const GraphDev &G, bool mine, bool out_side, uint32_t r, uint32_t *tab,
+                                              uint32_t *s_base, uint32_t *s_end, uint8_t *s_src, uint8_t *flag,
+                                              uint32_t lane) {
+    uint32_t ro = 0, rd = 0;  // this lane's row: start and length of N+(r)
+    if (mine) {
+        const uint64_t o0 = gload(G.off + r);
+        ro = (uint32_t)o0;
+        rd = (uint32_t)(gload(G.off + r + 1) - o0);
+#pragma unroll
+        for (uint32_t j = 0; j < kTinyMax; ++j)
+            if (j < rd) tab[lane * kTinyMax + j] = gload(G.col + ro + j);
+    }
+    uint32_t lane_hits = 0;
+#pragma unroll 1
+    for (int side = 0; side < 2; ++side) {
+        uint32_t s = 0, len = 0;
+        if (mine) {
+            if (side == 0) {
+                s = ro;
+                len = out_side ? rd : 0u;
+            } else {
+                const uint64_t i0 = gload(G.in_off + r);
+                s = (uint32_t)i0;
+                len = (uint32_t)(gload(G.in_off + r + 1) - i0);
+            }
+        }
+        const uint32_t *list = side == 0 ? G.col : G.in_col;
+        const uint32_t incl = wave_incl_scan(len, lane);
+        const uint32_t total = uniform(gather(incl, 63));
+        if (total == 0) continue;
+        const uint64_t nzm = ballot(len != 0);
+        const uint32_t nseg = (uint32_t)__popcll(nzm);
+        if (len != 0) {
+            const uint32_t slot = mask_rank(nzm);
+            s_base[slot] = s - (incl - len);
+            s_end[slot] = incl;
+            s_src[slot] = (uint8_t)lane;
+        }
+        const uint32_t mybase = s_base[lane];
+        const uint32_t myend = lane < nseg ? s_end[lane] : kEmpty;
+        const uint32_t mysrc = s_src[lane];
+        for (uint32_t B = 0; B < total; B += 64) {
+            uint32_t sg = (uint32_t)__popcll(ballot(myend <= B));
+            const bool mk = myend > B && myend < B + 64;
+            if (mk) flag[myend - B] = 1;
+            const uint64_t M = ballot(flag[lane] != 0);
+            if (mk) flag[myend - B] = 0;
+            if (M != 0) sg += (uint32_t)__popcll(M & ((2ull << lane) - 1ull));
+            sg &= 63u;  // lanes past the end take some owner; their arc is not counted
+            const uint32_t base = gather(mybase, sg), src = gather(mysrc, sg);
+            const uint32_t t = B + lane;
+            const uint32_t x = gload(t < total ? list + (uint32_t)(base + t) : list);
+            const uint32_t ard = gather(rd, src);
+            const uint32_t xi = t < total ? x : 0u;
+            const uint64_t xo0 = gload(G.off + xi);
+            const uint32_t xo = (uint32_t)xo0, xd = (uint32_t)(gload(G.off + xi + 1) - xo0);
+            // the out side keeps d+(x) < d+(r); the in side holds only d+(x) <= d+(r)
+            const bool live = t < total && xd != 0 && (side == 1 || xd < ard);
+            uint32_t b[kTinyMax];
+#pragma unroll
+            for (uint32_t j = 0; j < kTinyMax; ++j)  // unconditional, in bounds (col[0] when unused)
+                b[j] = gload(live && j < xd ? G.col + xo + j : G.col);
+            if (live) {
+                const uint32_t *a = tab + src * kTinyMax;
+                uint32_t c = 0;
+#pragma unroll
+                for (uint32_t i = 0; i < kTinyMax; ++i) {
+                    if (i < ard) {
+                        const uint32_t ai = a[i];
+#pragma unroll
+                        for (uint32_t j = 0; j < kTinyMax; ++j) c += (j < xd && ai == b[j]) ? 1u : 0u;
+                    }
+                }
+                lane_hits += c;
+            }
+        }
+    }
+    return wave_sum32(lane_hits);
 }
 
 // ------------------------------------------------------------------ persistent count kernel
@@ -366,32 +555,36 @@ __global__ __launch_bounds__(kWaves * 64) void k_count(GraphSet gs, const unsign
                                                        unsigned long long *__restrict__ counts, uint32_t scale,
                                                        uint32_t small_group) {
     __shared__ uint32_t s_tab[kBigCap];
-    __shared__ uint64_t s_base[kWaves][64];
+    __shared__ uint32_t s_base[kWaves][64];
     __shared__ uint32_t s_end[kWaves][64];
     __shared__ uint32_t s_flag[kWaves][16 * UNROLL];  // 64*UNROLL bytes per wave
+    __shared__ uint8_t s_src[kWaves][64];
     __shared__ uint32_t s_ticket;
     __shared__ unsigned long long s_acc[kWaves];
     __shared__ uint32_t s_g[kWaves];
 
-    const uint32_t tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    // item and graph fields are wave-uniform: they live in scalar registers
+    const uint32_t tid = threadIdx.x, wv = uniform(tid >> 6), lane = tid & 63;
     for (uint32_t i = lane; i < 16 * UNROLL; i += 64) s_flag[wv][i] = 0;
-    const uint32_t n_big = ctl[C_NBIG], n_small = ctl[C_NSMALL], n_tickets = ctl[C_NTICK];
+    const uint32_t n_big = sload(ctl + C_NBIG), n_small = sload(ctl + C_NSMALL), n_tickets = sload(ctl + C_NTICK);
+    const uint32_t small_end = n_big + sload(ctl + C_NSMALLT);  // first tiny ticket
+    const uint32_t tiny0 = sload(ctl + C_TINY0), tiny_end = tiny0 + sload(ctl + C_NTINY);
     uint32_t cur_g = kEmpty;
     unsigned long long acc = 0;
 
     for (;;) {
         if (tid == 0) s_ticket = atomicAdd(ctl + C_HEAD, 1u);
         __syncthreads();
-        const uint32_t t = s_ticket;
+        const uint32_t t = uniform(s_ticket);
         __syncthreads();
         if (t >= n_tickets) break;
         if (t < n_big) {
             // big item: the whole workgroup shares one table
-            const Item itm = decode(gload(queue + t));
+            const Item itm = decode(sload(queue + t));
             const uint32_t u = itm.row, chunk = itm.chunk, kind = itm.kind, g = itm.g;
-            const GraphDev G = graph_at(gs, g);
-            const uint64_t rb = gload(G.off + u);
-            const uint32_t K = (uint32_t)(gload(G.off + u + 1) - rb);
+            const GraphDev G = graph_at<true>(gs, g);
+            const uint64_t rb = sload(G.off + u);
+            const uint32_t K = (uint32_t)(sload(G.off + u + 1) - rb);
             const Segs sg = item_segs(G, kind, chunk, u, rb, K);
             unsigned long long hits = 0;
             for (uint32_t t0 = 0; t0 < K; t0 += kBigTile) {
@@ -412,17 +605,39 @@ __global__ __launch_bounds__(kWaves * 64) void k_count(GraphSet gs, const unsign
             }
             acc += hits;
             if (G.per_row && lane == 0 && hits) atomicAdd(G.per_row + u, hits);
+        } else if (t >= small_end) {
+            // tiny rows: 64 per wave, lane-per-arc; the lanes are split by graph
+            const uint32_t e0 = tiny0 + (t - small_end) * (kWaves * 64) + wv * 64;
+            const uint32_t e = e0 + lane;
+            const bool valid = e < tiny_end;
+            Item itm{0, 0, 0, 0};
+            if (valid) itm = decode(gload(queue + e));
+            uint64_t rem = ballot(valid);
+            while (rem) {
+                const uint32_t g = read_lane(itm.g, (uint32_t)__builtin_ctzll(rem));
+                const bool mine = valid && itm.g == g;
+                rem &= ~ballot(mine);
+                const GraphDev G = graph_at<true>(gs, g);
+                const uint32_t hits = tiny_walk(G, mine, mine && itm.kind == 0, itm.row, s_tab + wv * kSlice,
+                                                s_base[wv], s_end[wv], s_src[wv], (uint8_t *)s_flag[wv], lane);
+                if (g != cur_g) {
+                    if (cur_g != kEmpty && acc != 0 && lane == 0) atomicAdd(counts + cur_g, acc);
+                    cur_g = g;
+                    acc = 0;
+                }
+                acc += hits;
+            }
         } else {
             // small items: each wave takes every kWaves-th item of the group
             const uint32_t first = n_big + (t - n_big) * small_group;
             const uint32_t last = min(first + small_group, n_big + n_small);
             uint32_t *tab = s_tab + wv * kSlice;
             for (uint32_t q = first + wv; q < last; q += kWaves) {
-                const Item itm = decode(gload(queue + q));
+                const Item itm = decode(sload(queue + q));
                 const uint32_t u = itm.row, chunk = itm.chunk, kind = itm.kind, g = itm.g;
-                const GraphDev G = graph_at(gs, g);
-                const uint64_t rb = gload(G.off + u);
-                const uint32_t K = (uint32_t)(gload(G.off + u + 1) - rb);
+                const GraphDev G = graph_at<true>(gs, g);
+                const uint64_t rb = sload(G.off + u);
+                const uint32_t K = (uint32_t)(sload(G.off + u + 1) - rb);
                 const Segs sg = item_segs(G, kind, chunk, u, rb, K);
                 const uint32_t tl = table_log2_for(K, scale, kSliceLog2);
                 for (uint32_t i = lane; i < (1u << tl); i += 64) tab[i] = kEmpty;
@@ -461,6 +676,7 @@ size_t count_ws_bytes(uint32_t G, uint64_t total_rows, uint64_t total_m) {
     (void)G;
     Carver c(nullptr, 0);
     c.take<uint32_t>(64);
+    c.take<uint32_t>(kPlanEntries);
     c.take<unsigned long long>(queue_cap(total_rows, total_m));
     return c.need();
 }
@@ -475,12 +691,15 @@ lstc_status count_launch(const HostGraph *hosts, uint32_t G, const GraphDev *d_t
     const TuneR tr = resolve_tune(tune);
     uint64_t total = 0, total_m = 0;
     for (uint32_t g = 0; g < G; ++g) {
-        total += hosts[g].n_owned;
+        total += plan_rows(hosts[g]);
         total_m += hosts[g].m;
     }
     if (total >= 0xFFFFFFFFull) return fail(LSTC_ERR_INVALID, "count: more than 2^32-1 owned rows in one launch");
     Carver c(ws, ws_bytes);
+    if (queue_cap(total, total_m) >= 0xFFFFFFFFull)
+        return fail(LSTC_ERR_INVALID, "count: more than 2^32-1 work items in one launch");
     uint32_t *ctl = c.take<uint32_t>(64);
+    uint32_t *wcnt = c.take<uint32_t>(kPlanEntries);
     unsigned long long *queue = c.take<unsigned long long>(queue_cap(total, total_m));
     if (!ws || !c.ok()) return fail(LSTC_ERR_WORKSPACE, "count: workspace %zu < %zu bytes", ws_bytes, c.need());
 
@@ -498,16 +717,15 @@ lstc_status count_launch(const HostGraph *hosts, uint32_t G, const GraphDev *d_t
     hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(256), 0, st, ctl, d_counts, G);
     LSTC_LAUNCHED("k_plan_init");
     const int cus = device_cus();
-    Bounds bd{{16, 64, tr.small_max, 1024, 2048, 0xFFFFFFFFu}};
+    Bounds bd{{16, 64, tr.small_max, 1024, 2048, 0xFFFFFFFFu}, tr.tiny_max};
     if (total) {
-        const uint32_t grid = (uint32_t)hmin<uint64_t>((total + 255) / 256, (uint64_t)cus * 8);
-        hipLaunchKernelGGL(k_plan_rows<0>, dim3(grid), dim3(256), 0, st, gs, bd, ctl, queue);
+        hipLaunchKernelGGL(k_plan_rows<0>, dim3(kPlanBlocks), dim3(256), 0, st, gs, bd, wcnt, queue);
         LSTC_LAUNCHED("k_plan_rows<0>");
-        hipLaunchKernelGGL(k_plan_rows<1>, dim3(grid), dim3(256), 0, st, gs, bd, ctl, queue);
+        hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(1024), 0, st, wcnt, ctl, tr.small_group);
+        LSTC_LAUNCHED("k_plan_scan");
+        hipLaunchKernelGGL(k_plan_rows<1>, dim3(kPlanBlocks), dim3(256), 0, st, gs, bd, wcnt, queue);
         LSTC_LAUNCHED("k_plan_rows<1>");
     }
-    hipLaunchKernelGGL(k_plan_finish, dim3(1), dim3(64), 0, st, ctl, tr.small_group);
-    LSTC_LAUNCHED("k_plan_finish");
     if (total) {
         const uint32_t grid = (uint32_t)cus * tr.grid_per_cu;
         if (tr.unroll == 8)

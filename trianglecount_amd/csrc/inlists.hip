@@ -1,0 +1,136 @@
+// inlists.hip -- target-side in-lists of an oriented CSR (enables split counting
+// for any caller CSR, e.g. the halo subgraphs of a partition).
+//
+// in-list(w) = { u < n_owned : u -> w is an arc and d+(u) <= d+(w) }, ascending,
+// where d+ is the row length in this CSR.  Together with the kind A filter
+// d+(w) < d+(u) this assigns every arc leaving an owned row to exactly one side:
+// the count over owned rows is unchanged, and each arc streams its shorter list.
+//
+// Steps: per owned row, count and emit keys (u << 32 | w) of the kept arcs in
+// row order; a stable LSD radix sort on the low (target) bits groups them by
+// target with sources still ascending; in_col takes the sources and in_off[w]
+// is found by binary search over the sorted targets.
+#include "lstc_internal.h"
+
+namespace lstc {
+
+__global__ void k_inl_count(const uint64_t *__restrict__ off, const uint32_t *__restrict__ col, uint32_t n_owned,
+                            uint32_t *__restrict__ cnt) {
+    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u <= n_owned;
+         u += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t c = 0;
+        if (u < n_owned) {
+            const uint64_t b = off[u], e = off[u + 1];
+            const uint64_t du = e - b;
+            for (uint64_t i = b; i < e; ++i) {
+                const uint32_t w = col[i];
+                c += (du <= off[w + 1] - off[w]) ? 1u : 0u;
+            }
+        }
+        cnt[u] = c;
+    }
+}
+
+__global__ void k_inl_keys(const uint64_t *__restrict__ off, const uint32_t *__restrict__ col, uint32_t n_owned,
+                           const uint64_t *__restrict__ pos, uint64_t *__restrict__ keys) {
+    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n_owned;
+         u += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t b = off[u], e = off[u + 1];
+        const uint64_t du = e - b;
+        uint64_t p = pos[u];
+        for (uint64_t i = b; i < e; ++i) {
+            const uint32_t w = col[i];
+            if (du <= off[w + 1] - off[w]) keys[p++] = (u << 32) | w;
+        }
+    }
+}
+
+__global__ void k_inl_cols(const uint64_t *__restrict__ keys, uint64_t k, uint32_t *__restrict__ in_col) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < k; i += (uint64_t)gridDim.x * blockDim.x)
+        in_col[i] = (uint32_t)(keys[i] >> 32);
+}
+
+// in_off[w] = number of sorted keys whose target is below w
+__global__ void k_inl_offsets(const uint64_t *__restrict__ keys, uint64_t k, uint32_t n_rows,
+                              uint64_t *__restrict__ in_off) {
+    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w <= n_rows;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t lo = 0, hi = k;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if ((uint32_t)keys[mid] < w) lo = mid + 1; else hi = mid;
+        }
+        in_off[w] = lo;
+    }
+}
+
+static int bits_for(uint32_t n) {
+    int b = 8;
+    while (b < 32 && ((uint64_t)n >> b) != 0) b += 8;
+    return b;
+}
+
+static size_t inl_ws(uint32_t n_owned, uint64_t m, size_t *parts) {
+    Carver c(nullptr, 0);
+    c.take<uint32_t>((size_t)n_owned + 1);
+    c.take<uint64_t>((size_t)n_owned + 1);
+    c.take<uint64_t>(m ? m : 1);
+    c.take<uint64_t>(m ? m : 1);
+    const size_t head = c.need();
+    const size_t tail = hmax(scan_ws_bytes((uint64_t)n_owned + 1), radix_sort_ws_bytes(m));
+    if (parts) *parts = head;
+    return head + tail + 256;
+}
+
+}  // namespace lstc
+
+using namespace lstc;
+
+extern "C" lstc_status lstc_in_lists_workspace(uint32_t n_rows, uint32_t n_owned, uint64_t m, size_t *bytes) {
+    if (!bytes) return fail(LSTC_ERR_INVALID, "lstc_in_lists_workspace: NULL bytes");
+    if (n_owned > n_rows) return fail(LSTC_ERR_INVALID, "lstc_in_lists_workspace: n_owned > n_rows");
+    *bytes = inl_ws(n_owned, m, nullptr);
+    return LSTC_OK;
+}
+
+extern "C" lstc_status lstc_in_lists(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows, uint32_t n_owned,
+                                     uint64_t m, uint64_t *d_in_off, uint32_t *d_in_col, uint64_t *n_entries,
+                                     void *d_ws, size_t ws_bytes, lstc_stream s) {
+    hipStream_t st = (hipStream_t)s;
+    if (n_owned > n_rows) return fail(LSTC_ERR_INVALID, "lstc_in_lists: n_owned %u > n_rows %u", n_owned, n_rows);
+    if (n_rows == 0xFFFFFFFFu) return fail(LSTC_ERR_INVALID, "lstc_in_lists: n_rows must be < 2^32-1");
+    if (!d_off || !d_in_off || (m && (!d_col || !d_in_col)))
+        return fail(LSTC_ERR_INVALID, "lstc_in_lists: NULL array");
+    size_t head = 0;
+    const size_t need = inl_ws(n_owned, m, &head);
+    if (!d_ws || ws_bytes < need) return fail(LSTC_ERR_WORKSPACE, "lstc_in_lists: workspace %zu < %zu", ws_bytes, need);
+    Carver c(d_ws, ws_bytes);
+    uint32_t *cnt = c.take<uint32_t>((size_t)n_owned + 1);
+    uint64_t *pos = c.take<uint64_t>((size_t)n_owned + 1);
+    uint64_t *keys = c.take<uint64_t>(m ? m : 1);
+    uint64_t *alt = c.take<uint64_t>(m ? m : 1);
+    char *tail = (char *)d_ws + head;
+    const size_t tail_bytes = ws_bytes - head;
+    const uint32_t grid_rows = (uint32_t)hmin<uint64_t>(((uint64_t)n_owned + 256) / 256, 65535);
+    hipLaunchKernelGGL(k_inl_count, dim3(grid_rows), dim3(256), 0, st, d_off, d_col, n_owned, cnt);
+    LSTC_LAUNCHED("k_inl_count");
+    LSTC_TRY(scan_exclusive_u32_to_u64(cnt, pos, (uint64_t)n_owned + 1, nullptr, tail, tail_bytes, st));
+    uint64_t k = 0;
+    LSTC_HIP(hipMemcpyAsync(&k, pos + n_owned, sizeof(k), hipMemcpyDeviceToHost, st));
+    LSTC_HIP(hipStreamSynchronize(st));
+    if (k > m) return fail(LSTC_ERR_INVALID, "lstc_in_lists: %llu entries exceed m", (unsigned long long)k);
+    if (n_owned)
+        hipLaunchKernelGGL(k_inl_keys, dim3(grid_rows), dim3(256), 0, st, d_off, d_col, n_owned, pos, keys);
+    LSTC_LAUNCHED("k_inl_keys");
+    uint64_t *sorted = keys;
+    LSTC_TRY(radix_sort_u64(keys, alt, k, bits_for(n_rows), &sorted, tail, tail_bytes, st));
+    const uint32_t grid_k = (uint32_t)hmin<uint64_t>((k + 255) / 256 + 1, 65535);
+    if (k) hipLaunchKernelGGL(k_inl_cols, dim3(grid_k), dim3(256), 0, st, sorted, k, d_in_col);
+    LSTC_LAUNCHED("k_inl_cols");
+    const uint32_t grid_w = (uint32_t)hmin<uint64_t>(((uint64_t)n_rows + 256) / 256, 65535);
+    hipLaunchKernelGGL(k_inl_offsets, dim3(grid_w), dim3(256), 0, st, sorted, k, n_rows, d_in_off);
+    LSTC_LAUNCHED("k_inl_offsets");
+    LSTC_HIP(hipStreamSynchronize(st));
+    if (n_entries) *n_entries = k;
+    return LSTC_OK;
+}

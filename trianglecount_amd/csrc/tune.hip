@@ -92,12 +92,13 @@ extern "C" lstc_status lstc_autotune(const lstc_graph_desc *graphs, uint32_t G, 
     const uint32_t cand_scale[] = {2, 4, 8};
     const uint32_t cand_small[] = {64, 128, 256};
     const uint32_t cand_group[] = {4, 16, 64};
-    const uint32_t cand_grid[] = {3, 5, 7};
+    const uint32_t cand_grid[] = {4, 6, 8};
     const uint32_t cand_unroll[] = {4, 8};
+    const uint32_t cand_tiny[] = {1, 4, 8};  // 1 = lane-per-arc path off
     struct Knob { uint32_t lstc_tune::*field; const uint32_t *vals; int nvals; };
     const Knob knobs[] = {{&lstc_tune::table_scale, cand_scale, 3}, {&lstc_tune::small_max, cand_small, 3},
                           {&lstc_tune::small_group, cand_group, 3}, {&lstc_tune::grid_per_cu, cand_grid, 3},
-                          {&lstc_tune::unroll, cand_unroll, 2}};
+                          {&lstc_tune::unroll, cand_unroll, 2}, {&lstc_tune::tiny_max, cand_tiny, 3}};
     for (const Knob &k : knobs) {
         if (rc != LSTC_OK) break;
         lstc_tune base = cur;
@@ -127,6 +128,7 @@ extern "C" lstc_status lstc_autotune(const lstc_graph_desc *graphs, uint32_t G, 
     best->small_group = r.small_group;
     best->grid_per_cu = r.grid_per_cu;
     best->unroll = r.unroll;
+    best->tiny_max = r.tiny_max ? r.tiny_max : 1u;
     memset(best->reserved, 0, sizeof(best->reserved));
     if (best_ms) *best_ms = cur_ms;
     return LSTC_OK;
@@ -143,12 +145,12 @@ extern "C" lstc_status lstc_tune_cache_lookup(const char *path, uint32_t shape_c
     lstc_tune t{};
     while (fgets(line, sizeof(line), f)) {
         char d[128];
-        unsigned cls, a, b, c, g, u;
+        unsigned cls, a, b, c, g, u, y;
         double ms;
-        if (sscanf(line, "%127s %u %u %u %u %u %u %lf", d, &cls, &a, &b, &c, &g, &u, &ms) != 8) continue;
+        if (sscanf(line, "%127s %u %u %u %u %u %u %u %lf", d, &cls, &a, &b, &c, &g, &u, &y, &ms) != 9) continue;
         if (strcmp(d, dev) != 0 || cls != shape_class) continue;
         t = lstc_tune{};
-        t.table_scale = a; t.small_max = b; t.small_group = c; t.grid_per_cu = g; t.unroll = u;
+        t.table_scale = a; t.small_max = b; t.small_group = c; t.grid_per_cu = g; t.unroll = u; t.tiny_max = y;
         found = true;  // the last matching line wins
     }
     fclose(f);
@@ -163,8 +165,8 @@ extern "C" lstc_status lstc_tune_cache_store(const char *path, uint32_t shape_cl
     device_key(dev, sizeof(dev));
     FILE *f = fopen(path, "a");
     if (!f) return fail(LSTC_ERR_INVALID, "lstc_tune_cache_store: cannot open %s", path);
-    fprintf(f, "%s\t%u\t%u\t%u\t%u\t%u\t%u\t%.6f\n", dev, shape_class, t->table_scale, t->small_max, t->small_group,
-            t->grid_per_cu, t->unroll, ms);
+    fprintf(f, "%s\t%u\t%u\t%u\t%u\t%u\t%u\t%u\t%.6f\n", dev, shape_class, t->table_scale, t->small_max,
+            t->small_group, t->grid_per_cu, t->unroll, t->tiny_max, ms);
     fclose(f);
     return LSTC_OK;
 }

@@ -15,6 +15,15 @@ __device__ __forceinline__ T gload(const T *p) {
     return *(const LSTC_GLOBAL_AS T *)p;
 }
 
+// Loads through the scalar data cache (constant address space).  Only for
+// addresses that are uniform across the wave and memory that nothing writes
+// while the kernel runs (written by earlier kernels or copies is fine).
+#define LSTC_CONST_AS __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ T sload(const T *p) {
+    return *(const LSTC_CONST_AS T *)p;
+}
+
 __device__ __forceinline__ uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
@@ -43,15 +52,22 @@ __device__ __forceinline__ uint64_t gather64(uint64_t v, uint32_t src) {
     uint32_t lo = gather((uint32_t)v, src), hi = gather((uint32_t)(v >> 32), src);
     return ((uint64_t)hi << 32) | lo;
 }
-// Inclusive prefix sum across the wave (Hillis-Steele over ds_bpermute).
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
-#pragma unroll
-    for (uint32_t d = 1; d < 64; d <<= 1) {
-        uint32_t y = gather(v, lane >= d ? lane - d : lane);
-        if (lane >= d) v += y;
-    }
+// Inclusive prefix sum across the wave with DPP (no LDS round trips):
+// Hillis-Steele inside each row of 16 lanes (row_shr 1, 2, 4, 8), then lane 15
+// of rows 0 and 2 is added to rows 1 and 3 (row_bcast:15, row mask 0b1010) and
+// lane 31 to rows 2 and 3 (row_bcast:31, row mask 0b1100).  Lanes whose DPP
+// source is out of range or masked receive the `old` operand, 0.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t /*lane*/) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
     return v;
 }
+// Sum across the wave (every lane receives it).
+__device__ __forceinline__ uint32_t wave_sum32(uint32_t v) { return read_lane(wave_incl_scan(v, 0), 63); }
 __device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t v, uint32_t lane) {
 #pragma unroll
     for (uint32_t d = 1; d < 64; d <<= 1) {
