@@ -312,42 +312,60 @@ __device__ __forceinline__ void table_insert(uint32_t *tab, uint32_t tlog2, cons
 // wave holds segment i: mybase = its column start minus its flat start (mod
 // 2^32: column indices are below 2^32 because m < 2^32 - 4096), myend =
 // its exclusive flat end (kEmpty for lanes without a segment); ends ascend.
-// The ends that fall inside the round are marked in `flag` (one byte per
-// position, left all zero again), so every chunk finds each lane's segment with
-// one LDS read + ballot + popcount, independently of the other chunks.
+// The ends that fall inside the round are marked in `flag`, one byte per
+// position stored lane-major (position p at (p % 64) * UNROLL + p / 64), so a
+// single LDS read gives each lane its UNROLL chunk flags; the flags are cleared
+// again right after.  Each chunk then finds a lane's segment with a ballot and
+// a popcount, and the bases of all chunks are gathered back to back.
 // Positions at or past `total` load col[0] (in bounds, since total > 0 means the
 // graph has arcs) and are never probed: every load is unconditional, so its
 // result register is never copied and the loads stay in flight until probed.
 template <int UNROLL>
+struct FlagWord;  // the UNROLL flag bytes of one lane
+template <>
+struct FlagWord<4> {
+    using T = uint32_t;
+};
+template <>
+struct FlagWord<8> {
+    using T = uint64_t;
+};
+
+template <int UNROLL>
 __device__ __forceinline__ void issue_round(const uint32_t *__restrict__ col, uint32_t B, uint32_t total,
                                             uint32_t mybase, uint32_t myend, uint8_t *flag, uint32_t lane,
                                             uint32_t (&x)[UNROLL]) {
+    using FW = typename FlagWord<UNROLL>::T;
     constexpr uint32_t RB = 64u * UNROLL;
     // segments that end at or before B lie wholly before this round
     uint32_t sg = (uint32_t)__popcll(ballot(myend <= B));
     // an end e inside the round: position e already belongs to the next segment
     const bool mk = myend > B && myend < B + RB;
-    if (mk) flag[myend - B] = 1;
+    const uint32_t e = myend - B;
+    const uint32_t fi = (e & 63u) * UNROLL + (e >> 6);
+    if (mk) flag[fi] = 1;
+    const FW fw = ((const FW *)flag)[lane];
+    if (mk) flag[fi] = 0;
+    const uint64_t upto = (2ull << lane) - 1ull;  // this lane and the ones below
+    uint32_t idx[UNROLL];
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) {
-        const uint64_t M = ballot(flag[64u * k + lane] != 0);
-        uint32_t base;
-        if (M == 0)
-            base = read_lane(mybase, sg & 63u);  // the chunk lies inside one segment
-        else
-            base = gather(mybase, sg + (uint32_t)__popcll(M & ((2ull << lane) - 1ull)));
+        const uint64_t M = ballot(((fw >> (8 * k)) & 0xFFu) != 0);
+        idx[k] = sg + (uint32_t)__popcll(M & upto);
         sg += (uint32_t)__popcll(M);
-        const uint32_t t = B + 64u * k + lane;
-        const uint32_t *p = t < total ? col + (uint32_t)(base + t) : col;
-        x[k] = gload(p);
     }
-    if (mk) flag[myend - B] = 0;
+#pragma unroll
+    for (int k = 0; k < UNROLL; ++k) {
+        const uint32_t base = gather(mybase, idx[k]);  // lanes past the end: any base, not probed
+        const uint32_t t = B + 64u * k + lane;
+        x[k] = gload(t < total ? col + (uint32_t)(base + t) : col);
+    }
 }
 
-// Probes the live entries of one round through tab (linear probing, lockstep
-// over the UNROLL chunks: one LDS round trip per step for all of them).  Hits
-// are counted per lane inside the divergent probe (a lane that has finished is
-// inactive there) and reduced across the wave per item.
+// Probes the live entries of one round through tab: linear probing in lockstep
+// over the UNROLL chunks, branch-free (every lane reads its slot each step, so
+// the UNROLL LDS reads of a step go out together; finished lanes just stop
+// counting).  Hits are counted per lane and reduced across the wave per item.
 template <int UNROLL>
 __device__ __forceinline__ void probe_round(const uint32_t (&x)[UNROLL], uint32_t B, uint32_t total, const uint32_t *tab,
                                             uint32_t mask, uint32_t shift, uint32_t lane, uint32_t &lane_hits) {
@@ -359,17 +377,17 @@ __device__ __forceinline__ void probe_round(const uint32_t (&x)[UNROLL], uint32_
         h[k] = hslot(x[k], shift);
     }
     for (;;) {
+        uint32_t v[UNROLL];
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) v[k] = tab[h[k]];
         bool more = false;
 #pragma unroll
         for (int k = 0; k < UNROLL; ++k) {
-            if (live[k]) {
-                const uint32_t v = tab[h[k]];
-                const bool hit = v == x[k];
-                lane_hits += hit ? 1u : 0u;
-                live[k] = !(hit || v == kEmpty);
-                h[k] = (h[k] + 1) & mask;
-                more |= live[k];
-            }
+            const bool hit = v[k] == x[k];
+            lane_hits += (live[k] & hit) ? 1u : 0u;
+            live[k] = live[k] & !hit & (v[k] != kEmpty);
+            h[k] = (h[k] + 1) & mask;
+            more |= live[k];
         }
         if (ballot(more) == 0) break;
     }
@@ -557,7 +575,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_count(GraphSet gs, const unsign
     __shared__ uint32_t s_tab[kBigCap];
     __shared__ uint32_t s_base[kWaves][64];
     __shared__ uint32_t s_end[kWaves][64];
-    __shared__ uint32_t s_flag[kWaves][16 * UNROLL];  // 64*UNROLL bytes per wave
+    __shared__ __align__(16) uint32_t s_flag[kWaves][16 * UNROLL];  // 64*UNROLL bytes per wave
     __shared__ uint8_t s_src[kWaves][64];
     __shared__ uint32_t s_ticket;
     __shared__ unsigned long long s_acc[kWaves];
