@@ -41,6 +41,7 @@
 namespace lstc {
 
 static constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+static constexpr uint32_t kNoId = 0xFFFFFFFEu;  // never a vertex id (ids < 2^32 - 2), never kEmpty
 static constexpr int kWaves = 4;
 static constexpr uint32_t kSliceLog2 = 10;
 static constexpr uint32_t kSlice = 1u << kSliceLog2;  // wave-private slice (small items)
@@ -86,7 +87,7 @@ TuneR resolve_tune(const lstc_tune *t) {
     r.small_max = hmax(64u, hmin(kSlice / r.table_scale, r.small_max));
     r.small_group = hmax(4u, hmin(256u, r.small_group));
     r.unroll = r.unroll >= 8 ? 8 : 4;
-    if (r.grid_per_cu == 0) r.grid_per_cu = 7;  // LDS-limited occupancy: 7 workgroups per CU
+    if (r.grid_per_cu == 0) r.grid_per_cu = 8;  // k_count<4> fits 8 workgroups per CU (LDS and VGPRs)
     r.grid_per_cu = hmax(1u, hmin(16u, r.grid_per_cu));
     r.tiny_max = hmin(kTinyMax, r.tiny_max);
     if (r.tiny_max < 2) r.tiny_max = 0;
@@ -362,18 +363,21 @@ __device__ __forceinline__ void issue_round(const uint32_t *__restrict__ col, ui
     }
 }
 
-// Probes the live entries of one round through tab: linear probing in lockstep
-// over the UNROLL chunks, branch-free (every lane reads its slot each step, so
-// the UNROLL LDS reads of a step go out together; finished lanes just stop
-// counting).  Hits are counted per lane and reduced across the wave per item.
+// Probes one round through tab: linear probing in lockstep over the UNROLL
+// chunks, branch-free, the UNROLL LDS reads of a step issued together.  A lane
+// that is done (hit, or reached an empty slot) is parked on `parked`, a slot
+// outside the table that always holds kEmpty, so it keeps reading kEmpty and
+// counts nothing more: a present key lies before the first empty slot of its
+// chain, so no lane hits twice.  Positions past the window's end carry an id
+// that no table holds (kNoId: ids are below 2^32 - 2).
 template <int UNROLL>
-__device__ __forceinline__ void probe_round(const uint32_t (&x)[UNROLL], uint32_t B, uint32_t total, const uint32_t *tab,
-                                            uint32_t mask, uint32_t shift, uint32_t lane, uint32_t &lane_hits) {
-    uint32_t h[UNROLL];
-    bool live[UNROLL];
+__device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint32_t B, uint32_t total, const uint32_t *tab,
+                                            uint32_t mask, uint32_t shift, uint32_t parked, uint32_t lane,
+                                            uint32_t &lane_hits) {
+    uint32_t x[UNROLL], h[UNROLL];
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) {
-        live[k] = B + 64u * k + lane < total;
+        x[k] = B + 64u * k + lane < total ? xin[k] : kNoId;
         h[k] = hslot(x[k], shift);
     }
     for (;;) {
@@ -384,10 +388,10 @@ __device__ __forceinline__ void probe_round(const uint32_t (&x)[UNROLL], uint32_
 #pragma unroll
         for (int k = 0; k < UNROLL; ++k) {
             const bool hit = v[k] == x[k];
-            lane_hits += (live[k] & hit) ? 1u : 0u;
-            live[k] = live[k] & !hit & (v[k] != kEmpty);
-            h[k] = (h[k] + 1) & mask;
-            more |= live[k];
+            const bool done = hit || v[k] == kEmpty;
+            lane_hits += hit ? 1u : 0u;
+            h[k] = done ? parked : ((h[k] + 1) & mask);
+            more |= !done;
         }
         if (ballot(more) == 0) break;
     }
@@ -403,8 +407,8 @@ template <int UNROLL>
 __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const uint32_t *__restrict__ col,
                                                    uint32_t n_rows, const uint32_t *__restrict__ seg, uint32_t nseg_all,
                                                    uint32_t below, uint32_t w0, uint32_t wstep, const uint32_t *tab,
-                                                   uint32_t tlog2, uint32_t *s_base, uint32_t *s_end, uint8_t *flag,
-                                                   uint32_t lane) {
+                                                   uint32_t tlog2, uint32_t parked, uint32_t *s_base, uint32_t *s_end,
+                                                   uint8_t *flag, uint32_t lane) {
     constexpr uint32_t RB = 64u * UNROLL;
     const uint32_t mask = (1u << tlog2) - 1, shift = 32 - tlog2;
     uint32_t lane_hits = 0;
@@ -440,16 +444,16 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
         for (uint32_t B = 0;; B += 2 * RB) {
             if (B + RB < total) {
                 issue_round<UNROLL>(col, B + RB, total, mybase, myend, flag, lane, xb);
-                probe_round<UNROLL>(xa, B, total, tab, mask, shift, lane, lane_hits);
+                probe_round<UNROLL>(xa, B, total, tab, mask, shift, parked, lane, lane_hits);
             } else {
-                probe_round<UNROLL>(xa, B, total, tab, mask, shift, lane, lane_hits);
+                probe_round<UNROLL>(xa, B, total, tab, mask, shift, parked, lane, lane_hits);
                 break;
             }
             if (B + 2 * RB < total) {
                 issue_round<UNROLL>(col, B + 2 * RB, total, mybase, myend, flag, lane, xa);
-                probe_round<UNROLL>(xb, B + RB, total, tab, mask, shift, lane, lane_hits);
+                probe_round<UNROLL>(xb, B + RB, total, tab, mask, shift, parked, lane, lane_hits);
             } else {
-                probe_round<UNROLL>(xb, B + RB, total, tab, mask, shift, lane, lane_hits);
+                probe_round<UNROLL>(xb, B + RB, total, tab, mask, shift, parked, lane, lane_hits);
                 break;
             }
         }
@@ -568,11 +572,11 @@ __device__ __forceinline__ uint32_t tiny_walk(const GraphDev &G, bool mine, bool
 
 // ------------------------------------------------------------------ persistent count kernel
 template <int UNROLL>
-__global__ __launch_bounds__(kWaves * 64) void k_count(GraphSet gs, const unsigned long long *__restrict__ queue,
+__global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 4) void k_count(GraphSet gs, const unsigned long long *__restrict__ queue,
                                                        uint32_t *__restrict__ ctl,
                                                        unsigned long long *__restrict__ counts, uint32_t scale,
                                                        uint32_t small_group) {
-    __shared__ uint32_t s_tab[kBigCap];
+    __shared__ uint32_t s_tab[kBigCap + 1];  // s_tab[kBigCap] stays kEmpty: the parking slot of every table
     __shared__ uint32_t s_base[kWaves][64];
     __shared__ uint32_t s_end[kWaves][64];
     __shared__ __align__(16) uint32_t s_flag[kWaves][16 * UNROLL];  // 64*UNROLL bytes per wave
@@ -584,6 +588,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_count(GraphSet gs, const unsign
     // item and graph fields are wave-uniform: they live in scalar registers
     const uint32_t tid = threadIdx.x, wv = uniform(tid >> 6), lane = tid & 63;
     for (uint32_t i = lane; i < 16 * UNROLL; i += 64) s_flag[wv][i] = 0;
+    if (tid == 0) s_tab[kBigCap] = kEmpty;  // visible to all waves after the first ticket's barrier
     const uint32_t n_big = sload(ctl + C_NBIG), n_small = sload(ctl + C_NSMALL), n_tickets = sload(ctl + C_NTICK);
     const uint32_t small_end = n_big + sload(ctl + C_NSMALLT);  // first tiny ticket
     const uint32_t tiny0 = sload(ctl + C_TINY0), tiny_end = tiny0 + sload(ctl + C_NTINY);
@@ -613,7 +618,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_count(GraphSet gs, const unsign
                 table_insert(s_tab, tl, G.col, rb + t0, tk, tid, kWaves * 64);
                 __syncthreads();
                 hits += walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.n, sg.below, wv * 64, kWaves * 64, s_tab, tl,
-                                     s_base[wv], s_end[wv], (uint8_t *)s_flag[wv], lane);
+                                     kBigCap, s_base[wv], s_end[wv], (uint8_t *)s_flag[wv], lane);
                 __syncthreads();
             }
             if (g != cur_g) {
@@ -660,8 +665,9 @@ __global__ __launch_bounds__(kWaves * 64) void k_count(GraphSet gs, const unsign
                 const uint32_t tl = table_log2_for(K, scale, kSliceLog2);
                 for (uint32_t i = lane; i < (1u << tl); i += 64) tab[i] = kEmpty;
                 table_insert(tab, tl, G.col, rb, K, lane, 64);
-                const unsigned long long hits = walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.n, sg.below, 0, 64,
-                                                             tab, tl, s_base[wv], s_end[wv], (uint8_t *)s_flag[wv], lane);
+                const unsigned long long hits =
+                    walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.n, sg.below, 0, 64, tab, tl, kBigCap - wv * kSlice,
+                                 s_base[wv], s_end[wv], (uint8_t *)s_flag[wv], lane);
                 if (g != cur_g) {
                     if (cur_g != kEmpty && acc != 0 && lane == 0) atomicAdd(counts + cur_g, acc);
                     cur_g = g;
