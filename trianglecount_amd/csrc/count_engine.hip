@@ -364,36 +364,55 @@ __device__ __forceinline__ void issue_round(const uint32_t *__restrict__ col, ui
 }
 
 // Probes one round through tab: linear probing in lockstep over the UNROLL
-// chunks, branch-free, the UNROLL LDS reads of a step issued together.  A lane
-// that is done (hit, or reached an empty slot) is parked on `parked`, a slot
-// outside the table that always holds kEmpty, so it keeps reading kEmpty and
-// counts nothing more: a present key lies before the first empty slot of its
-// chain, so no lane hits twice.  Positions past the window's end carry an id
-// that no table holds (kNoId: ids are below 2^32 - 2).
+// chunks, branch-free, reading aligned slot PAIRS (one 8-byte LDS read per
+// chunk and step), so a step covers two slots of a chain.  In the first pair
+// of a chain only the slots at or after the home slot count for "reached an
+// empty slot" (the slot before an odd home cannot hold the key: that would
+// need the chain to wrap the whole table, and tables are at most half full).
+// A lane that is done (hit, or reached an empty slot) is parked on `parked`,
+// a pair outside the table that always holds (kEmpty, kEmpty), so it counts
+// nothing more: a present key lies before the first empty slot of its chain,
+// so no lane hits twice.  Positions past the window's end carry an id that no
+// table holds (kNoId: ids are below 2^32 - 2).
 template <int UNROLL>
 __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint32_t B, uint32_t total, const uint32_t *tab,
                                             uint32_t mask, uint32_t shift, uint32_t parked, uint32_t lane,
                                             uint32_t &lane_hits) {
-    uint32_t x[UNROLL], h[UNROLL];
+    const uint2 *pairs = (const uint2 *)tab;
+    const uint32_t pmask = mask >> 1;
+    uint32_t x[UNROLL], p[UNROLL];
+    bool more = false;
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) {
         x[k] = B + 64u * k + lane < total ? xin[k] : kNoId;
-        h[k] = hslot(x[k], shift);
+        p[k] = hslot(x[k], shift);  // home slot for now
     }
-    for (;;) {
-        uint32_t v[UNROLL];
+    {  // first pair of every chain
+        uint2 v[UNROLL];
 #pragma unroll
-        for (int k = 0; k < UNROLL; ++k) v[k] = tab[h[k]];
-        bool more = false;
+        for (int k = 0; k < UNROLL; ++k) v[k] = pairs[p[k] >> 1];
 #pragma unroll
         for (int k = 0; k < UNROLL; ++k) {
-            const bool hit = v[k] == x[k];
-            const bool done = hit || v[k] == kEmpty;
+            const bool hit = v[k].x == x[k] || v[k].y == x[k];
+            const bool done = hit || v[k].y == kEmpty || ((p[k] & 1u) == 0 && v[k].x == kEmpty);
             lane_hits += hit ? 1u : 0u;
-            h[k] = done ? parked : ((h[k] + 1) & mask);
+            p[k] = done ? parked : (((p[k] >> 1) + 1) & pmask);
             more |= !done;
         }
-        if (ballot(more) == 0) break;
+    }
+    while (ballot(more) != 0) {
+        uint2 v[UNROLL];
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) v[k] = pairs[p[k]];
+        more = false;
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) {
+            const bool hit = v[k].x == x[k] || v[k].y == x[k];
+            const bool done = hit || v[k].x == kEmpty || v[k].y == kEmpty;
+            lane_hits += hit ? 1u : 0u;
+            p[k] = done ? parked : ((p[k] + 1) & pmask);
+            more |= !done;
+        }
     }
 }
 
@@ -576,7 +595,8 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 4) void k_count(Grap
                                                        uint32_t *__restrict__ ctl,
                                                        unsigned long long *__restrict__ counts, uint32_t scale,
                                                        uint32_t small_group) {
-    __shared__ uint32_t s_tab[kBigCap + 1];  // s_tab[kBigCap] stays kEmpty: the parking slot of every table
+    // s_tab[kBigCap .. kBigCap+1] stay kEmpty: the parking pair of every table
+    __shared__ __align__(16) uint32_t s_tab[kBigCap + 2];
     __shared__ uint32_t s_base[kWaves][64];
     __shared__ uint32_t s_end[kWaves][64];
     __shared__ __align__(16) uint32_t s_flag[kWaves][16 * UNROLL];  // 64*UNROLL bytes per wave
@@ -588,7 +608,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 4) void k_count(Grap
     // item and graph fields are wave-uniform: they live in scalar registers
     const uint32_t tid = threadIdx.x, wv = uniform(tid >> 6), lane = tid & 63;
     for (uint32_t i = lane; i < 16 * UNROLL; i += 64) s_flag[wv][i] = 0;
-    if (tid == 0) s_tab[kBigCap] = kEmpty;  // visible to all waves after the first ticket's barrier
+    if (tid < 2) s_tab[kBigCap + tid] = kEmpty;  // visible to all waves after the first ticket's barrier
     const uint32_t n_big = sload(ctl + C_NBIG), n_small = sload(ctl + C_NSMALL), n_tickets = sload(ctl + C_NTICK);
     const uint32_t small_end = n_big + sload(ctl + C_NSMALLT);  // first tiny ticket
     const uint32_t tiny0 = sload(ctl + C_TINY0), tiny_end = tiny0 + sload(ctl + C_NTINY);
@@ -618,7 +638,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 4) void k_count(Grap
                 table_insert(s_tab, tl, G.col, rb + t0, tk, tid, kWaves * 64);
                 __syncthreads();
                 hits += walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.n, sg.below, wv * 64, kWaves * 64, s_tab, tl,
-                                     kBigCap, s_base[wv], s_end[wv], (uint8_t *)s_flag[wv], lane);
+                                     kBigCap / 2, s_base[wv], s_end[wv], (uint8_t *)s_flag[wv], lane);
                 __syncthreads();
             }
             if (g != cur_g) {
@@ -666,7 +686,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 4) void k_count(Grap
                 for (uint32_t i = lane; i < (1u << tl); i += 64) tab[i] = kEmpty;
                 table_insert(tab, tl, G.col, rb, K, lane, 64);
                 const unsigned long long hits =
-                    walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.n, sg.below, 0, 64, tab, tl, kBigCap - wv * kSlice,
+                    walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.n, sg.below, 0, 64, tab, tl, (kBigCap - wv * kSlice) / 2,
                                  s_base[wv], s_end[wv], (uint8_t *)s_flag[wv], lane);
                 if (g != cur_g) {
                     if (cur_g != kEmpty && acc != 0 && lane == 0) atomicAdd(counts + cur_g, acc);
