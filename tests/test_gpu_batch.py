@@ -67,7 +67,8 @@ def test_plan_run_and_graph_replay(lstc, oracle):
 
 
 @pytest.mark.parametrize("tune", [dict(table_scale=2), dict(table_scale=8), dict(small_max=64, small_group=4),
-                                  dict(unroll=4, grid_per_cu=2), dict(small_max=512, table_scale=2)])
+                                  dict(unroll=4, grid_per_cu=2), dict(small_max=512, table_scale=2),
+                                  dict(table_scale=16)])
 def test_tuning_knobs_do_not_change_counts(lstc, oracle, tune):
     gs, want = _mixed(lstc, oracle)
     t = lstc.Tune(**tune)

@@ -110,7 +110,8 @@ def test_split_count_matches_oracle(lstc, oracle, name, builder):
     want = n * (n - 1) * (n - 2) // 6 if name.startswith("K") else oracle.count(og)
     assert lstc.count(g) == want, name
     for t in (lstc.Tune(table_scale=2), lstc.Tune(table_scale=4, small_max=64), lstc.Tune(unroll=8),
-              lstc.Tune(tiny_max=1), lstc.Tune(tiny_max=2), lstc.Tune(tiny_max=5, small_group=4)):
+              lstc.Tune(tiny_max=1), lstc.Tune(tiny_max=2), lstc.Tune(tiny_max=5, small_group=4),
+              lstc.Tune(table_scale=16, small_max=32)):
         assert lstc.count(g, tune=t) == want, (name, t.as_dict())
     # a batch mixing split and plain graphs
     plain = lstc.Graph.from_numpy(og.off, og.col)

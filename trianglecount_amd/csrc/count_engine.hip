@@ -82,7 +82,7 @@ TuneR resolve_tune(const lstc_tune *t) {
         if (t->grid_per_cu) r.grid_per_cu = t->grid_per_cu;
         if (t->unroll) r.unroll = t->unroll;
     }
-    r.table_scale = hmax(2u, hmin(8u, r.table_scale));
+    r.table_scale = hmax(2u, hmin(16u, r.table_scale));
     // a small row must fit its slice at the chosen load factor
     r.small_max = hmax(64u, hmin(kSlice / r.table_scale, r.small_max));
     r.small_group = hmax(4u, hmin(256u, r.small_group));

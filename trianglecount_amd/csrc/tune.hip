@@ -89,7 +89,7 @@ extern "C" lstc_status lstc_autotune(const lstc_graph_desc *graphs, uint32_t G, 
     };
 
     rc = measure(cur, &cur_ms, ref);
-    const uint32_t cand_scale[] = {2, 4, 8};
+    const uint32_t cand_scale[] = {4, 8, 16};
     const uint32_t cand_small[] = {64, 128, 256};
     const uint32_t cand_group[] = {4, 16, 64};
     const uint32_t cand_grid[] = {4, 6, 8};
