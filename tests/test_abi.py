@@ -58,3 +58,44 @@ def test_workspace_queries_are_host_only():
     assert lib.lstc_clean_orient_workspace(1000, 5000, ctypes.byref(nb)) == 0
     assert nb.value >= 2 * 5000 * 8 * 2
     assert lib.lstc_count_workspace(1, 1, 0, None, None) == _lib.LSTC_ERR_INVALID
+
+
+# Shape validation happens on the host before any HIP call, so these run without
+# a device.  The pointers are placeholders that are never dereferenced.
+_FAKE = ctypes.c_void_p(0x10000)
+
+
+def _status(st):
+    return _lib.load().lstc_status_string(st).decode()
+
+
+def test_count_rejects_bad_shapes_on_host():
+    lib = _lib.load()
+    ws = ctypes.c_size_t(1 << 20)
+    # n_owned > n_rows
+    assert lib.lstc_count(_FAKE, _FAKE, 10, 11, 20, _FAKE, None, None, _FAKE, ws, None) == _lib.LSTC_ERR_INVALID
+    # m beyond the 32-bit work positions
+    assert lib.lstc_count(_FAKE, _FAKE, 10, 10, 0xFFFFF000, _FAKE, None, None, _FAKE, ws, None) == _lib.LSTC_ERR_INVALID
+    # no output
+    assert lib.lstc_count(_FAKE, _FAKE, 10, 10, 20, None, None, None, _FAKE, ws, None) == _lib.LSTC_ERR_INVALID
+    # split count without in-lists
+    assert lib.lstc_count_split(_FAKE, _FAKE, None, None, 10, 10, 20, _FAKE, None, _FAKE, ws,
+                                None) == _lib.LSTC_ERR_INVALID
+    assert "lstc_count" in lib.lstc_last_error().decode()
+
+
+def test_batch_and_in_lists_reject_bad_shapes_on_host():
+    lib = _lib.load()
+    descs = (_lib.GraphDesc * 2)()
+    for d in descs:
+        d.off, d.col, d.m, d.n_rows, d.n_owned = 0x10000, 0x10000, 20, 10, 10
+    descs[1].in_off = 0x10000  # in_off without in_col
+    nb = ctypes.c_size_t()
+    assert lib.lstc_count_batch(descs, 2, _FAKE, None, _FAKE, ctypes.c_size_t(1 << 20), None) == _lib.LSTC_ERR_INVALID
+    big = (_lib.GraphDesc * 2048)()
+    assert lib.lstc_count_batch(big, 2048, _FAKE, None, _FAKE, ctypes.c_size_t(1 << 20), None) == _lib.LSTC_ERR_INVALID
+    assert lib.lstc_in_lists_workspace(10, 11, 20, ctypes.byref(nb)) == _lib.LSTC_ERR_INVALID
+    k = ctypes.c_uint64()
+    assert lib.lstc_in_lists(_FAKE, _FAKE, 10, 11, 20, _FAKE, _FAKE, ctypes.byref(k), _FAKE, ctypes.c_size_t(1 << 20),
+                             None) == _lib.LSTC_ERR_INVALID
+    assert lib.lstc_in_lists_workspace(10, 10, 20, ctypes.byref(nb)) == _lib.LSTC_OK and nb.value > 0
