@@ -284,3 +284,12 @@ def test_owned_in_lists_helper_by_brute_force(oracle, cut_frac):
     got_off, got_col = G.owned_in_lists(g.off, g.col, cut)
     assert np.array_equal(got_off, want_off)
     assert np.array_equal(got_col, want_col)
+
+
+def test_rmat_golden_counts_reproduce(oracle):
+    """The committed full-size counts come from this oracle: the small scales regenerate exactly."""
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "rmat_counts.json")))
+    for s in (16, 17, 18):
+        n, src, dst = oracle.gen_rmat(s, 16, seed=1)
+        _, g = oracle.clean_orient(n, src, dst)
+        assert (g.n, g.m, oracle.count(g)) == (gold[str(s)]["n"], gold[str(s)]["m"], gold[str(s)]["triangles"])

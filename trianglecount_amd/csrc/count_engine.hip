@@ -374,10 +374,16 @@ __device__ __forceinline__ void issue_round(const uint32_t *__restrict__ col, ui
 // nothing more: a present key lies before the first empty slot of its chain,
 // so no lane hits twice.  Positions past the window's end carry an id that no
 // table holds (kNoId: ids are below 2^32 - 2).
+// After the first pair most lanes are done; when at most 64 of the round's
+// lanes still probe, they are gathered (id, next pair) through the wave's
+// staging arrays into one 64-lane chunk, so the remaining steps cost one chunk
+// instead of UNROLL.  The staging arrays are the window's segment tables,
+// which the round no longer reads (the segment bases and ends are in
+// registers by then).
 template <int UNROLL>
 __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint32_t B, uint32_t total, const uint32_t *tab,
                                             uint32_t mask, uint32_t shift, uint32_t parked, uint32_t lane,
-                                            uint32_t &lane_hits) {
+                                            uint32_t &lane_hits, uint32_t *stage_x, uint32_t *stage_p) {
     const uint2 *pairs = (const uint2 *)tab;
     const uint32_t pmask = mask >> 1;
     uint32_t x[UNROLL], p[UNROLL];
@@ -387,6 +393,7 @@ __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint3
         x[k] = B + 64u * k + lane < total ? xin[k] : kNoId;
         p[k] = hslot(x[k], shift);  // home slot for now
     }
+    bool done[UNROLL];
     {  // first pair of every chain
         uint2 v[UNROLL];
 #pragma unroll
@@ -394,11 +401,45 @@ __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint3
 #pragma unroll
         for (int k = 0; k < UNROLL; ++k) {
             const bool hit = v[k].x == x[k] || v[k].y == x[k];
-            const bool done = hit || v[k].y == kEmpty || ((p[k] & 1u) == 0 && v[k].x == kEmpty);
+            done[k] = hit || v[k].y == kEmpty || ((p[k] & 1u) == 0 && v[k].x == kEmpty);
             lane_hits += hit ? 1u : 0u;
-            p[k] = done ? parked : (((p[k] >> 1) + 1) & pmask);
-            more |= !done;
+            p[k] = done[k] ? parked : (((p[k] >> 1) + 1) & pmask);
+            more |= !done[k];
         }
+    }
+    uint64_t M[UNROLL];
+    uint32_t live = 0;
+#pragma unroll
+    for (int k = 0; k < UNROLL; ++k) {
+        M[k] = ballot(!done[k]);
+        live += (uint32_t)__popcll(M[k]);
+    }
+    if (live == 0) return;
+    if (live <= 64) {
+        // gather the lanes still probing into one 64-lane chunk (staging in LDS)
+        uint32_t at = 0;
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) {
+            if (!done[k]) {
+                const uint32_t slot = at + mask_rank(M[k]);
+                stage_x[slot] = x[k];
+                stage_p[slot] = p[k];
+            }
+            at += (uint32_t)__popcll(M[k]);
+        }
+        const bool act = lane < live;
+        const uint32_t xv = act ? stage_x[lane] : kNoId;
+        uint32_t pv = act ? stage_p[lane] : parked;
+        bool go = act;
+        while (ballot(go) != 0) {
+            const uint2 v = pairs[pv];
+            const bool hit = v.x == xv || v.y == xv;
+            const bool dn = hit || v.x == kEmpty || v.y == kEmpty;
+            lane_hits += hit ? 1u : 0u;
+            pv = dn ? parked : ((pv + 1) & pmask);
+            go = !dn;
+        }
+        return;
     }
     while (ballot(more) != 0) {
         uint2 v[UNROLL];
@@ -408,10 +449,10 @@ __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint3
 #pragma unroll
         for (int k = 0; k < UNROLL; ++k) {
             const bool hit = v[k].x == x[k] || v[k].y == x[k];
-            const bool done = hit || v[k].x == kEmpty || v[k].y == kEmpty;
+            const bool dn = hit || v[k].x == kEmpty || v[k].y == kEmpty;
             lane_hits += hit ? 1u : 0u;
-            p[k] = done ? parked : ((p[k] + 1) & pmask);
-            more |= !done;
+            p[k] = dn ? parked : ((p[k] + 1) & pmask);
+            more |= !dn;
         }
     }
 }
@@ -463,16 +504,16 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
         for (uint32_t B = 0;; B += 2 * RB) {
             if (B + RB < total) {
                 issue_round<UNROLL>(col, B + RB, total, mybase, myend, flag, lane, xb);
-                probe_round<UNROLL>(xa, B, total, tab, mask, shift, parked, lane, lane_hits);
+                probe_round<UNROLL>(xa, B, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
             } else {
-                probe_round<UNROLL>(xa, B, total, tab, mask, shift, parked, lane, lane_hits);
+                probe_round<UNROLL>(xa, B, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
                 break;
             }
             if (B + 2 * RB < total) {
                 issue_round<UNROLL>(col, B + 2 * RB, total, mybase, myend, flag, lane, xa);
-                probe_round<UNROLL>(xb, B + RB, total, tab, mask, shift, parked, lane, lane_hits);
+                probe_round<UNROLL>(xb, B + RB, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
             } else {
-                probe_round<UNROLL>(xb, B + RB, total, tab, mask, shift, parked, lane, lane_hits);
+                probe_round<UNROLL>(xb, B + RB, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
                 break;
             }
         }
