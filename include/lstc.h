@@ -191,7 +191,10 @@ lstc_status lstc_count_batch(const lstc_graph_desc *graphs, uint32_t G, uint64_t
 
 /* A plan owns device copies of the descriptors and its workspace (allocated at
  * creation), so running it is pure kernel enqueues; launch_graph captures that
- * sequence once into a hipGraph (per d_counts) and replays it. */
+ * sequence once into a hipGraph (per d_counts) and replays it.  One plan has
+ * one workspace: runs of the same plan must not overlap (use one stream, or
+ * one plan per stream).  The graphs' CSRs must stay allocated while the plan
+ * is used. */
 typedef struct lstc_batch_plan lstc_batch_plan;
 lstc_status lstc_batch_plan_create(const lstc_graph_desc *graphs, uint32_t G, const lstc_tune *tune,
                                    lstc_batch_plan **out);

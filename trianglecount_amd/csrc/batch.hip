@@ -160,6 +160,8 @@ extern "C" lstc_status lstc_batch_plan_run(lstc_batch_plan *p, uint64_t *d_count
 extern "C" lstc_status lstc_batch_plan_launch_graph(lstc_batch_plan *p, uint64_t *d_counts, lstc_stream s) {
     if (!p || !d_counts) return fail(LSTC_ERR_INVALID, "lstc_batch_plan_launch_graph: NULL argument");
     if (!p->exec || p->captured_counts != d_counts) {
+        // a replay of the old graph may still be queued: let it finish first
+        if (p->exec) LSTC_HIP(hipDeviceSynchronize());
         if (p->exec) { (void)hipGraphExecDestroy(p->exec); p->exec = nullptr; }
         if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
         LSTC_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
