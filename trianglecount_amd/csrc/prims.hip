@@ -163,14 +163,23 @@ __global__ __launch_bounds__(kSortWaves * 64) void k_radix_hist(const uint64_t *
     hist[(uint64_t)threadIdx.x * num_tiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// Scatter of one tile: keys are ranked stably per digit (ballot match inside a
+// wave, running counts across rounds and waves), staged in LDS in digit order,
+// then written out by consecutive threads, so each digit's run of the tile
+// leaves as contiguous stores instead of one scattered store per key.
 __global__ __launch_bounds__(kSortWaves * 64) void k_radix_scatter(
     const uint64_t *__restrict__ keys, uint64_t *__restrict__ out, uint64_t n, uint32_t shift,
     uint32_t num_tiles, const uint64_t *__restrict__ goff) {
     __shared__ uint32_t wcnt[kSortWaves][256];
-    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int w = 0; w < kSortWaves; ++w) wcnt[w][threadIdx.x] = 0;
+    __shared__ uint32_t dstart[256];
+    __shared__ uint32_t wsum[kSortWaves];
+    __shared__ uint64_t gbase[256];
+    __shared__ uint64_t stage[kSortTile];
+    const uint32_t tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    for (int w = 0; w < kSortWaves; ++w) wcnt[w][tid] = 0;
     __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * kSortTile + (uint64_t)wv * (kSortRounds * 64);
+    const uint64_t tile0 = (uint64_t)blockIdx.x * kSortTile;
+    const uint64_t base = tile0 + (uint64_t)wv * (kSortRounds * 64);
     uint64_t k[kSortRounds];
     uint32_t rank[kSortRounds];
 #pragma unroll
@@ -192,22 +201,36 @@ __global__ __launch_bounds__(kSortWaves * 64) void k_radix_scatter(
         if (valid && mask_rank(peers) == 0) wcnt[wv][d] = before + (uint32_t)__popcll(peers);
     }
     __syncthreads();
-    {
-        uint32_t run = 0;
-        for (int w = 0; w < kSortWaves; ++w) {
-            uint32_t t = wcnt[w][threadIdx.x];
-            wcnt[w][threadIdx.x] = run;
-            run += t;
-        }
+    // thread tid owns digit tid: offsets of the waves inside the digit, the
+    // digit's start inside the tile, and its global base for this tile
+    uint32_t tot = 0;
+    for (int w = 0; w < kSortWaves; ++w) {
+        const uint32_t t = wcnt[w][tid];
+        wcnt[w][tid] = tot;
+        tot += t;
     }
+    const uint32_t incl = wave_incl_scan(tot, lane);
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wv; ++w) before += wsum[w];
+    const uint32_t start = before + incl - tot;
+    dstart[tid] = start;
+    gbase[tid] = goff[(uint64_t)tid * num_tiles + blockIdx.x] - start;
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kSortRounds; ++r) {
         const uint64_t i = base + (uint64_t)r * 64 + lane;
         if (i < n) {
             const uint32_t d = (uint32_t)(k[r] >> shift) & 255u;
-            out[goff[(uint64_t)d * num_tiles + blockIdx.x] + wcnt[wv][d] + rank[r]] = k[r];
+            stage[dstart[d] + wcnt[wv][d] + rank[r]] = k[r];
         }
+    }
+    __syncthreads();
+    const uint32_t tile_n = (uint32_t)(n - tile0 < kSortTile ? n - tile0 : kSortTile);
+    for (uint32_t i = tid; i < tile_n; i += kSortWaves * 64) {
+        const uint64_t key = stage[i];
+        out[gbase[(uint32_t)(key >> shift) & 255u] + i] = key;
     }
 }
 
