@@ -14,11 +14,12 @@
 // u -> w with d+(u) <= d+(w)), so each arc streams its shorter list.  Without
 // in-lists every arc is a kind A arc.
 //
-// Plan (k_plan_rows): owned rows with d+ >= 2 of every graph of the launch emit
-// one kind A item and ceil(in-degree / kSegChunk) kind B items, binned by the
-// hashed row's d+ into one queue, largest bin first.  Items of rows above
-// small_max are "big" tickets (one item each); smaller ones form "small"
-// tickets of small_group items.
+// Plan (k_plan_rows): every planned row with d+ >= 2 emits its kind A item
+// (source side; owned rows only) and kind B items (one kInChunk-source chunk of
+// its in-list each), binned by the hashed row's d+ into one queue, largest bin
+// first; rows with d+ <= tiny_max go to the tiny region
+// instead.  Items of rows above small_max are "big" tickets (one item each);
+// smaller ones form "small" tickets of small_group items.
 //
 // Count (k_count, one persistent launch): each workgroup of kWaves waves takes
 // tickets from an atomic head.
@@ -48,7 +49,17 @@ static constexpr uint32_t kSlice = 1u << kSliceLog2;  // wave-private slice (sma
 static constexpr uint32_t kBigLog2 = 12;
 static constexpr uint32_t kBigCap = 1u << kBigLog2;   // shared table (big items)
 static constexpr uint32_t kBigTile = kBigCap / 2;     // keys per table fill of a big row
-static constexpr uint32_t kSegChunk = 4096;           // in-neighbour segments per kind B item
+// A row's in-list is split into kind B items of kInChunk sources, so that one
+// hub with a short list and a huge in-list (power-law graphs) becomes many
+// items instead of one straggler.  Larger chunks only when the chunk index
+// would not fit its 20-bit field.
+static constexpr uint32_t kInChunk = 1024;
+static constexpr uint32_t kMaxChunks = (1u << 20) - 1;
+
+__host__ __device__ __forceinline__ uint32_t in_chunk(uint64_t in_degree) {
+    const uint64_t need = (in_degree + kMaxChunks - 1) / kMaxChunks;
+    return need <= kInChunk ? kInChunk : (uint32_t)((need + 63u) & ~63ull);
+}
 static constexpr int kBins = 6;                       // hash-path bins
 static constexpr int kTinyBin = kBins;                // the lane-per-arc rows
 static constexpr int kSlots = kBins + 1;              // queue regions
@@ -206,8 +217,9 @@ __global__ __launch_bounds__(256) void k_plan_rows(GraphSet gs, Bounds bd, uint3
                     tiny_kind = owned ? 0u : 1u;
                 }
             } else if (k >= 2) {
+                const uint32_t sb = in_chunk(kb);
                 n_a = owned ? 1u : 0u;
-                n_b = (uint32_t)((kb + kSegChunk - 1) / kSegChunk);
+                n_b = (uint32_t)((kb + sb - 1) / sb);
                 if (n_a + n_b) {
                     b = kBins - 1;
                     for (int i = kBins - 2; i >= 0; --i)
@@ -534,8 +546,9 @@ __device__ __forceinline__ Segs item_segs(const GraphDev &G, uint32_t kind, uint
                                           uint32_t K) {
     if (kind == 0) return Segs{G.col + rb, K, G.in_off ? K : 0u};
     const uint64_t ib = sload(G.in_off + r), ie = sload(G.in_off + r + 1);
-    const uint64_t c0 = ib + (uint64_t)chunk * kSegChunk;
-    const uint64_t c1 = (c0 + kSegChunk < ie) ? c0 + kSegChunk : ie;
+    const uint32_t sb = in_chunk(ie - ib);
+    const uint64_t c0 = ib + (uint64_t)chunk * sb;
+    const uint64_t c1 = (c0 + sb < ie) ? c0 + sb : ie;
     return Segs{G.in_col + c0, (uint32_t)(c1 - c0), 0u};
 }
 
@@ -754,8 +767,8 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 4) void k_count(Grap
 }
 
 // ------------------------------------------------------------------ host side
-// queue entries: one kind A item per row plus at most rows + arcs / kSegChunk kind B items
-static uint64_t queue_cap(uint64_t total_rows, uint64_t total_m) { return 2 * total_rows + total_m / kSegChunk + 1; }
+// queue entries: per row one kind A item and at most 1 + in-degree / kInChunk kind B items
+static uint64_t queue_cap(uint64_t total_rows, uint64_t total_m) { return 2 * total_rows + total_m / kInChunk + 1; }
 
 size_t count_ws_bytes(uint32_t G, uint64_t total_rows, uint64_t total_m) {
     (void)G;
