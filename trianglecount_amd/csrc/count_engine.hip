@@ -344,7 +344,9 @@ struct FlagWord<8> {
     using T = uint64_t;
 };
 
-template <int UNROLL>
+// FULL: the round lies wholly before `total` (every round of a window but the
+// last), so no position needs the past-the-end handling.
+template <int UNROLL, bool FULL>
 __device__ __forceinline__ void issue_round(const uint32_t *__restrict__ col, uint32_t B, uint32_t total,
                                             uint32_t mybase, uint32_t myend, uint8_t *flag, uint32_t lane,
                                             uint32_t (&x)[UNROLL]) {
@@ -371,7 +373,7 @@ __device__ __forceinline__ void issue_round(const uint32_t *__restrict__ col, ui
     for (int k = 0; k < UNROLL; ++k) {
         const uint32_t base = gather(mybase, idx[k]);  // lanes past the end: any base, not probed
         const uint32_t t = B + 64u * k + lane;
-        x[k] = gload(t < total ? col + (uint32_t)(base + t) : col);
+        x[k] = gload(FULL || t < total ? col + (uint32_t)(base + t) : col);
     }
 }
 
@@ -392,7 +394,7 @@ __device__ __forceinline__ void issue_round(const uint32_t *__restrict__ col, ui
 // instead of UNROLL.  The staging arrays are the window's segment tables,
 // which the round no longer reads (the segment bases and ends are in
 // registers by then).
-template <int UNROLL>
+template <int UNROLL, bool FULL>
 __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint32_t B, uint32_t total, const uint32_t *tab,
                                             uint32_t mask, uint32_t shift, uint32_t parked, uint32_t lane,
                                             uint32_t &lane_hits, uint32_t *stage_x, uint32_t *stage_p) {
@@ -402,7 +404,7 @@ __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint3
     bool more = false;
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) {
-        x[k] = B + 64u * k + lane < total ? xin[k] : kNoId;
+        x[k] = FULL || B + 64u * k + lane < total ? xin[k] : kNoId;
         p[k] = hslot(x[k], shift);  // home slot for now
     }
     bool done[UNROLL];
@@ -509,23 +511,32 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
         const uint32_t mybase = s_base[lane];
         const uint32_t myend = lane < nseg ? s_end[lane] : kEmpty;
         uint32_t xa[UNROLL], xb[UNROLL];
-        issue_round<UNROLL>(col, 0, total, mybase, myend, flag, lane, xa);
-        // The probe is written out in both arms of each branch: a join between
+        if (RB <= total)
+            issue_round<UNROLL, true>(col, 0, total, mybase, myend, flag, lane, xa);
+        else
+            issue_round<UNROLL, false>(col, 0, total, mybase, myend, flag, lane, xa);
+        // The probe is written out in every arm of each branch: a join between
         // an issue and the probe would make the wait for the probed buffer also
-        // wait for the loads just issued.
+        // wait for the loads just issued.  A round followed by another is full.
         for (uint32_t B = 0;; B += 2 * RB) {
-            if (B + RB < total) {
-                issue_round<UNROLL>(col, B + RB, total, mybase, myend, flag, lane, xb);
-                probe_round<UNROLL>(xa, B, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
+            if (B + 2 * RB <= total) {
+                issue_round<UNROLL, true>(col, B + RB, total, mybase, myend, flag, lane, xb);
+                probe_round<UNROLL, true>(xa, B, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
+            } else if (B + RB < total) {
+                issue_round<UNROLL, false>(col, B + RB, total, mybase, myend, flag, lane, xb);
+                probe_round<UNROLL, true>(xa, B, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
             } else {
-                probe_round<UNROLL>(xa, B, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
+                probe_round<UNROLL, false>(xa, B, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
                 break;
             }
-            if (B + 2 * RB < total) {
-                issue_round<UNROLL>(col, B + 2 * RB, total, mybase, myend, flag, lane, xa);
-                probe_round<UNROLL>(xb, B + RB, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
+            if (B + 3 * RB <= total) {
+                issue_round<UNROLL, true>(col, B + 2 * RB, total, mybase, myend, flag, lane, xa);
+                probe_round<UNROLL, true>(xb, B + RB, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
+            } else if (B + 2 * RB < total) {
+                issue_round<UNROLL, false>(col, B + 2 * RB, total, mybase, myend, flag, lane, xa);
+                probe_round<UNROLL, true>(xb, B + RB, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
             } else {
-                probe_round<UNROLL>(xb, B + RB, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
+                probe_round<UNROLL, false>(xb, B + RB, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
                 break;
             }
         }
