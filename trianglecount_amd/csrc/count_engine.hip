@@ -325,11 +325,13 @@ __device__ __forceinline__ void table_insert(uint32_t *tab, uint32_t tlog2, cons
 // wave holds segment i: mybase = its column start minus its flat start (mod
 // 2^32: column indices are below 2^32 because m < 2^32 - 4096), myend =
 // its exclusive flat end (kEmpty for lanes without a segment); ends ascend.
-// The ends that fall inside the round are marked in `flag`, one byte per
-// position stored lane-major (position p at (p % 64) * UNROLL + p / 64), so a
-// single LDS read gives each lane its UNROLL chunk flags; the flags are cleared
-// again right after.  Each chunk then finds a lane's segment with a ballot and
-// a popcount, and the bases of all chunks are gathered back to back.
+// The last entry of every segment that ends inside the round is marked in
+// `flag`, one byte per position stored lane-major (position p at
+// (p % 64) * UNROLL + p / 64), so a single LDS read gives each lane its UNROLL
+// chunk flags; the flags are cleared again right after.  A position's segment
+// is then the round's first segment plus the marks strictly before it: one
+// ballot per chunk and one mbcnt pair per lane; the bases of all chunks are
+// gathered back to back.
 // Positions at or past `total` load col[0] (in bounds, since total > 0 means the
 // graph has arcs) and are never probed: every load is unconditional, so its
 // result register is never copied and the loads stay in flight until probed.
@@ -354,19 +356,18 @@ __device__ __forceinline__ void issue_round(const uint32_t *__restrict__ col, ui
     constexpr uint32_t RB = 64u * UNROLL;
     // segments that end at or before B lie wholly before this round
     uint32_t sg = (uint32_t)__popcll(ballot(myend <= B));
-    // an end e inside the round: position e already belongs to the next segment
-    const bool mk = myend > B && myend < B + RB;
-    const uint32_t e = myend - B;
+    // a segment ending inside the round (end e in (B, B + RB]) marks position e - 1
+    const bool mk = myend > B && myend <= B + RB;
+    const uint32_t e = myend - 1u - B;
     const uint32_t fi = (e & 63u) * UNROLL + (e >> 6);
     if (mk) flag[fi] = 1;
     const FW fw = ((const FW *)flag)[lane];
     if (mk) flag[fi] = 0;
-    const uint64_t upto = (2ull << lane) - 1ull;  // this lane and the ones below
     uint32_t idx[UNROLL];
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) {
         const uint64_t M = ballot(((fw >> (8 * k)) & 0xFFu) != 0);
-        idx[k] = sg + (uint32_t)__popcll(M & upto);
+        idx[k] = mask_rank_from(M, sg);
         sg += (uint32_t)__popcll(M);
     }
 #pragma unroll

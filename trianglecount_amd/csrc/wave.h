@@ -32,6 +32,10 @@ __device__ __forceinline__ uint32_t mask_rank(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
+// base + number of set bits of mask in lanes strictly below this lane.
+__device__ __forceinline__ uint32_t mask_rank_from(uint64_t mask, uint32_t base) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, base));
+}
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) {
