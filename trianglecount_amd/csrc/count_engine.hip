@@ -158,7 +158,11 @@ __device__ __forceinline__ uint32_t table_log2_for(uint32_t keys, uint32_t scale
 
 __device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 
-__device__ __forceinline__ uint32_t hslot(uint32_t x, uint32_t shift) { return (x * 0x9E3779B1u) >> shift; }
+// Tables are arrays of 4-slot buckets (16 bytes, aligned); a key's home is a
+// bucket (Fibonacci hash: qshift = 34 - log2(slots)).  Linear probing runs
+// over buckets and fills a bucket's slots in order, so a bucket whose last slot
+// is empty ends every chain through it.
+__device__ __forceinline__ uint32_t hquad(uint32_t x, uint32_t qshift) { return (x * 0x9E3779B1u) >> qshift; }
 
 // ------------------------------------------------------------------ planning
 __global__ void k_plan_init(uint32_t *ctl, unsigned long long *counts, uint32_t G) {
@@ -311,11 +315,18 @@ __global__ __launch_bounds__(1024) void k_plan_scan(uint32_t *__restrict__ wcnt,
 // ------------------------------------------------------------------ table build
 __device__ __forceinline__ void table_insert(uint32_t *tab, uint32_t tlog2, const uint32_t *__restrict__ col,
                                              uint64_t b0, uint32_t nk, uint32_t tid, uint32_t nthreads) {
-    const uint32_t mask = (1u << tlog2) - 1, shift = 32 - tlog2;
+    const uint32_t qmask = (1u << (tlog2 - 2)) - 1, qshift = 34 - tlog2;
     for (uint32_t j = tid; j < nk; j += nthreads) {
         const uint32_t key = gload(col + b0 + j);
-        uint32_t h = hslot(key, shift);
-        while (atomicCAS(&tab[h], kEmpty, key) != kEmpty) h = (h + 1) & mask;
+        uint32_t h = hquad(key, qshift);
+        // a slot is tried only once the ones before it in the bucket are taken
+        for (;;) {
+            uint32_t *b = tab + 4 * h;
+            if (atomicCAS(b, kEmpty, key) == kEmpty || atomicCAS(b + 1, kEmpty, key) == kEmpty ||
+                atomicCAS(b + 2, kEmpty, key) == kEmpty || atomicCAS(b + 3, kEmpty, key) == kEmpty)
+                break;
+            h = (h + 1) & qmask;
+        }
     }
 }
 
@@ -346,128 +357,106 @@ struct FlagWord<8> {
     using T = uint64_t;
 };
 
+// The round's first segment (segments that end at or before B lie wholly
+// before it) and whether this lane's segment ends inside the round (end e in
+// (B, B + RB): it marks position e - 1; an end at B + RB marks the round's last
+// position, before which every position already lies, so it needs no mark).
+template <int UNROLL>
+__device__ __forceinline__ uint32_t round_first(uint32_t B, uint32_t myend, bool &mk) {
+    constexpr uint32_t RB = 64u * UNROLL;
+    mk = myend > B && myend < B + RB;
+    return (uint32_t)__popcll(ballot(myend <= B));
+}
+
+// A full round that lies wholly in one segment: its column index at position
+// B + 64k is c0 + 64k (uniform, below m), so only the lane offset is per lane.
+template <int UNROLL>
+__device__ __forceinline__ void issue_single(const uint32_t *__restrict__ col, uint32_t c0, uint32_t lane,
+                                             uint32_t (&x)[UNROLL]) {
+#pragma unroll
+    for (int k = 0; k < UNROLL; ++k) x[k] = gload(col + (c0 + 64u * k) + lane);
+}
+
 // FULL: the round lies wholly before `total` (every round of a window but the
-// last), so no position needs the past-the-end handling.
+// last), so no position needs the past-the-end handling.  sg and mk come from
+// round_first.
 template <int UNROLL, bool FULL>
 __device__ __forceinline__ void issue_round(const uint32_t *__restrict__ col, uint32_t B, uint32_t total,
-                                            uint32_t mybase, uint32_t myend, uint8_t *flag, uint32_t lane,
-                                            uint32_t (&x)[UNROLL]) {
+                                            uint32_t mybase, uint32_t myend, uint32_t sg, bool mk, uint8_t *flag,
+                                            uint32_t lane, uint32_t (&x)[UNROLL]) {
     using FW = typename FlagWord<UNROLL>::T;
-    constexpr uint32_t RB = 64u * UNROLL;
-    // segments that end at or before B lie wholly before this round
-    uint32_t sg = (uint32_t)__popcll(ballot(myend <= B));
-    // a segment ending inside the round (end e in (B, B + RB]) marks position e - 1
-    const bool mk = myend > B && myend <= B + RB;
     const uint32_t e = myend - 1u - B;
     const uint32_t fi = (e & 63u) * UNROLL + (e >> 6);
     if (mk) flag[fi] = 1;
     const FW fw = ((const FW *)flag)[lane];
     if (mk) flag[fi] = 0;
-    uint32_t idx[UNROLL];
+    uint32_t src[UNROLL];  // byte address of the position's segment lane (for ds_bpermute)
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) {
         const uint64_t M = ballot(((fw >> (8 * k)) & 0xFFu) != 0);
-        idx[k] = mask_rank_from(M, sg);
+        src[k] = (mask_rank(M) << 2) + (sg << 2);
         sg += (uint32_t)__popcll(M);
     }
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) {
-        const uint32_t base = gather(mybase, idx[k]);  // lanes past the end: any base, not probed
+        // lanes past the end: any base, not probed
+        const uint32_t base = (uint32_t)__builtin_amdgcn_ds_bpermute((int)src[k], (int)mybase);
         const uint32_t t = B + 64u * k + lane;
         x[k] = gload(FULL || t < total ? col + (uint32_t)(base + t) : col);
     }
 }
 
-// Probes one round through tab: linear probing in lockstep over the UNROLL
-// chunks, branch-free, reading aligned slot PAIRS (one 8-byte LDS read per
-// chunk and step), so a step covers two slots of a chain.  In the first pair
-// of a chain only the slots at or after the home slot count for "reached an
-// empty slot" (the slot before an odd home cannot hold the key: that would
-// need the chain to wrap the whole table, and tables are at most half full).
-// A lane that is done (hit, or reached an empty slot) is parked on `parked`,
-// a pair outside the table that always holds (kEmpty, kEmpty), so it counts
-// nothing more: a present key lies before the first empty slot of its chain,
-// so no lane hits twice.  Positions past the window's end carry an id that no
-// table holds (kNoId: ids are below 2^32 - 2).
-// After the first pair most lanes are done; when at most 64 of the round's
-// lanes still probe, they are gathered (id, next pair) through the wave's
-// staging arrays into one 64-lane chunk, so the remaining steps cost one chunk
-// instead of UNROLL.  The staging arrays are the window's segment tables,
-// which the round no longer reads (the segment bases and ends are in
-// registers by then).
+// Probes one round through tab: linear probing over 4-slot buckets, the
+// first bucket of all UNROLL chunks in lockstep, one 16-byte LDS read per
+// chunk.  A lane is done once it hits or reads a bucket whose last slot is
+// empty (no chain continues past such a bucket; tables are at most half full,
+// so every chain has one).  Positions past the window's end carry an id that
+// no table holds (kNoId: ids are below 2^32 - 2).
+// First-bucket hits are counted per wave: each compare is balloted on its own
+// (a ballot of one compare is the compare's lane mask; a ballot of a combined
+// condition would go through a VGPR), the masks are combined in scalar
+// registers and their popcount is added to `hits`.  The few lanes whose chain
+// goes on (about one lane in a hundred at the default table scale) continue
+// alone, chunk by chunk, and count their hits in `lane_hits`.
+__device__ __forceinline__ uint64_t bucket_hits(const uint4 &v, uint32_t x) {
+    return ballot(v.x == x) | ballot(v.y == x) | ballot(v.z == x) | ballot(v.w == x);
+}
+
 template <int UNROLL, bool FULL>
 __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint32_t B, uint32_t total, const uint32_t *tab,
-                                            uint32_t mask, uint32_t shift, uint32_t parked, uint32_t lane,
-                                            uint32_t &lane_hits, uint32_t *stage_x, uint32_t *stage_p) {
-    const uint2 *pairs = (const uint2 *)tab;
-    const uint32_t pmask = mask >> 1;
+                                            uint32_t qmask, uint32_t qshift, uint32_t lane, uint32_t &hits,
+                                            uint32_t &lane_hits) {
+    const uint4 *buckets = (const uint4 *)tab;
+    const uint64_t active = ballot(true);
     uint32_t x[UNROLL], p[UNROLL];
-    bool more = false;
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) {
         x[k] = FULL || B + 64u * k + lane < total ? xin[k] : kNoId;
-        p[k] = hslot(x[k], shift);  // home slot for now
+        p[k] = hquad(x[k], qshift);
     }
-    bool done[UNROLL];
-    {  // first pair of every chain
-        uint2 v[UNROLL];
+    uint4 v[UNROLL];
 #pragma unroll
-        for (int k = 0; k < UNROLL; ++k) v[k] = pairs[p[k] >> 1];
-#pragma unroll
-        for (int k = 0; k < UNROLL; ++k) {
-            const bool hit = v[k].x == x[k] || v[k].y == x[k];
-            done[k] = hit || v[k].y == kEmpty || ((p[k] & 1u) == 0 && v[k].x == kEmpty);
-            lane_hits += hit ? 1u : 0u;
-            p[k] = done[k] ? parked : (((p[k] >> 1) + 1) & pmask);
-            more |= !done[k];
-        }
-    }
-    uint64_t M[UNROLL];
-    uint32_t live = 0;
+    for (int k = 0; k < UNROLL; ++k) v[k] = buckets[p[k]];
+    uint64_t going[UNROLL], any = 0;  // lanes whose chain goes on
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) {
-        M[k] = ballot(!done[k]);
-        live += (uint32_t)__popcll(M[k]);
+        const uint64_t H = bucket_hits(v[k], x[k]);
+        hits += (uint32_t)__popcll(H);
+        going[k] = active & ~(H | ballot(v[k].w == kEmpty));
+        any |= going[k];
     }
-    if (live == 0) return;
-    if (live <= 64) {
-        // gather the lanes still probing into one 64-lane chunk (staging in LDS)
-        uint32_t at = 0;
+    if (any == 0) return;
 #pragma unroll
-        for (int k = 0; k < UNROLL; ++k) {
-            if (!done[k]) {
-                const uint32_t slot = at + mask_rank(M[k]);
-                stage_x[slot] = x[k];
-                stage_p[slot] = p[k];
+    for (int k = 0; k < UNROLL; ++k) {
+        if (lane_in(going[k])) {
+            uint32_t q = (p[k] + 1) & qmask;
+            for (;;) {
+                const uint4 w = buckets[q];
+                const bool hit = w.x == x[k] || w.y == x[k] || w.z == x[k] || w.w == x[k];
+                lane_hits += hit ? 1u : 0u;
+                if (hit || w.w == kEmpty) break;
+                q = (q + 1) & qmask;
             }
-            at += (uint32_t)__popcll(M[k]);
-        }
-        const bool act = lane < live;
-        const uint32_t xv = act ? stage_x[lane] : kNoId;
-        uint32_t pv = act ? stage_p[lane] : parked;
-        bool go = act;
-        while (ballot(go) != 0) {
-            const uint2 v = pairs[pv];
-            const bool hit = v.x == xv || v.y == xv;
-            const bool dn = hit || v.x == kEmpty || v.y == kEmpty;
-            lane_hits += hit ? 1u : 0u;
-            pv = dn ? parked : ((pv + 1) & pmask);
-            go = !dn;
-        }
-        return;
-    }
-    while (ballot(more) != 0) {
-        uint2 v[UNROLL];
-#pragma unroll
-        for (int k = 0; k < UNROLL; ++k) v[k] = pairs[p[k]];
-        more = false;
-#pragma unroll
-        for (int k = 0; k < UNROLL; ++k) {
-            const bool hit = v[k].x == x[k] || v[k].y == x[k];
-            const bool dn = hit || v[k].x == kEmpty || v[k].y == kEmpty;
-            lane_hits += hit ? 1u : 0u;
-            p[k] = dn ? parked : ((p[k] + 1) & pmask);
-            more |= !dn;
         }
     }
 }
@@ -482,11 +471,11 @@ template <int UNROLL>
 __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const uint32_t *__restrict__ col,
                                                    uint32_t n_rows, const uint32_t *__restrict__ seg, uint32_t nseg_all,
                                                    uint32_t below, uint32_t w0, uint32_t wstep, const uint32_t *tab,
-                                                   uint32_t tlog2, uint32_t parked, uint32_t *s_base, uint32_t *s_end,
+                                                   uint32_t tlog2, uint32_t *s_base, uint32_t *s_end,
                                                    uint8_t *flag, uint32_t lane) {
     constexpr uint32_t RB = 64u * UNROLL;
-    const uint32_t mask = (1u << tlog2) - 1, shift = 32 - tlog2;
-    uint32_t lane_hits = 0;
+    const uint32_t qmask = (1u << (tlog2 - 2)) - 1, qshift = 34 - tlog2;
+    uint32_t hits = 0, lane_hits = 0;
     for (uint32_t j0 = w0; j0 < nseg_all; j0 += wstep) {
         const uint32_t j = j0 + lane;
         uint32_t s = 0, len = 0;
@@ -512,38 +501,64 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
         const uint32_t mybase = s_base[lane];
         const uint32_t myend = lane < nseg ? s_end[lane] : kEmpty;
         uint32_t xa[UNROLL], xb[UNROLL];
-        if (RB <= total)
-            issue_round<UNROLL, true>(col, 0, total, mybase, myend, flag, lane, xa);
-        else
-            issue_round<UNROLL, false>(col, 0, total, mybase, myend, flag, lane, xa);
+        {
+            bool mk;
+            const uint32_t sg = round_first<UNROLL>(0, myend, mk);
+            if (RB > total)
+                issue_round<UNROLL, false>(col, 0, total, mybase, myend, sg, mk, flag, lane, xa);
+            else if (ballot(mk) == 0)
+                issue_single<UNROLL>(col, (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg), lane, xa);
+            else
+                issue_round<UNROLL, true>(col, 0, total, mybase, myend, sg, mk, flag, lane, xa);
+        }
         // The probe is written out in every arm of each branch: a join between
         // an issue and the probe would make the wait for the probed buffer also
         // wait for the loads just issued.  A round followed by another is full.
         for (uint32_t B = 0;; B += 2 * RB) {
             if (B + 2 * RB <= total) {
-                issue_round<UNROLL, true>(col, B + RB, total, mybase, myend, flag, lane, xb);
-                probe_round<UNROLL, true>(xa, B, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
+                bool mk;
+                const uint32_t sg = round_first<UNROLL>(B + RB, myend, mk);
+                if (ballot(mk) == 0) {
+                    const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + RB;
+                    issue_single<UNROLL>(col, c0, lane, xb);
+                    probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, hits, lane_hits);
+                } else {
+                    issue_round<UNROLL, true>(col, B + RB, total, mybase, myend, sg, mk, flag, lane, xb);
+                    probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, hits, lane_hits);
+                }
             } else if (B + RB < total) {
-                issue_round<UNROLL, false>(col, B + RB, total, mybase, myend, flag, lane, xb);
-                probe_round<UNROLL, true>(xa, B, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
+                bool mk;
+                const uint32_t sg = round_first<UNROLL>(B + RB, myend, mk);
+                issue_round<UNROLL, false>(col, B + RB, total, mybase, myend, sg, mk, flag, lane, xb);
+                probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, hits, lane_hits);
             } else {
-                probe_round<UNROLL, false>(xa, B, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
+                probe_round<UNROLL, false>(xa, B, total, tab, qmask, qshift, lane, hits, lane_hits);
                 break;
             }
             if (B + 3 * RB <= total) {
-                issue_round<UNROLL, true>(col, B + 2 * RB, total, mybase, myend, flag, lane, xa);
-                probe_round<UNROLL, true>(xb, B + RB, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
+                bool mk;
+                const uint32_t sg = round_first<UNROLL>(B + 2 * RB, myend, mk);
+                if (ballot(mk) == 0) {
+                    const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + 2 * RB;
+                    issue_single<UNROLL>(col, c0, lane, xa);
+                    probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, hits, lane_hits);
+                } else {
+                    issue_round<UNROLL, true>(col, B + 2 * RB, total, mybase, myend, sg, mk, flag, lane, xa);
+                    probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, hits, lane_hits);
+                }
             } else if (B + 2 * RB < total) {
-                issue_round<UNROLL, false>(col, B + 2 * RB, total, mybase, myend, flag, lane, xa);
-                probe_round<UNROLL, true>(xb, B + RB, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
+                bool mk;
+                const uint32_t sg = round_first<UNROLL>(B + 2 * RB, myend, mk);
+                issue_round<UNROLL, false>(col, B + 2 * RB, total, mybase, myend, sg, mk, flag, lane, xa);
+                probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, hits, lane_hits);
             } else {
-                probe_round<UNROLL, false>(xb, B + RB, total, tab, mask, shift, parked, lane, lane_hits, s_base, s_end);
+                probe_round<UNROLL, false>(xb, B + RB, total, tab, qmask, qshift, lane, hits, lane_hits);
                 break;
             }
         }
     }
     // an item streams at most m < 2^32 entries, so its hits fit 32 bits
-    return wave_sum32(lane_hits);
+    return hits + wave_sum32(lane_hits);
 }
 
 // The segment source of a work item: kind A walks N+(r) with the split filter,
@@ -661,8 +676,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 4) void k_count(Grap
                                                        uint32_t *__restrict__ ctl,
                                                        unsigned long long *__restrict__ counts, uint32_t scale,
                                                        uint32_t small_group) {
-    // s_tab[kBigCap .. kBigCap+1] stay kEmpty: the parking pair of every table
-    __shared__ __align__(16) uint32_t s_tab[kBigCap + 2];
+    __shared__ __align__(16) uint32_t s_tab[kBigCap];
     __shared__ uint32_t s_base[kWaves][64];
     __shared__ uint32_t s_end[kWaves][64];
     __shared__ __align__(16) uint32_t s_flag[kWaves][16 * UNROLL];  // 64*UNROLL bytes per wave
@@ -674,7 +688,6 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 4) void k_count(Grap
     // item and graph fields are wave-uniform: they live in scalar registers
     const uint32_t tid = threadIdx.x, wv = uniform(tid >> 6), lane = tid & 63;
     for (uint32_t i = lane; i < 16 * UNROLL; i += 64) s_flag[wv][i] = 0;
-    if (tid < 2) s_tab[kBigCap + tid] = kEmpty;  // visible to all waves after the first ticket's barrier
     const uint32_t n_big = sload(ctl + C_NBIG), n_small = sload(ctl + C_NSMALL), n_tickets = sload(ctl + C_NTICK);
     const uint32_t small_end = n_big + sload(ctl + C_NSMALLT);  // first tiny ticket
     const uint32_t tiny0 = sload(ctl + C_TINY0), tiny_end = tiny0 + sload(ctl + C_NTINY);
@@ -704,7 +717,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 4) void k_count(Grap
                 table_insert(s_tab, tl, G.col, rb + t0, tk, tid, kWaves * 64);
                 __syncthreads();
                 hits += walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.n, sg.below, wv * 64, kWaves * 64, s_tab, tl,
-                                     kBigCap / 2, s_base[wv], s_end[wv], (uint8_t *)s_flag[wv], lane);
+                                     s_base[wv], s_end[wv], (uint8_t *)s_flag[wv], lane);
                 __syncthreads();
             }
             if (g != cur_g) {
@@ -752,7 +765,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 4) void k_count(Grap
                 for (uint32_t i = lane; i < (1u << tl); i += 64) tab[i] = kEmpty;
                 table_insert(tab, tl, G.col, rb, K, lane, 64);
                 const unsigned long long hits =
-                    walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.n, sg.below, 0, 64, tab, tl, (kBigCap - wv * kSlice) / 2,
+                    walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.n, sg.below, 0, 64, tab, tl,
                                  s_base[wv], s_end[wv], (uint8_t *)s_flag[wv], lane);
                 if (g != cur_g) {
                     if (cur_g != kEmpty && acc != 0 && lane == 0) atomicAdd(counts + cur_g, acc);

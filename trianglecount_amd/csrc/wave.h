@@ -32,11 +32,10 @@ __device__ __forceinline__ uint32_t mask_rank(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
-// base + number of set bits of mask in lanes strictly below this lane.
-__device__ __forceinline__ uint32_t mask_rank_from(uint64_t mask, uint32_t base) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, base));
-}
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// The inverse of ballot: is this lane's bit set in a wave-uniform mask (the
+// mask becomes the execution mask, no vector instruction).
+__device__ __forceinline__ bool lane_in(uint64_t mask) { return __builtin_amdgcn_inverse_ballot_w64(mask); }
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
