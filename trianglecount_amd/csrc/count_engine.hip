@@ -377,6 +377,35 @@ __device__ __forceinline__ void issue_single(const uint32_t *__restrict__ col, u
     for (int k = 0; k < UNROLL; ++k) x[k] = gload(col + (c0 + 64u * k) + lane);
 }
 
+// A full round in which segments end: they are the consecutive segments sg,
+// sg + 1, ... (ends ascend), so the round walks them with wave-uniform reads
+// of their bases and ends.  A chunk inside one segment is addressed from its
+// uniform base; in a chunk that a segment end crosses, each lane picks the
+// base of the last segment that starts at or before its position.
+template <int UNROLL>
+__device__ __forceinline__ void issue_walk(const uint32_t *__restrict__ col, uint32_t B, uint32_t mybase, uint32_t myend,
+                                           uint32_t sg, uint32_t lane, uint32_t (&x)[UNROLL]) {
+    uint32_t j = sg;
+    uint32_t bj = (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)j);
+    uint32_t ej = (uint32_t)__builtin_amdgcn_readlane((int)myend, (int)j);
+#pragma unroll
+    for (int k = 0; k < UNROLL; ++k) {
+        const uint32_t lo = B + 64u * k;
+        const uint32_t t = lo + lane;
+        uint32_t vb = bj;
+        // segments ending inside this chunk (none for most chunks); they end
+        // before B + RB <= total, so j stays below the window's segment count
+        while (ej < lo + 64u) {
+            const uint32_t e = ej;
+            ++j;
+            bj = (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)j);
+            ej = (uint32_t)__builtin_amdgcn_readlane((int)myend, (int)j);
+            vb = t >= e ? bj : vb;
+        }
+        x[k] = gload(col + (uint32_t)(vb + t));
+    }
+}
+
 // FULL: the round lies wholly before `total` (every round of a window but the
 // last), so no position needs the past-the-end handling.  sg and mk come from
 // round_first.
@@ -412,20 +441,19 @@ __device__ __forceinline__ void issue_round(const uint32_t *__restrict__ col, ui
 // empty (no chain continues past such a bucket; tables are at most half full,
 // so every chain has one).  Positions past the window's end carry an id that
 // no table holds (kNoId: ids are below 2^32 - 2).
-// First-bucket hits are counted per wave: each compare is balloted on its own
-// (a ballot of one compare is the compare's lane mask; a ballot of a combined
-// condition would go through a VGPR), the masks are combined in scalar
-// registers and their popcount is added to `hits`.  The few lanes whose chain
-// goes on (about one lane in a hundred at the default table scale) continue
-// alone, chunk by chunk, and count their hits in `lane_hits`.
+// Each compare is balloted on its own (a ballot of one compare is the
+// compare's lane mask; a ballot of a combined condition would go through a
+// VGPR), the masks are combined in scalar registers, and the hit mask is added
+// to the per-lane counter as the carry-in of one vector add.  The few lanes
+// whose chain goes on (about one lane in a hundred at the default table scale)
+// continue alone, chunk by chunk.
 __device__ __forceinline__ uint64_t bucket_hits(const uint4 &v, uint32_t x) {
     return ballot(v.x == x) | ballot(v.y == x) | ballot(v.z == x) | ballot(v.w == x);
 }
 
 template <int UNROLL, bool FULL>
 __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint32_t B, uint32_t total, const uint32_t *tab,
-                                            uint32_t qmask, uint32_t qshift, uint32_t lane, uint32_t &hits,
-                                            uint32_t &lane_hits) {
+                                            uint32_t qmask, uint32_t qshift, uint32_t lane, uint32_t &lane_hits) {
     const uint4 *buckets = (const uint4 *)tab;
     const uint64_t active = ballot(true);
     uint32_t x[UNROLL], p[UNROLL];
@@ -437,26 +465,29 @@ __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint3
     uint4 v[UNROLL];
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) v[k] = buckets[p[k]];
-    uint64_t going[UNROLL], any = 0;  // lanes whose chain goes on
+    // lanes whose chain goes on (bits of inactive lanes may be set here; the
+    // continuation masks them off)
+    uint64_t going[UNROLL], any = 0;
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) {
         const uint64_t H = bucket_hits(v[k], x[k]);
-        hits += (uint32_t)__popcll(H);
-        going[k] = active & ~(H | ballot(v[k].w == kEmpty));
+        add_lane_bit(lane_hits, H);
+        going[k] = ~(H | ballot(v[k].w == kEmpty));
         any |= going[k];
     }
     if (any == 0) return;
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) {
-        if (lane_in(going[k])) {
-            uint32_t q = (p[k] + 1) & qmask;
-            for (;;) {
-                const uint4 w = buckets[q];
-                const bool hit = w.x == x[k] || w.y == x[k] || w.z == x[k] || w.w == x[k];
-                lane_hits += hit ? 1u : 0u;
-                if (hit || w.w == kEmpty) break;
-                q = (q + 1) & qmask;
-            }
+        // a uniform loop over the chunk's remaining lanes: every lane reads
+        // its next bucket (in the table: q stays masked), only live lanes count
+        uint64_t live = going[k] & active;
+        uint32_t q = p[k];
+        while (live != 0) {
+            q = (q + 1) & qmask;
+            const uint4 w = buckets[q];
+            const uint64_t H = bucket_hits(w, x[k]) & live;
+            add_lane_bit(lane_hits, H);
+            live &= ~(H | ballot(w.w == kEmpty));
         }
     }
 }
@@ -475,7 +506,7 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
                                                    uint8_t *flag, uint32_t lane) {
     constexpr uint32_t RB = 64u * UNROLL;
     const uint32_t qmask = (1u << (tlog2 - 2)) - 1, qshift = 34 - tlog2;
-    uint32_t hits = 0, lane_hits = 0;
+    uint32_t lane_hits = 0;
     for (uint32_t j0 = w0; j0 < nseg_all; j0 += wstep) {
         const uint32_t j = j0 + lane;
         uint32_t s = 0, len = 0;
@@ -509,7 +540,7 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
             else if (ballot(mk) == 0)
                 issue_single<UNROLL>(col, (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg), lane, xa);
             else
-                issue_round<UNROLL, true>(col, 0, total, mybase, myend, sg, mk, flag, lane, xa);
+                issue_walk<UNROLL>(col, 0, mybase, myend, sg, lane, xa);
         }
         // The probe is written out in every arm of each branch: a join between
         // an issue and the probe would make the wait for the probed buffer also
@@ -521,18 +552,18 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
                 if (ballot(mk) == 0) {
                     const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + RB;
                     issue_single<UNROLL>(col, c0, lane, xb);
-                    probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, hits, lane_hits);
+                    probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits);
                 } else {
-                    issue_round<UNROLL, true>(col, B + RB, total, mybase, myend, sg, mk, flag, lane, xb);
-                    probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, hits, lane_hits);
+                    issue_walk<UNROLL>(col, B + RB, mybase, myend, sg, lane, xb);
+                    probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits);
                 }
             } else if (B + RB < total) {
                 bool mk;
                 const uint32_t sg = round_first<UNROLL>(B + RB, myend, mk);
                 issue_round<UNROLL, false>(col, B + RB, total, mybase, myend, sg, mk, flag, lane, xb);
-                probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, hits, lane_hits);
+                probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits);
             } else {
-                probe_round<UNROLL, false>(xa, B, total, tab, qmask, qshift, lane, hits, lane_hits);
+                probe_round<UNROLL, false>(xa, B, total, tab, qmask, qshift, lane, lane_hits);
                 break;
             }
             if (B + 3 * RB <= total) {
@@ -541,24 +572,24 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
                 if (ballot(mk) == 0) {
                     const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + 2 * RB;
                     issue_single<UNROLL>(col, c0, lane, xa);
-                    probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, hits, lane_hits);
+                    probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits);
                 } else {
-                    issue_round<UNROLL, true>(col, B + 2 * RB, total, mybase, myend, sg, mk, flag, lane, xa);
-                    probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, hits, lane_hits);
+                    issue_walk<UNROLL>(col, B + 2 * RB, mybase, myend, sg, lane, xa);
+                    probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits);
                 }
             } else if (B + 2 * RB < total) {
                 bool mk;
                 const uint32_t sg = round_first<UNROLL>(B + 2 * RB, myend, mk);
                 issue_round<UNROLL, false>(col, B + 2 * RB, total, mybase, myend, sg, mk, flag, lane, xa);
-                probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, hits, lane_hits);
+                probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits);
             } else {
-                probe_round<UNROLL, false>(xb, B + RB, total, tab, qmask, qshift, lane, hits, lane_hits);
+                probe_round<UNROLL, false>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits);
                 break;
             }
         }
     }
     // an item streams at most m < 2^32 entries, so its hits fit 32 bits
-    return hits + wave_sum32(lane_hits);
+    return wave_sum32(lane_hits);
 }
 
 // The segment source of a work item: kind A walks N+(r) with the split filter,

@@ -33,6 +33,13 @@ __device__ __forceinline__ uint32_t mask_rank(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// cnt += this lane's bit of a wave-uniform mask: one vector add whose carry-in
+// is the mask (no scalar popcount, no per-lane select).
+// The carry-out goes to VCC (clobbered), never to an SGPR the compiler might
+// hand to a memory instruction next.
+__device__ __forceinline__ void add_lane_bit(uint32_t &cnt, uint64_t mask) {
+    asm("v_addc_co_u32_e64 %0, vcc, %1, 0, %2" : "=v"(cnt) : "v"(cnt), "s"(mask) : "vcc");
+}
 // The inverse of ballot: is this lane's bit set in a wave-uniform mask (the
 // mask becomes the execution mask, no vector instruction).
 __device__ __forceinline__ bool lane_in(uint64_t mask) { return __builtin_amdgcn_inverse_ballot_w64(mask); }
