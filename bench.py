@@ -140,7 +140,7 @@ def partitioned(T, g, k, seed, steps, ws):
     t_inl = time.perf_counter() - t0
     ms, per = _replay_ms(T, parts, steps)
     for h in parts:
-        h.in_off = h.in_col = h.in_owned = None
+        h.in_off = h.in_col = h.in_owned = h.col_seg = h.in_seg = None
     ms_plain, per_plain = _replay_ms(T, parts, max(1, steps // 2))
     if per_plain != per:
         raise SystemExit(f"split part counts {per} differ from plain {per_plain}")
@@ -189,9 +189,10 @@ def _per_graph_calls(T, graphs, ws):
     for i, g in enumerate(graphs):
         out = C.c_void_p(res.data_ptr() + 8 * i)
         if g.split_ready():
+            seg = [C.c_void_p(t.data_ptr()) if t is not None else None for t in (g.col_seg, g.in_seg)]
             calls.append((L.lstc_count_split, (C.c_void_p(g.off.data_ptr()), C.c_void_p(g.col.data_ptr()),
                                                C.c_void_p(g.in_off.data_ptr()), C.c_void_p(g.in_col.data_ptr()),
-                                               g.n, g.owned, g.m, out, None, wp, wb, sh)))
+                                               seg[0], seg[1], g.n, g.owned, g.m, out, None, wp, wb, sh)))
         else:
             calls.append((L.lstc_count, (C.c_void_p(g.off.data_ptr()), C.c_void_p(g.col.data_ptr()), g.n, g.owned,
                                          g.m, out, None, None, wp, wb, sh)))

@@ -148,11 +148,13 @@ lstc_status lstc_count(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_
  * (any CSR, owned sources only): every arc (u, w) streams the shorter of N+(u)
  * and N+(w) through an LDS table of the other, and rows with d+ <= tiny_max
  * compare both lists lane-per-arc, which on power-law graphs streams well under
- * half the entries of lstc_count.  Workspace: lstc_count_workspace(n_rows, ., m). */
+ * half the entries of lstc_count.  d_col_seg / d_in_seg: the graph's segment
+ * words from lstc_segments, or both NULL (then each segment's list is located
+ * through d_off).  Workspace: lstc_count_workspace(n_rows, ., m). */
 lstc_status lstc_count_split(const uint64_t *d_off, const uint32_t *d_col, const uint64_t *d_in_off,
-                             const uint32_t *d_in_col, uint32_t n_rows, uint32_t n_owned, uint64_t m,
-                             uint64_t *d_count, const lstc_tune *tune, void *d_ws, size_t ws_bytes,
-                             lstc_stream s);
+                             const uint32_t *d_in_col, const uint64_t *d_col_seg, const uint64_t *d_in_seg,
+                             uint32_t n_rows, uint32_t n_owned, uint64_t m, uint64_t *d_count,
+                             const lstc_tune *tune, void *d_ws, size_t ws_bytes, lstc_stream s);
 
 /* Target-side in-lists of an oriented CSR for split counting: for every row w,
  * the owned sources u < n_owned of arcs u -> w with d+(u) <= d+(w) (row lengths
@@ -165,6 +167,19 @@ lstc_status lstc_in_lists_workspace(uint32_t n_rows, uint32_t n_owned, uint64_t 
 lstc_status lstc_in_lists(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows, uint32_t n_owned,
                           uint64_t m, uint64_t *d_in_off, uint32_t *d_in_col, uint64_t *n_entries, void *d_ws,
                           size_t ws_bytes, lstc_stream s);
+
+/* Segment words of a graph prepared for split counting: for every entry v of
+ * d_col[m] and of d_in_col[n_in] (n_in = d_in_off[n_rows]), the word
+ * start | length << 32 of v's row (start = d_off[v], a column index below
+ * 2^32; length 0 for ids v >= n_rows).  Built once per graph (one pass over
+ * the arcs, stream-ordered, no workspace); the count then reads one coalesced
+ * word per segment instead of two row offsets at a random place (the
+ * reference's kernel reads row_ptr[v], row_ptr[v+1] per neighbour,
+ * src/WithoutPartition/ShareKernelTCV62_update_v1.cu:80-81).  8 bytes per
+ * entry: d_col_seg[m], d_in_seg[n_in].                                        */
+lstc_status lstc_segments(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows, uint64_t m,
+                          const uint32_t *d_in_col, uint64_t n_in, uint64_t *d_col_seg, uint64_t *d_in_seg,
+                          lstc_stream s);
 
 /* ------------------------------------------------------------------ batching
  * Many graphs (e.g. the k halo parts of a partition, or unrelated graphs of
@@ -181,6 +196,8 @@ typedef struct lstc_graph_desc {
     uint64_t m;
     uint32_t n_rows;
     uint32_t n_owned;
+    const uint64_t *col_seg; /* segment words (lstc_segments) or NULL; used with   */
+    const uint64_t *in_seg;  /* the in-lists they describe                        */
 } lstc_graph_desc;          /* at most 2047 graphs per launch */
 
 lstc_status lstc_count_batch_workspace(const lstc_graph_desc *graphs, uint32_t G, const lstc_tune *tune,

@@ -79,9 +79,17 @@ def test_count_rejects_bad_shapes_on_host():
     # no output
     assert lib.lstc_count(_FAKE, _FAKE, 10, 10, 20, None, None, None, _FAKE, ws, None) == _lib.LSTC_ERR_INVALID
     # split count without in-lists
-    assert lib.lstc_count_split(_FAKE, _FAKE, None, None, 10, 10, 20, _FAKE, None, _FAKE, ws,
+    assert lib.lstc_count_split(_FAKE, _FAKE, None, None, None, None, 10, 10, 20, _FAKE, None, _FAKE, ws,
                                 None) == _lib.LSTC_ERR_INVALID
     assert "lstc_count" in lib.lstc_last_error().decode()
+    # segment words: both or neither, and only with in-lists
+    assert lib.lstc_count_split(_FAKE, _FAKE, _FAKE, _FAKE, _FAKE, None, 10, 10, 20, _FAKE, None, _FAKE, ws,
+                                None) == _lib.LSTC_ERR_INVALID
+    assert "segment" in lib.lstc_last_error().decode()
+    # segments: more in-list entries than arcs, NULL arrays
+    assert lib.lstc_segments(_FAKE, _FAKE, 10, 20, _FAKE, 21, _FAKE, _FAKE, None) == _lib.LSTC_ERR_INVALID
+    assert lib.lstc_segments(_FAKE, _FAKE, 10, 20, None, 5, _FAKE, _FAKE, None) == _lib.LSTC_ERR_INVALID
+    assert lib.lstc_segments(_FAKE, _FAKE, 10, 0xFFFFF000, None, 0, _FAKE, None, None) == _lib.LSTC_ERR_INVALID
 
 
 def test_batch_and_in_lists_reject_bad_shapes_on_host():
@@ -92,6 +100,9 @@ def test_batch_and_in_lists_reject_bad_shapes_on_host():
     descs[1].in_off = 0x10000  # in_off without in_col
     nb = ctypes.c_size_t()
     assert lib.lstc_count_batch(descs, 2, _FAKE, None, _FAKE, ctypes.c_size_t(1 << 20), None) == _lib.LSTC_ERR_INVALID
+    descs[1].in_col, descs[1].col_seg = 0x10000, 0x10000  # one segment-word array without the other
+    assert lib.lstc_count_batch(descs, 2, _FAKE, None, _FAKE, ctypes.c_size_t(1 << 20), None) == _lib.LSTC_ERR_INVALID
+    assert "segment" in lib.lstc_last_error().decode()
     big = (_lib.GraphDesc * 2048)()
     assert lib.lstc_count_batch(big, 2048, _FAKE, None, _FAKE, ctypes.c_size_t(1 << 20), None) == _lib.LSTC_ERR_INVALID
     assert lib.lstc_in_lists_workspace(10, 11, 20, ctypes.byref(nb)) == _lib.LSTC_ERR_INVALID

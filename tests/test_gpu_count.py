@@ -2,6 +2,8 @@
 
 Every GPU result must equal the oracle's uint64 exactly.
 """
+import copy
+
 import numpy as np
 import pytest
 import torch
@@ -116,6 +118,12 @@ def test_split_count_matches_oracle(lstc, oracle, name, builder):
     # a batch mixing split and plain graphs
     plain = lstc.Graph.from_numpy(og.off, og.col)
     assert lstc.count_batch([g, plain, g]) == [want, want, want]
+    # without segment words each segment's list is located through the row offsets
+    g2 = copy.copy(g)
+    g2.col_seg = g2.in_seg = None
+    assert lstc.count(g2) == want, name
+    assert lstc.count(g2, tune=lstc.Tune(table_scale=2, unroll=8)) == want, name
+    assert lstc.count_batch([g2, g, plain]) == [want, want, want]
 
 
 @pytest.mark.parametrize("cut_frac", [0.0, 0.3, 1.0])
@@ -132,3 +140,10 @@ def test_in_lists_of_any_csr(lstc, oracle, cut_frac):
     assert lstc.count(g) == want
     # in-lists built for another n_owned are not used
     assert lstc.count(g, n_owned=og.n) == oracle.count(og)
+    # segment words: start | length << 32 of every col entry's and in-list entry's row
+    d = np.diff(og.off.astype(np.int64))
+    def words(ids):
+        ids = ids.astype(np.int64)
+        return (og.off[ids].astype(np.uint64) & np.uint64(0xFFFFFFFF)) | (d[ids].astype(np.uint64) << np.uint64(32))
+    assert np.array_equal(g.col_seg[: og.m].cpu().numpy().view(np.uint64), words(og.col))
+    assert np.array_equal(g.in_seg[: len(want_col)].cpu().numpy().view(np.uint64), words(want_col))

@@ -43,7 +43,8 @@ class GraphDesc(C.Structure):
     """lstc_graph_desc (include/lstc.h)."""
     _fields_ = [("off", C.c_void_p), ("col", C.c_void_p), ("per_row", C.c_void_p),
                 ("in_off", C.c_void_p), ("in_col", C.c_void_p),
-                ("m", C.c_uint64), ("n_rows", C.c_uint32), ("n_owned", C.c_uint32)]
+                ("m", C.c_uint64), ("n_rows", C.c_uint32), ("n_owned", C.c_uint32),
+                ("col_seg", C.c_void_p), ("in_seg", C.c_void_p)]
 
 
 class Request(C.Structure):
@@ -94,9 +95,10 @@ SIGNATURES = {
     "lstc_halo_extract": (ST, [VP, VP, U32, VP, U32, I32, VP, VP, VP, U32, U64, VP, SZ, VP]),
     "lstc_count_workspace": (ST, [U32, U32, U64, C.POINTER(Tune), PSZ]),
     "lstc_count": (ST, [VP, VP, U32, U32, U64, VP, VP, C.POINTER(Tune), VP, SZ, VP]),
-    "lstc_count_split": (ST, [VP, VP, VP, VP, U32, U32, U64, VP, C.POINTER(Tune), VP, SZ, VP]),
+    "lstc_count_split": (ST, [VP, VP, VP, VP, VP, VP, U32, U32, U64, VP, C.POINTER(Tune), VP, SZ, VP]),
     "lstc_in_lists_workspace": (ST, [U32, U32, U64, PSZ]),
     "lstc_in_lists": (ST, [VP, VP, U32, U32, U64, VP, VP, PU64, VP, SZ, VP]),
+    "lstc_segments": (ST, [VP, VP, U32, U64, VP, U64, VP, VP, VP]),
     "lstc_count_batch_workspace": (ST, [C.POINTER(GraphDesc), U32, C.POINTER(Tune), PSZ]),
     "lstc_count_batch": (ST, [C.POINTER(GraphDesc), U32, VP, C.POINTER(Tune), VP, SZ, VP]),
     "lstc_batch_plan_create": (ST, [C.POINTER(GraphDesc), U32, C.POINTER(Tune), C.POINTER(VP)]),

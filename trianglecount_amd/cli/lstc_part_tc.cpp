@@ -31,17 +31,19 @@ int main(int argc, char **argv) {
     }
     const double t0 = now_s();
     std::vector<lstc_graph_desc> descs(k);
-    std::vector<uint64_t *> offs(k), in_offs(k, nullptr);
-    std::vector<uint32_t *> cols(k), in_cols(k, nullptr);
+    std::vector<uint64_t *> offs(k);
+    std::vector<uint32_t *> cols(k);
+    std::vector<SplitLists> split(k);
     Workspace ws_in;
     for (uint64_t p = 0; p < k; ++p) {
         offs[p] = upload(parts[p].off);
         cols[p] = upload(parts[p].col);
         if (!plain)
-            build_in_lists(offs[p], cols[p], (uint32_t)parts[p].n_local, (uint32_t)parts[p].n_owned,
-                           parts[p].col.size(), &in_offs[p], &in_cols[p], ws_in);
-        descs[p] = lstc_graph_desc{offs[p], cols[p], nullptr, in_offs[p], in_cols[p], parts[p].col.size(),
-                                   (uint32_t)parts[p].n_local, (uint32_t)parts[p].n_owned};
+            split[p] = build_in_lists(offs[p], cols[p], (uint32_t)parts[p].n_local, (uint32_t)parts[p].n_owned,
+                                      parts[p].col.size(), ws_in);
+        descs[p] = lstc_graph_desc{offs[p], cols[p], nullptr, split[p].in_off, split[p].in_col, parts[p].col.size(),
+                                   (uint32_t)parts[p].n_local, (uint32_t)parts[p].n_owned, split[p].col_seg,
+                                   split[p].in_seg};
     }
     uint64_t *d_counts = nullptr;
     CLI_HIP(hipMalloc((void **)&d_counts, sizeof(uint64_t) * (k ? k : 1)));
@@ -62,8 +64,8 @@ int main(int argc, char **argv) {
                                      d_counts + p, nullptr, nullptr, w, ws.bytes, nullptr));
             else
                 CLI_CHECK(lstc_count_split(descs[p].off, descs[p].col, descs[p].in_off, descs[p].in_col,
-                                           descs[p].n_rows, descs[p].n_owned, descs[p].m, d_counts + p, nullptr, w,
-                                           ws.bytes, nullptr));
+                                           descs[p].col_seg, descs[p].in_seg, descs[p].n_rows, descs[p].n_owned,
+                                           descs[p].m, d_counts + p, nullptr, w, ws.bytes, nullptr));
             CLI_HIP(hipEventRecord(e1, nullptr));
             CLI_HIP(hipEventSynchronize(e1));
             float ms = 0;
@@ -99,8 +101,7 @@ int main(int argc, char **argv) {
     for (uint64_t p = 0; p < k; ++p) {
         (void)hipFree(offs[p]);
         (void)hipFree(cols[p]);
-        (void)hipFree(in_offs[p]);
-        (void)hipFree(in_cols[p]);
+        split[p].release();
     }
     (void)hipFree(d_counts);
     return 0;

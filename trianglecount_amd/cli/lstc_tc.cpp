@@ -34,9 +34,8 @@ int main(int argc, char **argv) {
     size_t need = 0;
     CLI_CHECK(lstc_count_workspace((uint32_t)g.n, (uint32_t)g.n, g.col.size(), nullptr, &need));
     Workspace ws_in;
-    uint64_t *d_in_off = nullptr;
-    uint32_t *d_in_col = nullptr;
-    if (!plain) build_in_lists(d_off, d_col, (uint32_t)g.n, (uint32_t)g.n, g.col.size(), &d_in_off, &d_in_col, ws_in);
+    SplitLists sl;
+    if (!plain) sl = build_in_lists(d_off, d_col, (uint32_t)g.n, (uint32_t)g.n, g.col.size(), ws_in);
     void *w = ws.get(need);
     hipEvent_t e0, e1;
     CLI_HIP(hipEventCreate(&e0));
@@ -48,8 +47,8 @@ int main(int argc, char **argv) {
             CLI_CHECK(lstc_count(d_off, d_col, (uint32_t)g.n, (uint32_t)g.n, g.col.size(), d_count, nullptr, nullptr,
                                  w, ws.bytes, nullptr));
         else
-            CLI_CHECK(lstc_count_split(d_off, d_col, d_in_off, d_in_col, (uint32_t)g.n, (uint32_t)g.n, g.col.size(),
-                                       d_count, nullptr, w, ws.bytes, nullptr));
+            CLI_CHECK(lstc_count_split(d_off, d_col, sl.in_off, sl.in_col, sl.col_seg, sl.in_seg, (uint32_t)g.n,
+                                       (uint32_t)g.n, g.col.size(), d_count, nullptr, w, ws.bytes, nullptr));
         CLI_HIP(hipEventRecord(e1, nullptr));
         CLI_HIP(hipEventSynchronize(e1));
         float ms = 0;
@@ -64,8 +63,7 @@ int main(int argc, char **argv) {
     printf("\nTotal avg of %d RUNS : %.6f Sec\n", runs, runs ? total / runs : 0.0);
     (void)hipFree(d_off);
     (void)hipFree(d_col);
-    (void)hipFree(d_in_off);
-    (void)hipFree(d_in_col);
+    sl.release();
     (void)hipFree(d_count);
     return 0;
 }

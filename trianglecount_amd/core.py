@@ -84,6 +84,8 @@ class Graph:
     in_off: torch.Tensor | None = None  # target-side in-lists (split count), optional
     in_col: torch.Tensor | None = None
     in_owned: int | None = None  # the n_owned the in-lists were built for (owned sources only)
+    col_seg: torch.Tensor | None = None  # segment words of col / in_col (lstc_segments), optional
+    in_seg: torch.Tensor | None = None
 
     def split_ready(self, n_owned: int | None = None) -> bool:
         """True when the in-lists fit a count over the first ``n_owned`` rows."""
@@ -203,6 +205,21 @@ def clean_orient(n: int, src: torch.Tensor, dst: torch.Tensor, compact: bool = T
     g = Graph(n_out.value, m_out.value, off, col, sym_off, sym_col)
     if in_lists:
         g.in_off, g.in_col, g.in_owned = in_off, in_col, g.n
+        segments(g, stream=stream)
+    return g
+
+
+def segments(g: Graph, stream=None) -> Graph:
+    """Attach the segment words of ``g``'s col and in-lists (lstc_segments); needs the in-lists."""
+    if g.in_off is None:
+        raise ValueError("segments: build the in-lists first (clean_orient(in_lists=True) or in_lists(g))")
+    L = load()
+    n_in = int(g.in_off[g.n].item())
+    col_seg = empty_u64(max(g.m, 1))
+    in_seg = empty_u64(max(n_in, 1))
+    check(L.lstc_segments(_ptr(g.off), _ptr(g.col), g.n, g.m, _ptr(g.in_col), n_in, _ptr(col_seg), _ptr(in_seg),
+                          C.c_void_p(stream_handle(stream))), "lstc_segments")
+    g.col_seg, g.in_seg = col_seg, in_seg
     return g
 
 
@@ -220,7 +237,7 @@ def in_lists(g: Graph, ws: Workspace | None = None, stream=None) -> Graph:
                           C.c_void_p(stream_handle(stream))), "lstc_in_lists")
     g.in_off, g.in_col, g.in_owned = in_off, in_col, no
     g.meta["in_entries"] = k.value
-    return g
+    return segments(g, stream=stream)
 
 
 # ------------------------------------------------------------------ (c) count
@@ -242,8 +259,9 @@ def count(g: Graph, n_owned: int | None = None, per_row: bool = False, tune: Tun
     res = out if out is not None else empty_u64(1)
     pr = empty_u64(no) if per_row else None
     if g.split_ready(no) and not per_row and split:
-        check(L.lstc_count_split(_ptr(g.off), _ptr(g.col), _ptr(g.in_off), _ptr(g.in_col), g.n, no, g.m,
-                                 _ptr(res), tp, wp, wb, C.c_void_p(stream_handle(stream))), "lstc_count_split")
+        check(L.lstc_count_split(_ptr(g.off), _ptr(g.col), _ptr(g.in_off), _ptr(g.in_col), _ptr(g.col_seg),
+                                 _ptr(g.in_seg), g.n, no, g.m, _ptr(res), tp, wp, wb,
+                                 C.c_void_p(stream_handle(stream))), "lstc_count_split")
     else:
         check(L.lstc_count(_ptr(g.off), _ptr(g.col), g.n, no, g.m, _ptr(res), _ptr(pr), tp, wp, wb,
                            C.c_void_p(stream_handle(stream))), "lstc_count")
@@ -299,6 +317,9 @@ def _descs(graphs, per_row=None):
         if g.split_ready():
             arr[i].in_off = g.in_off.data_ptr()
             arr[i].in_col = g.in_col.data_ptr()
+            if g.col_seg is not None:
+                arr[i].col_seg = g.col_seg.data_ptr()
+                arr[i].in_seg = g.in_seg.data_ptr()
         arr[i].m = g.m
         arr[i].n_rows = g.n
         arr[i].n_owned = g.owned

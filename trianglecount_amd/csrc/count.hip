@@ -28,12 +28,13 @@ extern "C" lstc_status lstc_count(const uint64_t *d_off, const uint32_t *d_col, 
 }
 
 extern "C" lstc_status lstc_count_split(const uint64_t *d_off, const uint32_t *d_col, const uint64_t *d_in_off,
-                                        const uint32_t *d_in_col, uint32_t n_rows, uint32_t n_owned, uint64_t m,
+                                        const uint32_t *d_in_col, const uint64_t *d_col_seg,
+                                        const uint64_t *d_in_seg, uint32_t n_rows, uint32_t n_owned, uint64_t m,
                                         uint64_t *d_count, const lstc_tune *tune, void *d_ws, size_t ws_bytes,
                                         lstc_stream s) {
     if (!d_count) return fail(LSTC_ERR_INVALID, "lstc_count_split: NULL output");
     if (!d_in_off || !d_in_col) return fail(LSTC_ERR_INVALID, "lstc_count_split: NULL in-lists");
-    HostGraph h{d_off, d_col, nullptr, n_rows, n_owned, m, d_in_off, d_in_col};
+    HostGraph h{d_off, d_col, nullptr, n_rows, n_owned, m, d_in_off, d_in_col, d_col_seg, d_in_seg};
     LSTC_TRY(validate_graph(h, "lstc_count_split"));
     return count_launch(&h, 1, nullptr, nullptr, (unsigned long long *)d_count, tune, d_ws, ws_bytes,
                         (hipStream_t)s);

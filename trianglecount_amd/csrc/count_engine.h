@@ -14,6 +14,8 @@ struct GraphDev {
     const uint32_t *in_col;
     uint32_t n_rows;
     uint32_t n_owned;
+    const uint64_t *col_seg;      // segment words of col[] / in_col[] (split count), may be null
+    const uint64_t *in_seg;
 };
 
 // The graphs of one launch, passed by value.  A single graph travels inline
@@ -35,6 +37,8 @@ struct HostGraph {
     uint64_t m;
     const uint64_t *in_off = nullptr;  // optional target-side in-lists
     const uint32_t *in_col = nullptr;
+    const uint64_t *col_seg = nullptr;  // optional segment words (lstc_segments)
+    const uint64_t *in_seg = nullptr;
 };
 
 // Split counting (in-lists present) applies only to whole graphs without per-row output.
@@ -45,10 +49,12 @@ inline bool split_ok(const HostGraph &h) { return h.in_off && h.in_col && !h.per
 // Rows the planner visits: every row for split counting (in-list items of halo
 // rows), else the owned rows.
 inline uint64_t plan_rows(const HostGraph &h) { return split_ok(h) ? h.n_rows : h.n_owned; }
+// Segment words are used only with the in-lists they describe.
+inline bool seg_ok(const HostGraph &h) { return split_ok(h) && h.col_seg && h.in_seg; }
 inline GraphDev to_dev(const HostGraph &h) {
-    const bool sp = split_ok(h);
+    const bool sp = split_ok(h), sg = seg_ok(h);
     return GraphDev{h.off, h.col, (unsigned long long *)h.per_row, sp ? h.in_off : nullptr, sp ? h.in_col : nullptr,
-                    h.n_rows, h.n_owned};
+                    h.n_rows, h.n_owned, sg ? h.col_seg : nullptr, sg ? h.in_seg : nullptr};
 }
 
 // Resolved tuning (defaults filled in, ranges clamped).

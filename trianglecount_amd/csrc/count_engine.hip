@@ -113,6 +113,9 @@ lstc_status validate_graph(const HostGraph &h, const char *who) {
     if (h.n_owned && (!h.off || (h.m && !h.col))) return fail(LSTC_ERR_INVALID, "%s: NULL CSR arrays", who);
     if ((h.in_off == nullptr) != (h.in_col == nullptr))
         return fail(LSTC_ERR_INVALID, "%s: give both in-list arrays or neither", who);
+    if ((h.col_seg == nullptr) != (h.in_seg == nullptr))
+        return fail(LSTC_ERR_INVALID, "%s: give both segment-word arrays or neither", who);
+    if (h.col_seg && !h.in_off) return fail(LSTC_ERR_INVALID, "%s: segment words without in-lists", who);
     return LSTC_OK;
 }
 
@@ -136,6 +139,8 @@ __device__ __forceinline__ GraphDev graph_at(const GraphSet &gs, uint32_t g) {
         d.in_col = q->in_col;
         d.n_rows = q->n_rows;
         d.n_owned = q->n_owned;
+        d.col_seg = q->col_seg;
+        d.in_seg = q->in_seg;
         return d;
     }
     const LSTC_GLOBAL_AS GraphDev *q = (const LSTC_GLOBAL_AS GraphDev *)(gs.table + g);
@@ -495,12 +500,16 @@ __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint3
 // Streams the lists N+(s) of the segment ids s = seg[j], j in [w0, nseg) step
 // wstep (windows of 64), through tab.  A segment whose list is not shorter than
 // `below` is skipped (below = 0: none skipped); ids >= n_rows have no list.
+// With segment words (segw[j] = start | length << 32 of N+(seg[j])) a window
+// reads one coalesced word per segment instead of two row offsets at a random
+// place.
 // Rounds are software-pipelined over two buffers: the loads of round r+1 are
 // issued before round r is probed, so a wave keeps up to 2*UNROLL chunk loads
 // in flight.
 template <int UNROLL>
 __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const uint32_t *__restrict__ col,
-                                                   uint32_t n_rows, const uint32_t *__restrict__ seg, uint32_t nseg_all,
+                                                   uint32_t n_rows, const uint32_t *__restrict__ seg,
+                                                   const uint64_t *__restrict__ segw, uint32_t nseg_all,
                                                    uint32_t below, uint32_t w0, uint32_t wstep, const uint32_t *tab,
                                                    uint32_t tlog2, uint32_t *s_base, uint32_t *s_end,
                                                    uint8_t *flag, uint32_t lane) {
@@ -511,13 +520,19 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
         const uint32_t j = j0 + lane;
         uint32_t s = 0, len = 0;
         if (j < nseg_all) {
-            const uint32_t w = gload(seg + j);
-            if (w < n_rows) {
-                const uint64_t s0 = gload(off + w);
-                s = (uint32_t)s0;
-                len = (uint32_t)(gload(off + w + 1) - s0);
-                if (below && len >= below) len = 0;  // that arc is counted from the other side
+            if (segw) {  // one coalesced word: start | length << 32
+                const uint64_t sw = gload(segw + j);
+                s = (uint32_t)sw;
+                len = (uint32_t)(sw >> 32);
+            } else {
+                const uint32_t w = gload(seg + j);
+                if (w < n_rows) {
+                    const uint64_t s0 = gload(off + w);
+                    s = (uint32_t)s0;
+                    len = (uint32_t)(gload(off + w + 1) - s0);
+                }
             }
+            if (below && len >= below) len = 0;  // that arc is counted from the other side
         }
         const uint32_t incl = wave_incl_scan(len, lane);
         const uint32_t total = uniform(gather(incl, 63));
@@ -596,18 +611,19 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
 // kind B walks one chunk of r's in-list unfiltered.
 struct Segs {
     const uint32_t *ids;
+    const uint64_t *seg;  // the ids' segment words, or null
     uint32_t n;
     uint32_t below;
 };
 
 __device__ __forceinline__ Segs item_segs(const GraphDev &G, uint32_t kind, uint32_t chunk, uint32_t r, uint64_t rb,
                                           uint32_t K) {
-    if (kind == 0) return Segs{G.col + rb, K, G.in_off ? K : 0u};
+    if (kind == 0) return Segs{G.col + rb, G.col_seg ? G.col_seg + rb : nullptr, K, G.in_off ? K : 0u};
     const uint64_t ib = sload(G.in_off + r), ie = sload(G.in_off + r + 1);
     const uint32_t sb = in_chunk(ie - ib);
     const uint64_t c0 = ib + (uint64_t)chunk * sb;
     const uint64_t c1 = (c0 + sb < ie) ? c0 + sb : ie;
-    return Segs{G.in_col + c0, (uint32_t)(c1 - c0), 0u};
+    return Segs{G.in_col + c0, G.in_seg ? G.in_seg + c0 : nullptr, (uint32_t)(c1 - c0), 0u};
 }
 
 // ------------------------------------------------------------------ tiny rows
@@ -747,7 +763,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 4) void k_count(Grap
                 __syncthreads();
                 table_insert(s_tab, tl, G.col, rb + t0, tk, tid, kWaves * 64);
                 __syncthreads();
-                hits += walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.n, sg.below, wv * 64, kWaves * 64, s_tab, tl,
+                hits += walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.seg, sg.n, sg.below, wv * 64, kWaves * 64, s_tab, tl,
                                      s_base[wv], s_end[wv], (uint8_t *)s_flag[wv], lane);
                 __syncthreads();
             }
@@ -796,7 +812,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 4) void k_count(Grap
                 for (uint32_t i = lane; i < (1u << tl); i += 64) tab[i] = kEmpty;
                 table_insert(tab, tl, G.col, rb, K, lane, 64);
                 const unsigned long long hits =
-                    walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.n, sg.below, 0, 64, tab, tl,
+                    walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.seg, sg.n, sg.below, 0, 64, tab, tl,
                                  s_base[wv], s_end[wv], (uint8_t *)s_flag[wv], lane);
                 if (g != cur_g) {
                     if (cur_g != kEmpty && acc != 0 && lane == 0) atomicAdd(counts + cur_g, acc);

@@ -10,6 +10,10 @@
 // row order; a stable LSD radix sort on the low (target) bits groups them by
 // target with sources still ascending; in_col takes the sources and in_off[w]
 // is found by binary search over the sorted targets.
+//
+// Segment words (lstc_segments): for every entry v of col[] and in_col[], the
+// word start(N+(v)) | d+(v) << 32, so the counting walk reads one coalesced
+// word per segment instead of off[v] and off[v+1] at a random place.
 #include "lstc_internal.h"
 
 namespace lstc {
@@ -61,6 +65,19 @@ __global__ void k_inl_offsets(const uint64_t *__restrict__ keys, uint64_t k, uin
             if ((uint32_t)keys[mid] < w) lo = mid + 1; else hi = mid;
         }
         in_off[w] = lo;
+    }
+}
+
+__global__ void k_seg_words(const uint32_t *__restrict__ ids, uint64_t n, const uint64_t *__restrict__ off,
+                            uint32_t n_rows, uint64_t *__restrict__ words) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t v = ids[i];
+        uint64_t w = 0;  // ids >= n_rows (no list): length 0
+        if (v < n_rows) {
+            const uint64_t b = off[v];
+            w = (uint64_t)(uint32_t)b | ((off[v + 1] - b) << 32);
+        }
+        words[i] = w;
     }
 }
 
@@ -132,5 +149,29 @@ extern "C" lstc_status lstc_in_lists(const uint64_t *d_off, const uint32_t *d_co
     LSTC_LAUNCHED("k_inl_offsets");
     LSTC_HIP(hipStreamSynchronize(st));
     if (n_entries) *n_entries = k;
+    return LSTC_OK;
+}
+
+extern "C" lstc_status lstc_segments(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows, uint64_t m,
+                                     const uint32_t *d_in_col, uint64_t n_in, uint64_t *d_col_seg, uint64_t *d_in_seg,
+                                     lstc_stream s) {
+    hipStream_t st = (hipStream_t)s;
+    if (n_rows == 0xFFFFFFFFu) return fail(LSTC_ERR_INVALID, "lstc_segments: n_rows must be < 2^32-1");
+    if (m >= 0xFFFFF000ull)
+        return fail(LSTC_ERR_INVALID, "lstc_segments: m = %llu exceeds 2^32-4096", (unsigned long long)m);
+    if (n_in > m)
+        return fail(LSTC_ERR_INVALID, "lstc_segments: %llu in-list entries exceed m", (unsigned long long)n_in);
+    if (!d_off || (m && (!d_col || !d_col_seg)) || (n_in && (!d_in_col || !d_in_seg)))
+        return fail(LSTC_ERR_INVALID, "lstc_segments: NULL array");
+    if (m) {
+        const uint32_t grid = (uint32_t)hmin<uint64_t>((m + 255) / 256, 65535);
+        hipLaunchKernelGGL(k_seg_words, dim3(grid), dim3(256), 0, st, d_col, m, d_off, n_rows, d_col_seg);
+        LSTC_LAUNCHED("k_seg_words");
+    }
+    if (n_in) {
+        const uint32_t grid = (uint32_t)hmin<uint64_t>((n_in + 255) / 256, 65535);
+        hipLaunchKernelGGL(k_seg_words, dim3(grid), dim3(256), 0, st, d_in_col, n_in, d_off, n_rows, d_in_seg);
+        LSTC_LAUNCHED("k_seg_words");
+    }
     return LSTC_OK;
 }
