@@ -81,6 +81,36 @@ __global__ void k_seg_words(const uint32_t *__restrict__ ids, uint64_t n, const 
     }
 }
 
+// The same words for entry counts known only on the device (*m_at arcs,
+// *n_in_at in-list entries; the server's stream-ordered preparation).
+__global__ void k_seg_words_dev(const uint32_t *__restrict__ col, const uint32_t *__restrict__ in_col,
+                                const uint64_t *__restrict__ m_at, const uint64_t *__restrict__ n_in_at, uint64_t cap,
+                                const uint64_t *__restrict__ off, uint32_t n_rows, uint64_t *__restrict__ col_seg,
+                                uint64_t *__restrict__ in_seg) {
+    const uint64_t m = *m_at < cap ? *m_at : cap, n_in = *n_in_at < cap ? *n_in_at : cap;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m + n_in;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const bool in = i >= m;
+        const uint32_t v = in ? in_col[i - m] : col[i];
+        uint64_t w = 0;
+        if (v < n_rows) {
+            const uint64_t b = off[v];
+            w = (uint64_t)(uint32_t)b | ((off[v + 1] - b) << 32);
+        }
+        if (in) in_seg[i - m] = w; else col_seg[i] = w;
+    }
+}
+
+lstc_status segments_launch_dev(const uint64_t *d_off, const uint32_t *d_col, const uint64_t *d_in_off,
+                                const uint32_t *d_in_col, uint32_t n_rows, uint64_t cap, uint64_t *d_col_seg,
+                                uint64_t *d_in_seg, hipStream_t st) {
+    const uint32_t grid = (uint32_t)hmin<uint64_t>((2 * cap + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_seg_words_dev, dim3(grid), dim3(256), 0, st, d_col, d_in_col, d_off + n_rows,
+                       d_in_off + n_rows, cap, d_off, n_rows, d_col_seg, d_in_seg);
+    LSTC_LAUNCHED("k_seg_words_dev");
+    return LSTC_OK;
+}
+
 static int bits_for(uint32_t n) {
     int b = 8;
     while (b < 32 && ((uint64_t)n >> b) != 0) b += 8;

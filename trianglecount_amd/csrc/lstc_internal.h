@@ -96,6 +96,11 @@ lstc_status clean_orient_launch(const uint32_t *d_src, const uint32_t *d_dst, ui
                                 uint32_t *d_col, uint64_t *d_in_off, uint32_t *d_in_col, void *d_ws,
                                 size_t ws_bytes, hipStream_t st,
                                 const uint64_t **d_sizes, const uint32_t **d_bad);
+// Segment words (lstc_segments) of a graph whose arc and in-list entry counts
+// are only on the device (d_off[n_rows], d_in_off[n_rows], each at most cap).
+lstc_status segments_launch_dev(const uint64_t *d_off, const uint32_t *d_col, const uint64_t *d_in_off,
+                                const uint32_t *d_in_col, uint32_t n_rows, uint64_t cap, uint64_t *d_col_seg,
+                                uint64_t *d_in_seg, hipStream_t st);
 
 // Fill kernels (used instead of hipMemsetAsync so callers may capture them in graphs).
 lstc_status fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s);

@@ -104,6 +104,8 @@ struct Req {
     uint32_t *col = nullptr;
     uint64_t *in_off = nullptr;  // target-side in-lists (null for caller CSRs)
     uint32_t *in_col = nullptr;
+    uint64_t *col_seg = nullptr;  // their segment words
+    uint64_t *in_seg = nullptr;
     void *ws = nullptr;
     size_t ws_bytes = 0;
     uint32_t n_rows = 0;
@@ -177,8 +179,10 @@ static size_t prep_bytes(uint32_t n, uint64_t m_raw, size_t *ws_out) {
     return round256(8 * m_raw) + ws;  // raw src/dst + clean workspace
 }
 // resident per request: oriented CSR plus its target-side in-lists (split count)
+// oriented CSR, in-lists (both at most m_raw arcs) and their segment words
 static size_t resident_bytes(uint32_t n, uint64_t m_raw) {
-    return 2 * (round256(8 * ((size_t)n + 1)) + round256(4 * (m_raw ? m_raw : 1)));
+    return 2 * (round256(8 * ((size_t)n + 1)) + round256(4 * (m_raw ? m_raw : 1))) +
+           2 * round256(8 * (m_raw ? m_raw : 1));
 }
 static uint32_t request_rows(const lstc_request &r) {
     if (r.kind == LSTC_REQ_RMAT) return 1u << r.scale;
@@ -218,6 +222,8 @@ static lstc_status admit(lstc_server *S) {
         r->col = (uint32_t *)(res + round256(8 * ((size_t)n + 1)));
         r->in_off = (uint64_t *)(res + half);
         r->in_col = (uint32_t *)(res + half + round256(8 * ((size_t)n + 1)));
+        r->col_seg = (uint64_t *)(res + 2 * half);
+        r->in_seg = (uint64_t *)(res + 2 * half + round256(8 * (r->spec.m_raw ? r->spec.m_raw : 1)));
         r->n_rows = n;
         r->m_cap = r->spec.m_raw;
         hipStream_t st = S->prep_stream;
@@ -231,6 +237,8 @@ static lstc_status admit(lstc_server *S) {
         // compact = 0 keeps n rows, so the count needs no host-side sizes
         LSTC_TRY(clean_orient_launch(r->src, r->dst, r->spec.m_raw, n, 0, nullptr, nullptr, r->off, r->col, r->in_off,
                                      r->in_col, r->ws, r->ws_bytes, st, nullptr, &r->d_bad));
+        LSTC_TRY(segments_launch_dev(r->off, r->col, r->in_off, r->in_col, n, r->spec.m_raw, r->col_seg, r->in_seg,
+                                     st));
         LSTC_HIP(hipEventCreateWithFlags(&r->prep_done, hipEventDisableTiming));
         LSTC_HIP(hipEventRecord(r->prep_done, st));
         S->preparing.push_back(r);
@@ -284,7 +292,8 @@ static lstc_status launch_batch(lstc_server *S) {
     uint64_t acc = 0;
     for (uint32_t g = 0; g < G; ++g) {
         Req *r = batch[g];
-        hosts[g] = HostGraph{r->off, r->col, nullptr, r->n_rows, r->n_rows, r->m_cap, r->in_off, r->in_col};
+        hosts[g] = HostGraph{r->off, r->col, nullptr, r->n_rows, r->n_rows, r->m_cap, r->in_off, r->in_col,
+                             r->col_seg, r->in_seg};
         S->h_desc[g] = to_dev(hosts[g]);
         S->h_base[g] = acc;
         acc += r->n_rows;
