@@ -451,14 +451,34 @@ __device__ __forceinline__ void issue_round(const uint32_t *__restrict__ col, ui
 // VGPR), the masks are combined in scalar registers, and the hit mask is added
 // to the per-lane counter as the carry-in of one vector add.  The few lanes
 // whose chain goes on (about one lane in a hundred at the default table scale)
-// continue alone, chunk by chunk.
+// are queued (id, next bucket) in the window's segment tables, which the
+// rounds no longer read (bases and ends are in registers), and probed 64 at a
+// time when the queue fills and at the window's end.
 __device__ __forceinline__ uint64_t bucket_hits(const uint4 &v, uint32_t x) {
     return ballot(v.x == x) | ballot(v.y == x) | ballot(v.z == x) | ballot(v.w == x);
 }
 
+// Probes the queued lanes of a window (ids qx[i], next bucket qp[i], i < qn)
+// as one chunk, to the end of their chains.
+__device__ __forceinline__ void drain_queue(const uint4 *buckets, uint32_t qmask, uint32_t lane, uint32_t &lane_hits,
+                                            const uint32_t *qx, const uint32_t *qp, uint32_t qn) {
+    const bool act = lane < qn;
+    const uint32_t xv = act ? qx[lane] : kNoId;
+    uint32_t q = act ? qp[lane] : 0u;
+    uint64_t live = ballot(act);
+    while (live != 0) {
+        const uint4 w = buckets[q];
+        const uint64_t H = bucket_hits(w, xv) & live;
+        add_lane_bit(lane_hits, H);
+        live &= ~(H | ballot(w.w == kEmpty));
+        q = (q + 1) & qmask;
+    }
+}
+
 template <int UNROLL, bool FULL>
 __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint32_t B, uint32_t total, const uint32_t *tab,
-                                            uint32_t qmask, uint32_t qshift, uint32_t lane, uint32_t &lane_hits) {
+                                            uint32_t qmask, uint32_t qshift, uint32_t lane, uint32_t &lane_hits,
+                                            uint32_t *qx, uint32_t *qp, uint32_t &qn) {
     const uint4 *buckets = (const uint4 *)tab;
     const uint64_t active = ballot(true);
     uint32_t x[UNROLL], p[UNROLL];
@@ -470,8 +490,8 @@ __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint3
     uint4 v[UNROLL];
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) v[k] = buckets[p[k]];
-    // lanes whose chain goes on (bits of inactive lanes may be set here; the
-    // continuation masks them off)
+    // lanes whose chain goes on (bits of inactive lanes may be set here; they
+    // are masked off below)
     uint64_t going[UNROLL], any = 0;
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) {
@@ -480,19 +500,23 @@ __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint3
         going[k] = ~(H | ballot(v[k].w == kEmpty));
         any |= going[k];
     }
-    if (any == 0) return;
+    if ((any & active) == 0) return;
+    // queue them (id, next bucket); a full queue is probed first
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) {
-        // a uniform loop over the chunk's remaining lanes: every lane reads
-        // its next bucket (in the table: q stays masked), only live lanes count
-        uint64_t live = going[k] & active;
-        uint32_t q = p[k];
-        while (live != 0) {
-            q = (q + 1) & qmask;
-            const uint4 w = buckets[q];
-            const uint64_t H = bucket_hits(w, x[k]) & live;
-            add_lane_bit(lane_hits, H);
-            live &= ~(H | ballot(w.w == kEmpty));
+        const uint64_t g = going[k] & active;
+        if (g != 0) {
+            const uint32_t c = (uint32_t)__popcll(g);
+            if (qn + c > 64u) {
+                drain_queue(buckets, qmask, lane, lane_hits, qx, qp, qn);
+                qn = 0;
+            }
+            if (lane_in(g)) {
+                const uint32_t slot = qn + mask_rank(g);
+                qx[slot] = x[k];
+                qp[slot] = (p[k] + 1) & qmask;
+            }
+            qn += c;
         }
     }
 }
@@ -547,6 +571,7 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
         const uint32_t mybase = s_base[lane];
         const uint32_t myend = lane < nseg ? s_end[lane] : kEmpty;
         uint32_t xa[UNROLL], xb[UNROLL];
+        uint32_t qn = 0;  // queued lanes (in s_base / s_end from here on)
         {
             bool mk;
             const uint32_t sg = round_first<UNROLL>(0, myend, mk);
@@ -567,18 +592,18 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
                 if (ballot(mk) == 0) {
                     const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + RB;
                     issue_single<UNROLL>(col, c0, lane, xb);
-                    probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits);
+                    probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
                 } else {
                     issue_walk<UNROLL>(col, B + RB, mybase, myend, sg, lane, xb);
-                    probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits);
+                    probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
                 }
             } else if (B + RB < total) {
                 bool mk;
                 const uint32_t sg = round_first<UNROLL>(B + RB, myend, mk);
                 issue_round<UNROLL, false>(col, B + RB, total, mybase, myend, sg, mk, flag, lane, xb);
-                probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits);
+                probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
             } else {
-                probe_round<UNROLL, false>(xa, B, total, tab, qmask, qshift, lane, lane_hits);
+                probe_round<UNROLL, false>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
                 break;
             }
             if (B + 3 * RB <= total) {
@@ -587,21 +612,23 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
                 if (ballot(mk) == 0) {
                     const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + 2 * RB;
                     issue_single<UNROLL>(col, c0, lane, xa);
-                    probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits);
+                    probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
                 } else {
                     issue_walk<UNROLL>(col, B + 2 * RB, mybase, myend, sg, lane, xa);
-                    probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits);
+                    probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
                 }
             } else if (B + 2 * RB < total) {
                 bool mk;
                 const uint32_t sg = round_first<UNROLL>(B + 2 * RB, myend, mk);
                 issue_round<UNROLL, false>(col, B + 2 * RB, total, mybase, myend, sg, mk, flag, lane, xa);
-                probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits);
+                probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
             } else {
-                probe_round<UNROLL, false>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits);
+                probe_round<UNROLL, false>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
                 break;
             }
         }
+        // the next window rewrites the segment tables
+        if (qn != 0) drain_queue((const uint4 *)tab, qmask, lane, lane_hits, s_base, s_end, qn);
     }
     // an item streams at most m < 2^32 entries, so its hits fit 32 bits
     return wave_sum32(lane_hits);
