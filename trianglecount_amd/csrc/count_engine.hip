@@ -341,36 +341,20 @@ __device__ __forceinline__ void table_insert(uint32_t *tab, uint32_t tlog2, cons
 // wave holds segment i: mybase = its column start minus its flat start (mod
 // 2^32: column indices are below 2^32 because m < 2^32 - 4096), myend =
 // its exclusive flat end (kEmpty for lanes without a segment); ends ascend.
-// The last entry of every segment that ends inside the round is marked in
-// `flag`, one byte per position stored lane-major (position p at
-// (p % 64) * UNROLL + p / 64), so a single LDS read gives each lane its UNROLL
-// chunk flags; the flags are cleared again right after.  A position's segment
-// is then the round's first segment plus the marks strictly before it: one
-// ballot per chunk and one mbcnt pair per lane; the bases of all chunks are
-// gathered back to back.
 // Positions at or past `total` load col[0] (in bounds, since total > 0 means the
 // graph has arcs) and are never probed: every load is unconditional, so its
 // result register is never copied and the loads stay in flight until probed.
-template <int UNROLL>
-struct FlagWord;  // the UNROLL flag bytes of one lane
-template <>
-struct FlagWord<4> {
-    using T = uint32_t;
-};
-template <>
-struct FlagWord<8> {
-    using T = uint64_t;
-};
 
-// The round's first segment (segments that end at or before B lie wholly
-// before it) and whether this lane's segment ends inside the round (end e in
-// (B, B + RB): it marks position e - 1; an end at B + RB marks the round's last
-// position, before which every position already lies, so it needs no mark).
-template <int UNROLL>
-__device__ __forceinline__ uint32_t round_first(uint32_t B, uint32_t myend, bool &mk) {
-    constexpr uint32_t RB = 64u * UNROLL;
-    mk = myend > B && myend < B + RB;
+// The round's first segment: segments that end at or before B lie wholly
+// before it.
+__device__ __forceinline__ uint32_t first_segment(uint32_t B, uint32_t myend) {
     return (uint32_t)__popcll(ballot(myend <= B));
+}
+// Whether no segment ends inside the round (an end at B + RB leaves every
+// position of the round before it), so the round lies in one segment.
+template <int UNROLL>
+__device__ __forceinline__ bool one_segment(uint32_t B, uint32_t myend) {
+    return ballot(myend > B && myend < B + 64u * UNROLL) == 0;
 }
 
 // A full round that lies wholly in one segment: its column index at position
@@ -382,14 +366,19 @@ __device__ __forceinline__ void issue_single(const uint32_t *__restrict__ col, u
     for (int k = 0; k < UNROLL; ++k) x[k] = gload(col + (c0 + 64u * k) + lane);
 }
 
-// A full round in which segments end: they are the consecutive segments sg,
-// sg + 1, ... (ends ascend), so the round walks them with wave-uniform reads
-// of their bases and ends.  A chunk inside one segment is addressed from its
-// uniform base; in a chunk that a segment end crosses, each lane picks the
-// base of the last segment that starts at or before its position.
-template <int UNROLL>
-__device__ __forceinline__ void issue_walk(const uint32_t *__restrict__ col, uint32_t B, uint32_t mybase, uint32_t myend,
-                                           uint32_t sg, uint32_t lane, uint32_t (&x)[UNROLL]) {
+// A round in which segments end: they are the consecutive segments sg, sg + 1,
+// ... (ends ascend), so the round walks them with wave-uniform reads of their
+// bases and ends.  A chunk inside one segment is addressed from its uniform
+// base; in a chunk that a segment end crosses, each lane picks the base of the
+// last segment that starts at or before its position.  The walk stops at the
+// segment that ends at `total` (the window's last), so j stays below the
+// window's segment count.  FULL: the round lies wholly before `total` (every
+// round of a window but the last), so no position needs the past-the-end
+// handling.
+template <int UNROLL, bool FULL>
+__device__ __forceinline__ void issue_walk(const uint32_t *__restrict__ col, uint32_t B, uint32_t total,
+                                           uint32_t mybase, uint32_t myend, uint32_t sg, uint32_t lane,
+                                           uint32_t (&x)[UNROLL]) {
     uint32_t j = sg;
     uint32_t bj = (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)j);
     uint32_t ej = (uint32_t)__builtin_amdgcn_readlane((int)myend, (int)j);
@@ -398,45 +387,15 @@ __device__ __forceinline__ void issue_walk(const uint32_t *__restrict__ col, uin
         const uint32_t lo = B + 64u * k;
         const uint32_t t = lo + lane;
         uint32_t vb = bj;
-        // segments ending inside this chunk (none for most chunks); they end
-        // before B + RB <= total, so j stays below the window's segment count
-        while (ej < lo + 64u) {
+        // segments ending inside this chunk (none for most chunks)
+        while (ej < lo + 64u && (FULL || ej < total)) {
             const uint32_t e = ej;
             ++j;
             bj = (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)j);
             ej = (uint32_t)__builtin_amdgcn_readlane((int)myend, (int)j);
             vb = t >= e ? bj : vb;
         }
-        x[k] = gload(col + (uint32_t)(vb + t));
-    }
-}
-
-// FULL: the round lies wholly before `total` (every round of a window but the
-// last), so no position needs the past-the-end handling.  sg and mk come from
-// round_first.
-template <int UNROLL, bool FULL>
-__device__ __forceinline__ void issue_round(const uint32_t *__restrict__ col, uint32_t B, uint32_t total,
-                                            uint32_t mybase, uint32_t myend, uint32_t sg, bool mk, uint8_t *flag,
-                                            uint32_t lane, uint32_t (&x)[UNROLL]) {
-    using FW = typename FlagWord<UNROLL>::T;
-    const uint32_t e = myend - 1u - B;
-    const uint32_t fi = (e & 63u) * UNROLL + (e >> 6);
-    if (mk) flag[fi] = 1;
-    const FW fw = ((const FW *)flag)[lane];
-    if (mk) flag[fi] = 0;
-    uint32_t src[UNROLL];  // byte address of the position's segment lane (for ds_bpermute)
-#pragma unroll
-    for (int k = 0; k < UNROLL; ++k) {
-        const uint64_t M = ballot(((fw >> (8 * k)) & 0xFFu) != 0);
-        src[k] = (mask_rank(M) << 2) + (sg << 2);
-        sg += (uint32_t)__popcll(M);
-    }
-#pragma unroll
-    for (int k = 0; k < UNROLL; ++k) {
-        // lanes past the end: any base, not probed
-        const uint32_t base = (uint32_t)__builtin_amdgcn_ds_bpermute((int)src[k], (int)mybase);
-        const uint32_t t = B + 64u * k + lane;
-        x[k] = gload(FULL || t < total ? col + (uint32_t)(base + t) : col);
+        x[k] = gload(FULL || t < total ? col + (uint32_t)(vb + t) : col);
     }
 }
 
@@ -536,7 +495,7 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
                                                    const uint64_t *__restrict__ segw, uint32_t nseg_all,
                                                    uint32_t below, uint32_t w0, uint32_t wstep, const uint32_t *tab,
                                                    uint32_t tlog2, uint32_t *s_base, uint32_t *s_end,
-                                                   uint8_t *flag, uint32_t lane) {
+                                                   uint32_t lane) {
     constexpr uint32_t RB = 64u * UNROLL;
     const uint32_t qmask = (1u << (tlog2 - 2)) - 1, qshift = 34 - tlog2;
     uint32_t lane_hits = 0;
@@ -573,54 +532,49 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
         uint32_t xa[UNROLL], xb[UNROLL];
         uint32_t qn = 0;  // queued lanes (in s_base / s_end from here on)
         {
-            bool mk;
-            const uint32_t sg = round_first<UNROLL>(0, myend, mk);
+            const uint32_t sg = first_segment(0, myend);
             if (RB > total)
-                issue_round<UNROLL, false>(col, 0, total, mybase, myend, sg, mk, flag, lane, xa);
-            else if (ballot(mk) == 0)
+                issue_walk<UNROLL, false>(col, 0, total, mybase, myend, sg, lane, xa);
+            else if (one_segment<UNROLL>(0, myend))
                 issue_single<UNROLL>(col, (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg), lane, xa);
             else
-                issue_walk<UNROLL>(col, 0, mybase, myend, sg, lane, xa);
+                issue_walk<UNROLL, true>(col, 0, total, mybase, myend, sg, lane, xa);
         }
         // The probe is written out in every arm of each branch: a join between
         // an issue and the probe would make the wait for the probed buffer also
         // wait for the loads just issued.  A round followed by another is full.
         for (uint32_t B = 0;; B += 2 * RB) {
             if (B + 2 * RB <= total) {
-                bool mk;
-                const uint32_t sg = round_first<UNROLL>(B + RB, myend, mk);
-                if (ballot(mk) == 0) {
+                const uint32_t sg = first_segment(B + RB, myend);
+                if (one_segment<UNROLL>(B + RB, myend)) {
                     const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + RB;
                     issue_single<UNROLL>(col, c0, lane, xb);
                     probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
                 } else {
-                    issue_walk<UNROLL>(col, B + RB, mybase, myend, sg, lane, xb);
+                    issue_walk<UNROLL, true>(col, B + RB, total, mybase, myend, sg, lane, xb);
                     probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
                 }
             } else if (B + RB < total) {
-                bool mk;
-                const uint32_t sg = round_first<UNROLL>(B + RB, myend, mk);
-                issue_round<UNROLL, false>(col, B + RB, total, mybase, myend, sg, mk, flag, lane, xb);
+                const uint32_t sg = first_segment(B + RB, myend);
+                issue_walk<UNROLL, false>(col, B + RB, total, mybase, myend, sg, lane, xb);
                 probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
             } else {
                 probe_round<UNROLL, false>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
                 break;
             }
             if (B + 3 * RB <= total) {
-                bool mk;
-                const uint32_t sg = round_first<UNROLL>(B + 2 * RB, myend, mk);
-                if (ballot(mk) == 0) {
+                const uint32_t sg = first_segment(B + 2 * RB, myend);
+                if (one_segment<UNROLL>(B + 2 * RB, myend)) {
                     const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + 2 * RB;
                     issue_single<UNROLL>(col, c0, lane, xa);
                     probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
                 } else {
-                    issue_walk<UNROLL>(col, B + 2 * RB, mybase, myend, sg, lane, xa);
+                    issue_walk<UNROLL, true>(col, B + 2 * RB, total, mybase, myend, sg, lane, xa);
                     probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
                 }
             } else if (B + 2 * RB < total) {
-                bool mk;
-                const uint32_t sg = round_first<UNROLL>(B + 2 * RB, myend, mk);
-                issue_round<UNROLL, false>(col, B + 2 * RB, total, mybase, myend, sg, mk, flag, lane, xa);
+                const uint32_t sg = first_segment(B + 2 * RB, myend);
+                issue_walk<UNROLL, false>(col, B + 2 * RB, total, mybase, myend, sg, lane, xa);
                 probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
             } else {
                 probe_round<UNROLL, false>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
@@ -753,7 +707,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 4) void k_count(Grap
     __shared__ __align__(16) uint32_t s_tab[kBigCap];
     __shared__ uint32_t s_base[kWaves][64];
     __shared__ uint32_t s_end[kWaves][64];
-    __shared__ __align__(16) uint32_t s_flag[kWaves][16 * UNROLL];  // 64*UNROLL bytes per wave
+    __shared__ __align__(16) uint32_t s_flag[kWaves][16];  // 64 position flags per wave (tiny rows)
     __shared__ uint8_t s_src[kWaves][64];
     __shared__ uint32_t s_ticket;
     __shared__ unsigned long long s_acc[kWaves];
@@ -761,7 +715,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 4) void k_count(Grap
 
     // item and graph fields are wave-uniform: they live in scalar registers
     const uint32_t tid = threadIdx.x, wv = uniform(tid >> 6), lane = tid & 63;
-    for (uint32_t i = lane; i < 16 * UNROLL; i += 64) s_flag[wv][i] = 0;
+    if (lane < 16) s_flag[wv][lane] = 0;
     const uint32_t n_big = sload(ctl + C_NBIG), n_small = sload(ctl + C_NSMALL), n_tickets = sload(ctl + C_NTICK);
     const uint32_t small_end = n_big + sload(ctl + C_NSMALLT);  // first tiny ticket
     const uint32_t tiny0 = sload(ctl + C_TINY0), tiny_end = tiny0 + sload(ctl + C_NTINY);
@@ -791,7 +745,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 4) void k_count(Grap
                 table_insert(s_tab, tl, G.col, rb + t0, tk, tid, kWaves * 64);
                 __syncthreads();
                 hits += walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.seg, sg.n, sg.below, wv * 64, kWaves * 64, s_tab, tl,
-                                     s_base[wv], s_end[wv], (uint8_t *)s_flag[wv], lane);
+                                     s_base[wv], s_end[wv], lane);
                 __syncthreads();
             }
             if (g != cur_g) {
@@ -840,7 +794,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 4) void k_count(Grap
                 table_insert(tab, tl, G.col, rb, K, lane, 64);
                 const unsigned long long hits =
                     walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.seg, sg.n, sg.below, 0, 64, tab, tl,
-                                 s_base[wv], s_end[wv], (uint8_t *)s_flag[wv], lane);
+                                 s_base[wv], s_end[wv], lane);
                 if (g != cur_g) {
                     if (cur_g != kEmpty && acc != 0 && lane == 0) atomicAdd(counts + cur_g, acc);
                     cur_g = g;

@@ -1,7 +1,8 @@
 // selftest.hip -- device check of the wave64 primitives the kernels build on.
 //
 // The counting walk turns DPP prefix sums, lane permutes and ballot ranks into
-// load addresses, so a wrong lane layout would send loads astray.  This check
+// load addresses and counts hits by adding ballot masks lane by lane, so a
+// wrong lane layout would send loads astray or miscount.  This check
 // runs each primitive on fixed patterns, one wave per pattern, and compares the
 // lanes with host arithmetic.  Its kernel only writes its own output buffer.
 #include <vector>
@@ -12,7 +13,8 @@
 namespace lstc {
 
 static constexpr uint32_t kPatterns = 64;
-static constexpr uint32_t kOutputs = 5;  // value, inclusive scan, sum, reversed gather, odd-rank
+static constexpr uint32_t kOutputs = 7;  // value, inclusive scan, sum, reversed gather, odd-rank,
+                                         // mask-bit adds, inverse ballot
 
 __device__ __forceinline__ uint32_t pattern_value(uint32_t p, uint32_t lane) {
     switch (p) {
@@ -42,6 +44,11 @@ __global__ __launch_bounds__(64) void k_wave_selftest(uint32_t *__restrict__ out
     o[2 * 64 + lane] = wave_sum32(v);
     o[3 * 64 + lane] = gather(v, 63 - lane);
     o[4 * 64 + lane] = mask_rank(ballot((v & 1u) != 0));
+    uint32_t cnt = 3u * lane;
+    add_lane_bit(cnt, ballot((v & 2u) != 0));
+    add_lane_bit(cnt, ballot((v & 4u) != 0));
+    o[5 * 64 + lane] = cnt;
+    o[6 * 64 + lane] = lane_in(ballot((v & 1u) != 0) ^ 0x5555555555555555ull) ? 1u : 0u;
 }
 
 }  // namespace lstc
@@ -72,6 +79,8 @@ extern "C" lstc_status lstc_selftest(uint32_t *mismatches) {
             bad += o[192 + l] != o[63 - l];
             bad += o[256 + l] != odd;
             odd += o[l] & 1u;
+            bad += o[320 + l] != 3u * l + ((o[l] >> 1) & 1u) + ((o[l] >> 2) & 1u);
+            bad += o[384 + l] != ((o[l] & 1u) ^ (l & 1u ? 0u : 1u));
         }
     }
     *mismatches = bad;
