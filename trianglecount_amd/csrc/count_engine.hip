@@ -410,20 +410,21 @@ __device__ __forceinline__ void issue_walk(const uint32_t *__restrict__ col, uin
 // VGPR), the masks are combined in scalar registers, and the hit mask is added
 // to the per-lane counter as the carry-in of one vector add.  The few lanes
 // whose chain goes on (about one lane in a hundred at the default table scale)
-// are queued (id, next bucket) in the window's segment tables, which the
-// rounds no longer read (bases and ends are in registers), and probed 64 at a
-// time when the queue fills and at the window's end.
+// are queued (their ids) in the window's segment table, which the rounds no
+// longer read (bases and ends are in registers), and probed 64 at a time when
+// the queue fills and at the window's end.
 __device__ __forceinline__ uint64_t bucket_hits(const uint4 &v, uint32_t x) {
     return ballot(v.x == x) | ballot(v.y == x) | ballot(v.z == x) | ballot(v.w == x);
 }
 
-// Probes the queued lanes of a window (ids qx[i], next bucket qp[i], i < qn)
-// as one chunk, to the end of their chains.
-__device__ __forceinline__ void drain_queue(const uint4 *buckets, uint32_t qmask, uint32_t lane, uint32_t &lane_hits,
-                                            const uint32_t *qx, const uint32_t *qp, uint32_t qn) {
+// Probes the queued lanes of a window (ids qx[i], i < qn, whose home bucket
+// was full without a hit) as one chunk, from the bucket after their home to
+// the end of their chains.
+__device__ __forceinline__ void drain_queue(const uint4 *buckets, uint32_t qmask, uint32_t qshift, uint32_t lane,
+                                            uint32_t &lane_hits, const uint32_t *qx, uint32_t qn) {
     const bool act = lane < qn;
     const uint32_t xv = act ? qx[lane] : kNoId;
-    uint32_t q = act ? qp[lane] : 0u;
+    uint32_t q = (hquad(xv, qshift) + 1u) & qmask;
     uint64_t live = ballot(act);
     while (live != 0) {
         const uint4 w = buckets[q];
@@ -437,7 +438,7 @@ __device__ __forceinline__ void drain_queue(const uint4 *buckets, uint32_t qmask
 template <int UNROLL, bool FULL>
 __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint32_t B, uint32_t total, const uint32_t *tab,
                                             uint32_t qmask, uint32_t qshift, uint32_t lane, uint32_t &lane_hits,
-                                            uint32_t *qx, uint32_t *qp, uint32_t &qn) {
+                                            uint32_t *qx, uint32_t &qn) {
     const uint4 *buckets = (const uint4 *)tab;
     const uint64_t active = ballot(true);
     uint32_t x[UNROLL], p[UNROLL];
@@ -467,14 +468,10 @@ __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint3
         if (g != 0) {
             const uint32_t c = (uint32_t)__popcll(g);
             if (qn + c > 64u) {
-                drain_queue(buckets, qmask, lane, lane_hits, qx, qp, qn);
+                drain_queue(buckets, qmask, qshift, lane, lane_hits, qx, qn);
                 qn = 0;
             }
-            if (lane_in(g)) {
-                const uint32_t slot = qn + mask_rank(g);
-                qx[slot] = x[k];
-                qp[slot] = (p[k] + 1) & qmask;
-            }
+            if (lane_in(g)) qx[qn + mask_rank(g)] = x[k];
             qn += c;
         }
     }
@@ -530,7 +527,7 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
         const uint32_t mybase = s_base[lane];
         const uint32_t myend = lane < nseg ? s_end[lane] : kEmpty;
         uint32_t xa[UNROLL], xb[UNROLL];
-        uint32_t qn = 0;  // queued lanes (in s_base / s_end from here on)
+        uint32_t qn = 0;  // queued lanes (in s_base from here on)
         {
             const uint32_t sg = first_segment(0, myend);
             if (RB > total)
@@ -549,17 +546,17 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
                 if (one_segment<UNROLL>(B + RB, myend)) {
                     const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + RB;
                     issue_single<UNROLL>(col, c0, lane, xb);
-                    probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
+                    probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
                 } else {
                     issue_walk<UNROLL, true>(col, B + RB, total, mybase, myend, sg, lane, xb);
-                    probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
+                    probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
                 }
             } else if (B + RB < total) {
                 const uint32_t sg = first_segment(B + RB, myend);
                 issue_walk<UNROLL, false>(col, B + RB, total, mybase, myend, sg, lane, xb);
-                probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
+                probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
             } else {
-                probe_round<UNROLL, false>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
+                probe_round<UNROLL, false>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
                 break;
             }
             if (B + 3 * RB <= total) {
@@ -567,22 +564,22 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
                 if (one_segment<UNROLL>(B + 2 * RB, myend)) {
                     const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + 2 * RB;
                     issue_single<UNROLL>(col, c0, lane, xa);
-                    probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
+                    probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
                 } else {
                     issue_walk<UNROLL, true>(col, B + 2 * RB, total, mybase, myend, sg, lane, xa);
-                    probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
+                    probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
                 }
             } else if (B + 2 * RB < total) {
                 const uint32_t sg = first_segment(B + 2 * RB, myend);
                 issue_walk<UNROLL, false>(col, B + 2 * RB, total, mybase, myend, sg, lane, xa);
-                probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
+                probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
             } else {
-                probe_round<UNROLL, false>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, s_end, qn);
+                probe_round<UNROLL, false>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
                 break;
             }
         }
         // the next window rewrites the segment tables
-        if (qn != 0) drain_queue((const uint4 *)tab, qmask, lane, lane_hits, s_base, s_end, qn);
+        if (qn != 0) drain_queue((const uint4 *)tab, qmask, qshift, lane, lane_hits, s_base, qn);
     }
     // an item streams at most m < 2^32 entries, so its hits fit 32 bits
     return wave_sum32(lane_hits);
