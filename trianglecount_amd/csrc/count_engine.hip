@@ -697,7 +697,7 @@ __device__ __forceinline__ uint32_t tiny_walk(const GraphDev &G, bool mine, bool
 
 // ------------------------------------------------------------------ persistent count kernel
 template <int UNROLL>
-__global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 4) void k_count(GraphSet gs, const unsigned long long *__restrict__ queue,
+__global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(GraphSet gs, const unsigned long long *__restrict__ queue,
                                                        uint32_t *__restrict__ ctl,
                                                        unsigned long long *__restrict__ counts, uint32_t scale,
                                                        uint32_t small_group) {
