@@ -360,10 +360,9 @@ __device__ __forceinline__ bool one_segment(uint32_t B, uint32_t myend) {
 // A full round that lies wholly in one segment: its column index at position
 // B + 64k is c0 + 64k (uniform, below m), so only the lane offset is per lane.
 template <int UNROLL>
-__device__ __forceinline__ void issue_single(const uint32_t *__restrict__ col, uint32_t c0, uint32_t lane,
-                                             uint32_t (&x)[UNROLL]) {
+__device__ __forceinline__ void index_single(uint32_t c0, uint32_t lane, uint32_t (&idx)[UNROLL]) {
 #pragma unroll
-    for (int k = 0; k < UNROLL; ++k) x[k] = gload(col + (c0 + 64u * k) + lane);
+    for (int k = 0; k < UNROLL; ++k) idx[k] = c0 + 64u * k + lane;
 }
 
 // A round in which segments end: they are the consecutive segments sg, sg + 1,
@@ -374,11 +373,10 @@ __device__ __forceinline__ void issue_single(const uint32_t *__restrict__ col, u
 // segment that ends at `total` (the window's last), so j stays below the
 // window's segment count.  FULL: the round lies wholly before `total` (every
 // round of a window but the last), so no position needs the past-the-end
-// handling.
+// handling (index 0 otherwise).
 template <int UNROLL, bool FULL>
-__device__ __forceinline__ void issue_walk(const uint32_t *__restrict__ col, uint32_t B, uint32_t total,
-                                           uint32_t mybase, uint32_t myend, uint32_t sg, uint32_t lane,
-                                           uint32_t (&x)[UNROLL]) {
+__device__ __forceinline__ void index_walk(uint32_t B, uint32_t total, uint32_t mybase, uint32_t myend, uint32_t sg,
+                                           uint32_t lane, uint32_t (&idx)[UNROLL]) {
     uint32_t j = sg;
     uint32_t bj = (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)j);
     uint32_t ej = (uint32_t)__builtin_amdgcn_readlane((int)myend, (int)j);
@@ -395,8 +393,18 @@ __device__ __forceinline__ void issue_walk(const uint32_t *__restrict__ col, uin
             ej = (uint32_t)__builtin_amdgcn_readlane((int)myend, (int)j);
             vb = t >= e ? bj : vb;
         }
-        x[k] = gload(FULL || t < total ? col + (uint32_t)(vb + t) : col);
+        idx[k] = FULL || t < total ? vb + t : 0u;
     }
+}
+
+// The loads of one round, one per chunk, unconditional (past-the-end positions
+// read col[0]).  Every round's loads go through this one code path, so each
+// buffer keeps its registers across the branches that chose the indices.
+template <int UNROLL>
+__device__ __forceinline__ void load_round(const uint32_t *__restrict__ col, const uint32_t (&idx)[UNROLL],
+                                           uint32_t (&x)[UNROLL]) {
+#pragma unroll
+    for (int k = 0; k < UNROLL; ++k) x[k] = gload(col + idx[k]);
 }
 
 // Probes one round through tab: linear probing over 4-slot buckets, the
@@ -528,54 +536,57 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
         const uint32_t myend = lane < nseg ? s_end[lane] : kEmpty;
         uint32_t xa[UNROLL], xb[UNROLL];
         uint32_t qn = 0;  // queued lanes (in s_base from here on)
+        // Each round's indices come from one of three cases (inside one
+        // segment, crossed by segment ends, the window's partial last round);
+        // the loads and the probe follow in one common path, so the two
+        // buffers keep their registers and the wait for the probed buffer
+        // never covers the loads just issued.
+        uint32_t idx[UNROLL];
         {
             const uint32_t sg = first_segment(0, myend);
             if (RB > total)
-                issue_walk<UNROLL, false>(col, 0, total, mybase, myend, sg, lane, xa);
+                index_walk<UNROLL, false>(0, total, mybase, myend, sg, lane, idx);
             else if (one_segment<UNROLL>(0, myend))
-                issue_single<UNROLL>(col, (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg), lane, xa);
+                index_single<UNROLL>((uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg), lane, idx);
             else
-                issue_walk<UNROLL, true>(col, 0, total, mybase, myend, sg, lane, xa);
+                index_walk<UNROLL, true>(0, total, mybase, myend, sg, lane, idx);
         }
-        // The probe is written out in every arm of each branch: a join between
-        // an issue and the probe would make the wait for the probed buffer also
-        // wait for the loads just issued.  A round followed by another is full.
-        for (uint32_t B = 0;; B += 2 * RB) {
-            if (B + 2 * RB <= total) {
-                const uint32_t sg = first_segment(B + RB, myend);
-                if (one_segment<UNROLL>(B + RB, myend)) {
-                    const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + RB;
-                    issue_single<UNROLL>(col, c0, lane, xb);
-                    probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
-                } else {
-                    issue_walk<UNROLL, true>(col, B + RB, total, mybase, myend, sg, lane, xb);
-                    probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
-                }
-            } else if (B + RB < total) {
-                const uint32_t sg = first_segment(B + RB, myend);
-                issue_walk<UNROLL, false>(col, B + RB, total, mybase, myend, sg, lane, xb);
-                probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
-            } else {
-                probe_round<UNROLL, false>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
-                break;
-            }
-            if (B + 3 * RB <= total) {
-                const uint32_t sg = first_segment(B + 2 * RB, myend);
-                if (one_segment<UNROLL>(B + 2 * RB, myend)) {
-                    const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + 2 * RB;
-                    issue_single<UNROLL>(col, c0, lane, xa);
-                    probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
-                } else {
-                    issue_walk<UNROLL, true>(col, B + 2 * RB, total, mybase, myend, sg, lane, xa);
-                    probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
-                }
-            } else if (B + 2 * RB < total) {
-                const uint32_t sg = first_segment(B + 2 * RB, myend);
-                issue_walk<UNROLL, false>(col, B + 2 * RB, total, mybase, myend, sg, lane, xa);
+        load_round<UNROLL>(col, idx, xa);
+        uint32_t B = 0;
+        for (; B + 3 * RB <= total; B += 2 * RB) {
+            // rounds B + RB and B + 2 * RB are full
+            uint32_t sg = first_segment(B + RB, myend);
+            if (one_segment<UNROLL>(B + RB, myend))
+                index_single<UNROLL>((uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + RB, lane, idx);
+            else
+                index_walk<UNROLL, true>(B + RB, total, mybase, myend, sg, lane, idx);
+            load_round<UNROLL>(col, idx, xb);
+            probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
+            sg = first_segment(B + 2 * RB, myend);
+            if (one_segment<UNROLL>(B + 2 * RB, myend))
+                index_single<UNROLL>((uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + 2 * RB, lane,
+                                     idx);
+            else
+                index_walk<UNROLL, true>(B + 2 * RB, total, mybase, myend, sg, lane, idx);
+            load_round<UNROLL>(col, idx, xa);
+            probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
+        }
+        // Tail: round B is in xa; rounds B + RB and B + 2 * RB may follow
+        // (B + 3 * RB > total, so the last one is partial).
+        if (B + RB >= total) {
+            probe_round<UNROLL, false>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
+        } else {
+            index_walk<UNROLL, false>(B + RB, total, mybase, myend, first_segment(B + RB, myend), lane, idx);
+            load_round<UNROLL>(col, idx, xb);
+            probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
+            if (B + 2 * RB < total) {
+                index_walk<UNROLL, false>(B + 2 * RB, total, mybase, myend, first_segment(B + 2 * RB, myend), lane,
+                                          idx);
+                load_round<UNROLL>(col, idx, xa);
                 probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
+                probe_round<UNROLL, false>(xa, B + 2 * RB, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
             } else {
                 probe_round<UNROLL, false>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
-                break;
             }
         }
         // the next window rewrites the segment tables
