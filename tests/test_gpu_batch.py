@@ -66,6 +66,28 @@ def test_plan_run_and_graph_replay(lstc, oracle):
         plan.close()
 
 
+def test_split_plan_run_and_graph_replay(lstc, oracle):
+    """Graphs carrying in-lists and segment words (and plain ones) in a plan and its hipGraph replays."""
+    gs, want = _mixed(lstc, oracle)
+    for g in gs[::2]:
+        lstc.in_lists(g)  # split counting with segment words for every other graph
+    for g in gs[1:4:2]:
+        lstc.in_lists(g)
+        g.col_seg = g.in_seg = None  # split counting through the row offsets
+    assert lstc.count_batch(gs) == want
+    plan = lstc.BatchPlan(gs, tune=lstc.Tune(table_scale=4, unroll=8))
+    try:
+        plan.run()
+        assert plan.results() == want
+        for _ in range(3):
+            plan.counts.fill_(777)
+            plan.launch_graph()
+            assert plan.results() == want
+            assert lstc.count(gs[0]) == want[0]
+    finally:
+        plan.close()
+
+
 @pytest.mark.parametrize("tune", [dict(table_scale=2), dict(table_scale=8), dict(small_max=64, small_group=4),
                                   dict(unroll=4, grid_per_cu=2), dict(small_max=512, table_scale=2),
                                   dict(table_scale=16)])
