@@ -318,12 +318,20 @@ __global__ __launch_bounds__(1024) void k_plan_scan(uint32_t *__restrict__ wcnt,
 }
 
 // ------------------------------------------------------------------ table build
-__device__ __forceinline__ void table_insert(uint32_t *tab, uint32_t tlog2, const uint32_t *__restrict__ col,
-                                             uint64_t b0, uint32_t nk, uint32_t tid, uint32_t nthreads) {
+// A key whose home bucket is full moves on; its home is then marked in the
+// overflow bitmap `ovf` (one bit per bucket).  table_mark afterwards records
+// the mark in the bucket itself by the order of its first two slots (an
+// overflowed bucket is full: slot 0 > slot 1; any other: slot 0 < slot 1, or
+// an empty slot 1), so a probe learns from its one bucket read whether a key
+// of that home can lie further on.
+__device__ __forceinline__ void table_insert(uint32_t *tab, uint32_t *ovf, uint32_t tlog2,
+                                             const uint32_t *__restrict__ col, uint64_t b0, uint32_t nk, uint32_t tid,
+                                             uint32_t nthreads) {
     const uint32_t qmask = (1u << (tlog2 - 2)) - 1, qshift = 34 - tlog2;
     for (uint32_t j = tid; j < nk; j += nthreads) {
         const uint32_t key = gload(col + b0 + j);
-        uint32_t h = hquad(key, qshift);
+        const uint32_t home = hquad(key, qshift);
+        uint32_t h = home;
         // a slot is tried only once the ones before it in the bucket are taken
         for (;;) {
             uint32_t *b = tab + 4 * h;
@@ -332,6 +340,18 @@ __device__ __forceinline__ void table_insert(uint32_t *tab, uint32_t tlog2, cons
                 break;
             h = (h + 1) & qmask;
         }
+        if (h != home) atomicOr(ovf + (home >> 5), 1u << (home & 31u));
+    }
+}
+
+// Orders the first two slots of every bucket by its overflow mark (the keys
+// stay in the bucket, so every chain and membership is unchanged).
+__device__ __forceinline__ void table_mark(uint32_t *tab, const uint32_t *ovf, uint32_t tlog2, uint32_t tid,
+                                           uint32_t nthreads) {
+    for (uint32_t b = tid; b < (1u << (tlog2 - 2)); b += nthreads) {
+        const uint2 v = *(const uint2 *)(tab + 4 * b);
+        const bool over = (ovf[b >> 5] >> (b & 31u)) & 1u;
+        if (over ? v.x < v.y : v.x > v.y) *(uint2 *)(tab + 4 * b) = uint2{v.y, v.x};
     }
 }
 
@@ -465,7 +485,7 @@ __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint3
     for (int k = 0; k < UNROLL; ++k) {
         const uint64_t H = bucket_hits(v[k], x[k]);
         add_lane_bit(lane_hits, H);
-        going[k] = ~(H | ballot(v[k].w == kEmpty));
+        going[k] = ~(H | ballot(v[k].x <= v[k].y));  // no hit and the home bucket overflowed
         any |= going[k];
     }
     if ((any & active) == 0) return;
@@ -713,6 +733,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(Grap
                                                        unsigned long long *__restrict__ counts, uint32_t scale,
                                                        uint32_t small_group) {
     __shared__ __align__(16) uint32_t s_tab[kBigCap];
+    __shared__ uint32_t s_ovf[kBigCap / 128];  // overflow bits, one per 4-slot bucket
     __shared__ uint32_t s_base[kWaves][64];
     __shared__ uint32_t s_end[kWaves][64];
     __shared__ __align__(16) uint32_t s_flag[kWaves][16];  // 64 position flags per wave (tiny rows)
@@ -749,8 +770,11 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(Grap
                 const uint32_t tk = min(K - t0, kBigTile);
                 const uint32_t tl = table_log2_for(tk, scale, kBigLog2);
                 for (uint32_t i = tid; i < (1u << tl); i += kWaves * 64) s_tab[i] = kEmpty;
+                if (tid < kBigCap / 128) s_ovf[tid] = 0;
                 __syncthreads();
-                table_insert(s_tab, tl, G.col, rb + t0, tk, tid, kWaves * 64);
+                table_insert(s_tab, s_ovf, tl, G.col, rb + t0, tk, tid, kWaves * 64);
+                __syncthreads();
+                table_mark(s_tab, s_ovf, tl, tid, kWaves * 64);
                 __syncthreads();
                 hits += walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.seg, sg.n, sg.below, wv * 64, kWaves * 64, s_tab, tl,
                                      s_base[wv], s_end[wv], lane);
@@ -798,8 +822,11 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(Grap
                 const uint32_t K = (uint32_t)(sload(G.off + u + 1) - rb);
                 const Segs sg = item_segs(G, kind, chunk, u, rb, K);
                 const uint32_t tl = table_log2_for(K, scale, kSliceLog2);
+                uint32_t *ovf = s_ovf + wv * (kSlice / 128);
                 for (uint32_t i = lane; i < (1u << tl); i += 64) tab[i] = kEmpty;
-                table_insert(tab, tl, G.col, rb, K, lane, 64);
+                if (lane < kSlice / 128) ovf[lane] = 0;
+                table_insert(tab, ovf, tl, G.col, rb, K, lane, 64);
+                table_mark(tab, ovf, tl, lane, 64);
                 const unsigned long long hits =
                     walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.seg, sg.n, sg.below, 0, 64, tab, tl,
                                  s_base[wv], s_end[wv], lane);
