@@ -147,3 +147,30 @@ def test_in_lists_of_any_csr(lstc, oracle, cut_frac):
         return (og.off[ids].astype(np.uint64) & np.uint64(0xFFFFFFFF)) | (d[ids].astype(np.uint64) << np.uint64(32))
     assert np.array_equal(g.col_seg[: og.m].cpu().numpy().view(np.uint64), words(og.col))
     assert np.array_equal(g.in_seg[: len(want_col)].cpu().numpy().view(np.uint64), words(want_col))
+
+
+def test_count_above_2_30_arcs(lstc):
+    """m > 2^30 arcs, so column byte offsets need more than 32 bits: disjoint K64
+    copies with id-ordered arcs, built in HBM directly.  Closed form: copies * C(64, 3),
+    plain and split (in-lists and segment words)."""
+    k = 64
+    per = k * (k - 1) // 2
+    copies = (1 << 30) // per + 1
+    n, m = k * copies, per * copies
+    dev = torch.device("cuda")
+    d = k - 1 - torch.arange(k, dtype=torch.int64)
+    row0 = torch.cumsum(d, 0) - d  # start of local row i inside a clique's block
+    tcol = torch.cat([torch.arange(i + 1, k, dtype=torch.int32) for i in range(k)]).to(dev)
+    base = torch.arange(copies, dtype=torch.int32, device=dev) * k
+    col = (tcol[None, :] + base[:, None]).reshape(-1)
+    off = torch.empty(n + 1, dtype=torch.int64, device=dev)
+    off[:n] = (torch.arange(copies, dtype=torch.int64, device=dev)[:, None] * per + row0.to(dev)[None, :]).reshape(-1)
+    off[n] = m
+    del base
+    g = lstc.Graph(n, m, off, col)
+    want = copies * (k * (k - 1) * (k - 2) // 6)
+    assert m > (1 << 30)
+    assert lstc.count(g) == want
+    lstc.in_lists(g)
+    assert lstc.count(g) == want
+    assert lstc.count(g, tune=lstc.Tune(unroll=8)) == want

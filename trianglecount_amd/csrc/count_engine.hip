@@ -377,12 +377,20 @@ __device__ __forceinline__ bool one_segment(uint32_t B, uint32_t myend) {
     return ballot(myend > B && myend < B + 64u * UNROLL) == 0;
 }
 
+// Column positions are element indices, or with NARROW (every index below
+// 2^30) byte offsets, which the loads take as they are.
+template <bool NARROW>
+__device__ __forceinline__ uint32_t col_pos(uint32_t i) {
+    return NARROW ? i << 2 : i;
+}
+
 // A full round that lies wholly in one segment: its column index at position
 // B + 64k is c0 + 64k (uniform, below m), so only the lane offset is per lane.
-template <int UNROLL>
+template <int UNROLL, bool NARROW>
 __device__ __forceinline__ void index_single(uint32_t c0, uint32_t lane, uint32_t (&idx)[UNROLL]) {
+    const uint32_t i0 = col_pos<NARROW>(c0 + lane);
 #pragma unroll
-    for (int k = 0; k < UNROLL; ++k) idx[k] = c0 + 64u * k + lane;
+    for (int k = 0; k < UNROLL; ++k) idx[k] = i0 + col_pos<NARROW>(64u * k);
 }
 
 // A round in which segments end: they are the consecutive segments sg, sg + 1,
@@ -394,7 +402,7 @@ __device__ __forceinline__ void index_single(uint32_t c0, uint32_t lane, uint32_
 // window's segment count.  FULL: the round lies wholly before `total` (every
 // round of a window but the last), so no position needs the past-the-end
 // handling (index 0 otherwise).
-template <int UNROLL, bool FULL>
+template <int UNROLL, bool FULL, bool NARROW>
 __device__ __forceinline__ void index_walk(uint32_t B, uint32_t total, uint32_t mybase, uint32_t myend, uint32_t sg,
                                            uint32_t lane, uint32_t (&idx)[UNROLL]) {
     uint32_t j = sg;
@@ -413,18 +421,18 @@ __device__ __forceinline__ void index_walk(uint32_t B, uint32_t total, uint32_t 
             ej = (uint32_t)__builtin_amdgcn_readlane((int)myend, (int)j);
             vb = t >= e ? bj : vb;
         }
-        idx[k] = FULL || t < total ? vb + t : 0u;
+        idx[k] = FULL || t < total ? col_pos<NARROW>(vb + t) : 0u;
     }
 }
 
 // The loads of one round, one per chunk, unconditional (past-the-end positions
 // read col[0]).  Every round's loads go through this one code path, so each
 // buffer keeps its registers across the branches that chose the indices.
-template <int UNROLL>
+template <int UNROLL, bool NARROW>
 __device__ __forceinline__ void load_round(const uint32_t *__restrict__ col, const uint32_t (&idx)[UNROLL],
                                            uint32_t (&x)[UNROLL]) {
 #pragma unroll
-    for (int k = 0; k < UNROLL; ++k) x[k] = gload(col + idx[k]);
+    for (int k = 0; k < UNROLL; ++k) x[k] = NARROW ? gload_off32(col, idx[k]) : gload(col + idx[k]);
 }
 
 // Probes one round through tab: linear probing over 4-slot buckets, the
@@ -510,11 +518,11 @@ __device__ __forceinline__ void probe_round(const uint32_t (&xin)[UNROLL], uint3
 // `below` is skipped (below = 0: none skipped); ids >= n_rows have no list.
 // With segment words (segw[j] = start | length << 32 of N+(seg[j])) a window
 // reads one coalesced word per segment instead of two row offsets at a random
-// place.
+// place, and the next window's words are loaded while this one streams.
 // Rounds are software-pipelined over two buffers: the loads of round r+1 are
 // issued before round r is probed, so a wave keeps up to 2*UNROLL chunk loads
 // in flight.
-template <int UNROLL>
+template <int UNROLL, bool NARROW>
 __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const uint32_t *__restrict__ col,
                                                    uint32_t n_rows, const uint32_t *__restrict__ seg,
                                                    const uint64_t *__restrict__ segw, uint32_t nseg_all,
@@ -524,23 +532,25 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
     constexpr uint32_t RB = 64u * UNROLL;
     const uint32_t qmask = (1u << (tlog2 - 2)) - 1, qshift = 34 - tlog2;
     uint32_t lane_hits = 0;
+    // the next window's segment words are loaded while this window streams
+    uint64_t sw_next = segw && w0 + lane < nseg_all ? gload(segw + w0 + lane) : 0;
     for (uint32_t j0 = w0; j0 < nseg_all; j0 += wstep) {
         const uint32_t j = j0 + lane;
         uint32_t s = 0, len = 0;
-        if (j < nseg_all) {
-            if (segw) {  // one coalesced word: start | length << 32
-                const uint64_t sw = gload(segw + j);
-                s = (uint32_t)sw;
-                len = (uint32_t)(sw >> 32);
-            } else {
-                const uint32_t w = gload(seg + j);
-                if (w < n_rows) {
-                    const uint64_t s0 = gload(off + w);
-                    s = (uint32_t)s0;
-                    len = (uint32_t)(gload(off + w + 1) - s0);
-                }
-            }
+        if (segw) {
+            const uint64_t sw = sw_next;  // one coalesced word: start | length << 32 (0 past the end)
+            s = (uint32_t)sw;
+            len = (uint32_t)(sw >> 32);
             if (below && len >= below) len = 0;  // that arc is counted from the other side
+            sw_next = j + wstep < nseg_all ? gload(segw + j + wstep) : 0;
+        } else if (j < nseg_all) {
+            const uint32_t w = gload(seg + j);
+            if (w < n_rows) {
+                const uint64_t s0 = gload(off + w);
+                s = (uint32_t)s0;
+                len = (uint32_t)(gload(off + w + 1) - s0);
+            }
+            if (below && len >= below) len = 0;
         }
         const uint32_t incl = wave_incl_scan(len, lane);
         const uint32_t total = uniform(gather(incl, 63));
@@ -565,30 +575,30 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
         {
             const uint32_t sg = first_segment(0, myend);
             if (RB > total)
-                index_walk<UNROLL, false>(0, total, mybase, myend, sg, lane, idx);
+                index_walk<UNROLL, false, NARROW>(0, total, mybase, myend, sg, lane, idx);
             else if (one_segment<UNROLL>(0, myend))
-                index_single<UNROLL>((uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg), lane, idx);
+                index_single<UNROLL, NARROW>((uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg), lane, idx);
             else
-                index_walk<UNROLL, true>(0, total, mybase, myend, sg, lane, idx);
+                index_walk<UNROLL, true, NARROW>(0, total, mybase, myend, sg, lane, idx);
         }
-        load_round<UNROLL>(col, idx, xa);
+        load_round<UNROLL, NARROW>(col, idx, xa);
         uint32_t B = 0;
         for (; B + 3 * RB <= total; B += 2 * RB) {
             // rounds B + RB and B + 2 * RB are full
             uint32_t sg = first_segment(B + RB, myend);
             if (one_segment<UNROLL>(B + RB, myend))
-                index_single<UNROLL>((uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + RB, lane, idx);
+                index_single<UNROLL, NARROW>((uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + RB, lane, idx);
             else
-                index_walk<UNROLL, true>(B + RB, total, mybase, myend, sg, lane, idx);
-            load_round<UNROLL>(col, idx, xb);
+                index_walk<UNROLL, true, NARROW>(B + RB, total, mybase, myend, sg, lane, idx);
+            load_round<UNROLL, NARROW>(col, idx, xb);
             probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
             sg = first_segment(B + 2 * RB, myend);
             if (one_segment<UNROLL>(B + 2 * RB, myend))
-                index_single<UNROLL>((uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + 2 * RB, lane,
+                index_single<UNROLL, NARROW>((uint32_t)__builtin_amdgcn_readlane((int)mybase, (int)sg) + B + 2 * RB, lane,
                                      idx);
             else
-                index_walk<UNROLL, true>(B + 2 * RB, total, mybase, myend, sg, lane, idx);
-            load_round<UNROLL>(col, idx, xa);
+                index_walk<UNROLL, true, NARROW>(B + 2 * RB, total, mybase, myend, sg, lane, idx);
+            load_round<UNROLL, NARROW>(col, idx, xa);
             probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
         }
         // Tail: round B is in xa; rounds B + RB and B + 2 * RB may follow
@@ -596,13 +606,13 @@ __device__ __forceinline__ uint32_t walk(const uint64_t *__restrict__ off, const
         if (B + RB >= total) {
             probe_round<UNROLL, false>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
         } else {
-            index_walk<UNROLL, false>(B + RB, total, mybase, myend, first_segment(B + RB, myend), lane, idx);
-            load_round<UNROLL>(col, idx, xb);
+            index_walk<UNROLL, false, NARROW>(B + RB, total, mybase, myend, first_segment(B + RB, myend), lane, idx);
+            load_round<UNROLL, NARROW>(col, idx, xb);
             probe_round<UNROLL, true>(xa, B, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
             if (B + 2 * RB < total) {
-                index_walk<UNROLL, false>(B + 2 * RB, total, mybase, myend, first_segment(B + 2 * RB, myend), lane,
+                index_walk<UNROLL, false, NARROW>(B + 2 * RB, total, mybase, myend, first_segment(B + 2 * RB, myend), lane,
                                           idx);
-                load_round<UNROLL>(col, idx, xa);
+                load_round<UNROLL, NARROW>(col, idx, xa);
                 probe_round<UNROLL, true>(xb, B + RB, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
                 probe_round<UNROLL, false>(xa, B + 2 * RB, total, tab, qmask, qshift, lane, lane_hits, s_base, qn);
             } else {
@@ -727,7 +737,8 @@ __device__ __forceinline__ uint32_t tiny_walk(const GraphDev &G, bool mine, bool
 }
 
 // ------------------------------------------------------------------ persistent count kernel
-template <int UNROLL>
+// NARROW: every graph of the launch has m <= 2^30 arcs.
+template <int UNROLL, bool NARROW>
 __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(GraphSet gs, const unsigned long long *__restrict__ queue,
                                                        uint32_t *__restrict__ ctl,
                                                        unsigned long long *__restrict__ counts, uint32_t scale,
@@ -776,7 +787,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(Grap
                 __syncthreads();
                 table_mark(s_tab, s_ovf, tl, tid, kWaves * 64);
                 __syncthreads();
-                hits += walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.seg, sg.n, sg.below, wv * 64, kWaves * 64, s_tab, tl,
+                hits += walk<UNROLL, NARROW>(G.off, G.col, G.n_rows, sg.ids, sg.seg, sg.n, sg.below, wv * 64, kWaves * 64, s_tab, tl,
                                      s_base[wv], s_end[wv], lane);
                 __syncthreads();
             }
@@ -828,7 +839,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(Grap
                 table_insert(tab, ovf, tl, G.col, rb, K, lane, 64);
                 table_mark(tab, ovf, tl, lane, 64);
                 const unsigned long long hits =
-                    walk<UNROLL>(G.off, G.col, G.n_rows, sg.ids, sg.seg, sg.n, sg.below, 0, 64, tab, tl,
+                    walk<UNROLL, NARROW>(G.off, G.col, G.n_rows, sg.ids, sg.seg, sg.n, sg.below, 0, 64, tab, tl,
                                  s_base[wv], s_end[wv], lane);
                 if (g != cur_g) {
                     if (cur_g != kEmpty && acc != 0 && lane == 0) atomicAdd(counts + cur_g, acc);
@@ -914,12 +925,12 @@ lstc_status count_launch(const HostGraph *hosts, uint32_t G, const GraphDev *d_t
     }
     if (total) {
         const uint32_t grid = (uint32_t)cus * tr.grid_per_cu;
-        if (tr.unroll == 8)
-            hipLaunchKernelGGL(k_count<8>, dim3(grid), dim3(kWaves * 64), 0, st, gs, queue, ctl, d_counts,
-                               tr.table_scale, tr.small_group);
-        else
-            hipLaunchKernelGGL(k_count<4>, dim3(grid), dim3(kWaves * 64), 0, st, gs, queue, ctl, d_counts,
-                               tr.table_scale, tr.small_group);
+        bool narrow = true;
+        for (uint32_t g = 0; g < G; ++g) narrow = narrow && hosts[g].m <= (1ull << 30);
+        auto kern = tr.unroll == 8 ? (narrow ? k_count<8, true> : k_count<8, false>)
+                                   : (narrow ? k_count<4, true> : k_count<4, false>);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, st, gs, queue, ctl, d_counts, tr.table_scale,
+                           tr.small_group);
         LSTC_LAUNCHED("k_count");
     }
     return LSTC_OK;

@@ -15,6 +15,14 @@ __device__ __forceinline__ T gload(const T *p) {
     return *(const LSTC_GLOBAL_AS T *)p;
 }
 
+// Loads the T at byte offset `byte_off` (32 bits) from a wave-uniform base:
+// a scalar base plus one VGPR offset, where a 64-bit address takes two VGPRs
+// and a 64-bit add.
+template <class T>
+__device__ __forceinline__ T gload_off32(const T *base, uint32_t byte_off) {
+    return *(const LSTC_GLOBAL_AS T *)((const LSTC_GLOBAL_AS char *)base + byte_off);
+}
+
 // Loads through the scalar data cache (constant address space).  Only for
 // addresses that are uniform across the wave and memory that nothing writes
 // while the kernel runs (written by earlier kernels or copies is fine).
