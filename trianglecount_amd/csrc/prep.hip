@@ -138,7 +138,9 @@ __global__ void k_in_flags(const uint64_t *__restrict__ uk, const uint64_t *__re
         const uint32_t s = (uint32_t)(k >> b), d = (uint32_t)(k & mask);
         const uint32_t ds = deg[s], dd = deg[d];
         const bool out_arc = ds < dd || (ds == dd && d < s);
-        f[i] = (!out_arc && dout[d] <= dout[s]) ? 1u : 0u;
+        uint32_t fl = 0;
+        if (!out_arc) fl = dout[d] <= dout[s] ? 1u : 0u;  // out-degrees only read for in-arcs
+        f[i] = fl;
     }
 }
 

@@ -68,30 +68,49 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const TI *__restri
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
 }
 
+// Downsweep of one tile: loads and stores are striped over the threads
+// (consecutive threads, consecutive items: coalesced), and the items pass
+// through LDS so that each thread scans 16 consecutive items serially.  LDS
+// rows are padded by one item per 16 so the per-thread reads do not collide
+// on banks.  `in` may alias `out`: every read of the tile precedes the first
+// barrier and every write follows it.
 template <class TI>
-__global__ __launch_bounds__(kScanThreads) void k_scan_down(const TI *__restrict__ in, uint64_t *out,
-                                                            uint64_t n,
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(const TI *in, uint64_t *out, uint64_t n,
                                                             const uint64_t *__restrict__ tile_off,
                                                             uint64_t *__restrict__ d_total) {
     __shared__ uint64_t s_w[kScanThreads / 64];
-    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
-    TI x[kScanItems];
-    uint64_t v = 0;
+    __shared__ uint64_t s_x[kScanTile + kScanTile / 16];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t tb = (uint64_t)blockIdx.x * kScanTile;
 #pragma unroll
     for (int r = 0; r < kScanItems; ++r) {
-        uint64_t i = base + r;
-        x[r] = i < n ? in[i] : (TI)0;
-        v += (uint64_t)x[r];
+        const uint32_t e = r * kScanThreads + tid;
+        s_x[e + e / 16] = tb + e < n ? (uint64_t)in[tb + e] : 0;
+    }
+    __syncthreads();
+    uint64_t x[kScanItems], v = 0;
+#pragma unroll
+    for (int r = 0; r < kScanItems; ++r) {
+        const uint32_t e = tid * kScanItems + r;
+        x[r] = s_x[e + e / 16];
+        v += x[r];
     }
     uint64_t tot;
     uint64_t run = block_excl_scan<TI>(v, s_w, &tot) + (tile_off ? tile_off[blockIdx.x] : 0);
+    // block_excl_scan's barriers order every thread's reads of s_x before these writes
 #pragma unroll
     for (int r = 0; r < kScanItems; ++r) {
-        uint64_t i = base + r;
-        if (i < n) out[i] = run;
-        run += (uint64_t)x[r];
+        const uint32_t e = tid * kScanItems + r;
+        s_x[e + e / 16] = run;
+        run += x[r];
     }
-    if (d_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) *d_total = run;
+    if (d_total && blockIdx.x == gridDim.x - 1 && tid == kScanThreads - 1) *d_total = run;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kScanItems; ++r) {
+        const uint32_t e = r * kScanThreads + tid;
+        if (tb + e < n) out[tb + e] = s_x[e + e / 16];
+    }
 }
 
 static size_t scan_ws_rec(uint64_t n) {
