@@ -76,31 +76,59 @@ def timed_counts(T, g, steps, warmup, stream, ws, split=True):
     return int(out[0].item()), ms, wall
 
 
-def cpu_baseline(g, target_s=15.0):
-    """Oracle merge counter (OpenMP) on a strided row sample of the same oriented CSR.
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    The sample is every k-th owned row, k chosen after a short calibration so the
-    timed run takes about target_s seconds; the rate is sampled oriented edges / s.
+
+def cpu_baseline(g, run_s=2.0, runs=5):
+    """Oracle merge counter (OpenMP) on strided row samples of the same oriented CSR.
+
+    All cores: every k-th row, k calibrated so one run takes about run_s seconds,
+    median of `runs` runs (OMP_PROC_BIND=close).  One thread: a 16x sparser
+    sample the same way, median of 3 runs.  Rates are sampled oriented edges / s.
     """
+    import statistics
     import numpy as np
+    os.environ.setdefault("OMP_PROC_BIND", "close")  # before the OpenMP runtime starts
     from oracle import oracle as O
     off, col = g.to_numpy()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     n = g.n
-    stride = max(1, n // 2000)
-    t0 = time.perf_counter()
-    O.count_rows(off, col, np.arange(0, n, stride, dtype=np.uint32), threads)
-    dt = time.perf_counter() - t0
-    frac = min(1.0, (target_s / max(dt, 1e-4)) / stride)
-    stride = max(1, int(round(1.0 / frac)))
-    rows = np.arange(0, n, stride, dtype=np.uint32)
-    edges = int((off[rows.astype(np.int64) + 1] - off[rows.astype(np.int64)]).sum())
-    t0 = time.perf_counter()
-    O.count_rows(off, col, rows, threads)
-    dt = time.perf_counter() - t0
-    return {"value": round(edges / dt / 1e9, 5), "unit": "GTEPS", "cores": threads, "kind": "port",
-            "sample": f"every {stride}th of {n} rows of the same oriented CSR ({len(rows)} rows, "
-                      f"{edges} oriented edges) counted by the oracle merge counter in {dt:.1f} s"}
+
+    def rate(nthreads, target_s, reps, sparsity):
+        stride = max(1, n // 2000) * sparsity
+        t0 = time.perf_counter()
+        O.count_rows(off, col, np.arange(0, n, stride, dtype=np.uint32), nthreads)
+        dt = time.perf_counter() - t0
+        stride = max(1, int(round(stride * max(dt, 1e-4) / target_s)))
+        rows = np.arange(0, n, stride, dtype=np.uint32)
+        edges = int((off[rows.astype(np.int64) + 1] - off[rows.astype(np.int64)]).sum())
+        O.count_rows(off, col, rows, nthreads)  # warm-up
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            O.count_rows(off, col, rows, nthreads)
+            times.append(time.perf_counter() - t0)
+        med = statistics.median(times)
+        return edges / med / 1e9, stride, len(rows), edges, med
+
+    v_all, st_all, nr_all, e_all, t_all = rate(threads, run_s, runs, 1)
+    v_one, st_one, nr_one, e_one, t_one = rate(1, run_s, 3, 16)
+    return {"value": round(v_all, 5), "unit": "GTEPS", "cores": threads, "kind": "port",
+            "sample": f"every {st_all}th of {n} rows of the same oriented CSR ({nr_all} rows, {e_all} oriented "
+                      f"edges) counted by the oracle merge counter, median of {runs} runs {t_all:.2f} s",
+            "single_thread_value": round(v_one, 6),
+            "single_thread_sample": f"every {st_one}th row ({nr_one} rows, {e_one} oriented edges), median of 3 "
+                                    f"runs {t_one:.2f} s",
+            "host": {"nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+                     "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}}
 
 
 def _replay_ms(T, graphs, steps):
@@ -405,7 +433,8 @@ def main():
             raise SystemExit("served counts differ from the CPU oracle")
     cpu = None if args.no_cpu else cpu_baseline(g)
     if cpu:
-        log(f"cpu baseline {cpu['value']} GTEPS on {cpu['cores']} threads")
+        log(f"cpu baseline {cpu['value']} GTEPS on {cpu['cores']} threads, {cpu['single_thread_value']} GTEPS on one "
+            f"({cpu['host']['cpu_model']}, nproc {cpu['host']['nproc']})")
     res = {
         "metric": "exact-count GTEPS at RMAT s23 ef16; batched vs per-graph launch; p99 latency",
         "value": round(gteps, 4), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
