@@ -104,6 +104,7 @@ def cpu_baseline(g, run_s=2.0, runs=5):
 
     def rate(nthreads, target_s, reps, sparsity):
         stride = max(1, n // 2000) * sparsity
+        O.count_rows(off, col, np.arange(0, n, stride * 8, dtype=np.uint32), nthreads)  # thread pool start-up
         t0 = time.perf_counter()
         O.count_rows(off, col, np.arange(0, n, stride, dtype=np.uint32), nthreads)
         dt = time.perf_counter() - t0
