@@ -115,7 +115,7 @@ lstc_status lstc_halo_extract(const uint64_t *d_sym_off, const uint32_t *d_sym_c
 /* ------------------------------------------------------------------ (c) counting
  * Tuning knobs (picked by lstc_autotune; zero-initialised = defaults).       */
 typedef struct lstc_tune {
-    uint32_t table_scale;  /* LDS hash capacity >= table_scale * keys (2..16), 0 = 8    */
+    uint32_t table_scale;  /* LDS hash capacity >= table_scale * keys (2..16), 0 = 6    */
     uint32_t small_max;    /* switch point: rows with d+ <= small_max get a wave-private
                               1024-slot table, larger rows a workgroup-shared 4096-slot
                               one (64..1024/table_scale), 0 = 256 (clamped)             */

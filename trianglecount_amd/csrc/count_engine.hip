@@ -83,8 +83,9 @@ __device__ __forceinline__ Item decode(unsigned long long it) {
 
 TuneR resolve_tune(const lstc_tune *t) {
     // defaults: the fastest point of a (table_scale, small_max, grid_per_cu, unroll)
-    // sweep at RMAT scales 21 and 23 on MI355X
-    TuneR r{8, 256, 16, 0, 4, kTinyMax};
+    // sweep at RMAT scales 16, 21 and 23 on MI355X (table_scale 6: rows of up to
+    // 170 keys keep a wave-private table)
+    TuneR r{6, 256, 16, 0, 4, kTinyMax};
     if (t) {
         if (t->tiny_max) r.tiny_max = t->tiny_max == 1 ? 0u : t->tiny_max;  // 1 = off
         if (t->table_scale) r.table_scale = t->table_scale;
