@@ -19,7 +19,8 @@
 // its in-list each), binned by the hashed row's d+ into one queue, largest bin
 // first; rows with d+ <= tiny_max go to the tiny region
 // instead.  Items of rows above small_max are "big" tickets (one item each);
-// smaller ones form "small" tickets of small_group items.
+// smaller ones form "small" tickets of small_group items (by default chosen by
+// the planner from the small items per workgroup).  k_count reads it from ctl.
 //
 // Count (k_count, one persistent launch): each workgroup of kWaves waves takes
 // tickets from an atomic head.
@@ -67,7 +68,7 @@ static constexpr uint32_t kTinyMax = 8;               // largest d+ of the lane-
 
 // ctl (uint32) layout
 enum { C_CNT = 0, C_HEAD = 12, C_NBIG = 13, C_NSMALL = 14, C_NTICK = 15, C_NTINY = 16, C_TINY0 = 17, C_NSMALLT = 18,
-       C_WORDS = 19 };
+       C_SGROUP = 19, C_WORDS = 20 };
 
 // work item: row (bits 0-31) | chunk (32-51) | kind (52) | graph (53-63).
 // A row's in-degree is below 2^32 (m < 2^32), so its chunk index fits 20 bits.
@@ -85,7 +86,7 @@ TuneR resolve_tune(const lstc_tune *t) {
     // defaults: the fastest point of a (table_scale, small_max, grid_per_cu, unroll)
     // sweep at RMAT scales 16, 21 and 23 on MI355X (table_scale 6: rows of up to
     // 170 keys keep a wave-private table)
-    TuneR r{6, 256, 16, 0, 4, kTinyMax};
+    TuneR r{6, 256, 0, 0, 4, kTinyMax};  // small_group 0: chosen by the planner
     if (t) {
         if (t->tiny_max) r.tiny_max = t->tiny_max == 1 ? 0u : t->tiny_max;  // 1 = off
         if (t->table_scale) r.table_scale = t->table_scale;
@@ -97,7 +98,7 @@ TuneR resolve_tune(const lstc_tune *t) {
     r.table_scale = hmax(2u, hmin(16u, r.table_scale));
     // a small row must fit its slice at the chosen load factor
     r.small_max = hmax(64u, hmin(kSlice / r.table_scale, r.small_max));
-    r.small_group = hmax(4u, hmin(256u, r.small_group));
+    if (r.small_group) r.small_group = hmax(4u, hmin(256u, r.small_group));
     r.unroll = r.unroll >= 8 ? 8 : 4;
     if (r.grid_per_cu == 0) r.grid_per_cu = 8;  // k_count<4> fits 8 workgroups per CU (LDS and VGPRs)
     r.grid_per_cu = hmax(1u, hmin(16u, r.grid_per_cu));
@@ -271,8 +272,12 @@ __global__ __launch_bounds__(256) void k_plan_rows(GraphSet gs, Bounds bd, uint3
 // One workgroup of 1024: exclusive scan of wcnt in place (kPlanEntries values,
 // 24 per thread), then the ticket counts.  Item totals stay below 2^32
 // (checked on the host against the queue capacity).
+// small_group 0: the planner picks the small items per ticket, the largest
+// power of two up to (small items per workgroup) / 32, within [16, 64]: about
+// 32-64 small tickets per workgroup (measured on RMAT s21 and s23: fewer,
+// larger tickets leave a ragged tail, more of them queue on the ticket head).
 __global__ __launch_bounds__(1024) void k_plan_scan(uint32_t *__restrict__ wcnt, uint32_t *__restrict__ ctl,
-                                                    uint32_t small_group) {
+                                                    uint32_t small_group, uint32_t grid) {
     constexpr uint32_t kPer = kPlanEntries / 1024;
     static_assert(kPlanEntries % 1024 == 0, "plan entries per thread");
     __shared__ uint32_t s_wave[16];
@@ -306,7 +311,14 @@ __global__ __launch_bounds__(1024) void k_plan_scan(uint32_t *__restrict__ wcnt,
         const uint32_t n_small = s_slot[6] - s_slot[3];
         const uint32_t n_tiny = s_slot[7] - s_slot[6];
         for (int b = 0; b < kBins; ++b) ctl[C_CNT + b] = s_slot[kBins - b] - s_slot[kBins - 1 - b];
-        const uint32_t small_t = (n_small + small_group - 1) / small_group;
+        uint32_t sg = small_group;
+        if (sg == 0) {
+            const uint32_t per_wg = n_small / (grid ? grid : 1u) / 32u;
+            sg = 16;
+            while (sg < 64 && 2 * sg <= per_wg) sg *= 2;
+        }
+        ctl[C_SGROUP] = sg;
+        const uint32_t small_t = (n_small + sg - 1) / sg;
         const uint32_t tiny_t = (n_tiny + kWaves * 64 - 1) / (kWaves * 64);
         ctl[C_NBIG] = n_big;
         ctl[C_NSMALL] = n_small;
@@ -742,8 +754,7 @@ __device__ __forceinline__ uint32_t tiny_walk(const GraphDev &G, bool mine, bool
 template <int UNROLL, bool NARROW>
 __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(GraphSet gs, const unsigned long long *__restrict__ queue,
                                                        uint32_t *__restrict__ ctl,
-                                                       unsigned long long *__restrict__ counts, uint32_t scale,
-                                                       uint32_t small_group) {
+                                                       unsigned long long *__restrict__ counts, uint32_t scale) {
     __shared__ __align__(16) uint32_t s_tab[kBigCap];
     __shared__ uint32_t s_ovf[kBigCap / 128];  // overflow bits, one per 4-slot bucket
     __shared__ uint32_t s_base[kWaves][64];
@@ -759,6 +770,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(Grap
     if (lane < 16) s_flag[wv][lane] = 0;
     const uint32_t n_big = sload(ctl + C_NBIG), n_small = sload(ctl + C_NSMALL), n_tickets = sload(ctl + C_NTICK);
     const uint32_t small_end = n_big + sload(ctl + C_NSMALLT);  // first tiny ticket
+    const uint32_t small_group = sload(ctl + C_SGROUP);
     const uint32_t tiny0 = sload(ctl + C_TINY0), tiny_end = tiny0 + sload(ctl + C_NTINY);
     uint32_t cur_g = kEmpty;
     unsigned long long acc = 0;
@@ -919,7 +931,8 @@ lstc_status count_launch(const HostGraph *hosts, uint32_t G, const GraphDev *d_t
     if (total) {
         hipLaunchKernelGGL(k_plan_rows<0>, dim3(kPlanBlocks), dim3(256), 0, st, gs, bd, wcnt, queue);
         LSTC_LAUNCHED("k_plan_rows<0>");
-        hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(1024), 0, st, wcnt, ctl, tr.small_group);
+        hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(1024), 0, st, wcnt, ctl, tr.small_group,
+                           (uint32_t)cus * tr.grid_per_cu);
         LSTC_LAUNCHED("k_plan_scan");
         hipLaunchKernelGGL(k_plan_rows<1>, dim3(kPlanBlocks), dim3(256), 0, st, gs, bd, wcnt, queue);
         LSTC_LAUNCHED("k_plan_rows<1>");
@@ -930,8 +943,7 @@ lstc_status count_launch(const HostGraph *hosts, uint32_t G, const GraphDev *d_t
         for (uint32_t g = 0; g < G; ++g) narrow = narrow && hosts[g].m <= (1ull << 30);
         auto kern = tr.unroll == 8 ? (narrow ? k_count<8, true> : k_count<8, false>)
                                    : (narrow ? k_count<4, true> : k_count<4, false>);
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, st, gs, queue, ctl, d_counts, tr.table_scale,
-                           tr.small_group);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, st, gs, queue, ctl, d_counts, tr.table_scale);
         LSTC_LAUNCHED("k_count");
     }
     return LSTC_OK;

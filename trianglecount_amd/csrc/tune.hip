@@ -91,13 +91,13 @@ extern "C" lstc_status lstc_autotune(const lstc_graph_desc *graphs, uint32_t G, 
     rc = measure(cur, &cur_ms, ref);
     const uint32_t cand_scale[] = {4, 6, 8, 16};
     const uint32_t cand_small[] = {64, 128, 256};
-    const uint32_t cand_group[] = {4, 16, 64};
+    const uint32_t cand_group[] = {4, 16, 32, 64};
     const uint32_t cand_grid[] = {4, 6, 8};
     const uint32_t cand_unroll[] = {4, 8};
     const uint32_t cand_tiny[] = {1, 4, 8};  // 1 = lane-per-arc path off
     struct Knob { uint32_t lstc_tune::*field; const uint32_t *vals; int nvals; };
     const Knob knobs[] = {{&lstc_tune::table_scale, cand_scale, 4}, {&lstc_tune::small_max, cand_small, 3},
-                          {&lstc_tune::small_group, cand_group, 3}, {&lstc_tune::grid_per_cu, cand_grid, 3},
+                          {&lstc_tune::small_group, cand_group, 4}, {&lstc_tune::grid_per_cu, cand_grid, 3},
                           {&lstc_tune::unroll, cand_unroll, 2}, {&lstc_tune::tiny_max, cand_tiny, 3}};
     for (const Knob &k : knobs) {
         if (rc != LSTC_OK) break;
