@@ -65,6 +65,7 @@ static constexpr int kBins = 6;                       // hash-path bins
 static constexpr int kTinyBin = kBins;                // the lane-per-arc rows
 static constexpr int kSlots = kBins + 1;              // queue regions
 static constexpr uint32_t kTinyMax = 8;               // largest d+ of the lane-per-arc path
+static constexpr uint32_t kBigGroup = 2;              // big items per ticket
 
 // ctl (uint32) layout
 enum { C_CNT = 0, C_HEAD = 12, C_NBIG = 13, C_NSMALL = 14, C_NTICK = 15, C_NTINY = 16, C_TINY0 = 17, C_NSMALLT = 18,
@@ -325,7 +326,7 @@ __global__ __launch_bounds__(1024) void k_plan_scan(uint32_t *__restrict__ wcnt,
         ctl[C_NSMALLT] = small_t;
         ctl[C_NTINY] = n_tiny;
         ctl[C_TINY0] = s_slot[6];
-        ctl[C_NTICK] = n_big + small_t + tiny_t;
+        ctl[C_NTICK] = (n_big + kBigGroup - 1) / kBigGroup + small_t + tiny_t;
         ctl[C_HEAD] = 0;
     }
 }
@@ -769,7 +770,8 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(Grap
     const uint32_t tid = threadIdx.x, wv = uniform(tid >> 6), lane = tid & 63;
     if (lane < 16) s_flag[wv][lane] = 0;
     const uint32_t n_big = sload(ctl + C_NBIG), n_small = sload(ctl + C_NSMALL), n_tickets = sload(ctl + C_NTICK);
-    const uint32_t small_end = n_big + sload(ctl + C_NSMALLT);  // first tiny ticket
+    const uint32_t n_bigt = (n_big + kBigGroup - 1) / kBigGroup;  // big tickets
+    const uint32_t small_end = n_bigt + sload(ctl + C_NSMALLT);  // first tiny ticket
     const uint32_t small_group = sload(ctl + C_SGROUP);
     const uint32_t tiny0 = sload(ctl + C_TINY0), tiny_end = tiny0 + sload(ctl + C_NTINY);
     uint32_t cur_g = kEmpty;
@@ -781,9 +783,10 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(Grap
         const uint32_t t = uniform(s_ticket);
         __syncthreads();
         if (t >= n_tickets) break;
-        if (t < n_big) {
-            // big item: the whole workgroup shares one table
-            const Item itm = decode(sload(queue + t));
+        if (t < n_bigt) {
+          // big items (kBigGroup per ticket): the whole workgroup shares one table
+          for (uint32_t bi = t * kBigGroup, be = min(bi + kBigGroup, n_big); bi < be; ++bi) {
+            const Item itm = decode(sload(queue + bi));
             const uint32_t u = itm.row, chunk = itm.chunk, kind = itm.kind, g = itm.g;
             const GraphDev G = graph_at<true>(gs, g);
             const uint64_t rb = sload(G.off + u);
@@ -811,6 +814,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(Grap
             }
             acc += hits;
             if (G.per_row && lane == 0 && hits) atomicAdd(G.per_row + u, hits);
+          }
         } else if (t >= small_end) {
             // tiny rows: 64 per wave, lane-per-arc; the lanes are split by graph
             const uint32_t e0 = tiny0 + (t - small_end) * (kWaves * 64) + wv * 64;
@@ -835,7 +839,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(Grap
             }
         } else {
             // small items: each wave takes every kWaves-th item of the group
-            const uint32_t first = n_big + (t - n_big) * small_group;
+            const uint32_t first = n_big + (t - n_bigt) * small_group;
             const uint32_t last = min(first + small_group, n_big + n_small);
             uint32_t *tab = s_tab + wv * kSlice;
             for (uint32_t q = first + wv; q < last; q += kWaves) {
