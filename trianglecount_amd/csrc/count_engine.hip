@@ -18,7 +18,7 @@
 // (source side; owned rows only) and kind B items (one kInChunk-source chunk of
 // its in-list each), binned by the hashed row's d+ into one queue, largest bin
 // first; rows with d+ <= tiny_max go to the tiny region
-// instead.  Items of rows above small_max are "big" tickets (one item each);
+// instead.  Items of rows above small_max form "big" tickets (kBigGroup items each);
 // smaller ones form "small" tickets of small_group items (by default chosen by
 // the planner from the small items per workgroup).  k_count reads it from ctl.
 //
