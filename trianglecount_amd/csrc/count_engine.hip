@@ -54,7 +54,7 @@ static constexpr uint32_t kBigTile = kBigCap / 2;     // keys per table fill of 
 // hub with a short list and a huge in-list (power-law graphs) becomes many
 // items instead of one straggler.  Larger chunks only when the chunk index
 // would not fit its 20-bit field.
-static constexpr uint32_t kInChunk = 1024;
+static constexpr uint32_t kInChunk = 512;
 static constexpr uint32_t kMaxChunks = (1u << 20) - 1;
 
 __host__ __device__ __forceinline__ uint32_t in_chunk(uint64_t in_degree) {
