@@ -902,12 +902,17 @@ lstc_status count_launch(const HostGraph *hosts, uint32_t G, const GraphDev *d_t
     if (G > kMaxGraphs) return fail(LSTC_ERR_INVALID, "count: at most %u graphs per launch", kMaxGraphs);
     if (!d_counts) return fail(LSTC_ERR_INVALID, "count: NULL counts");
     if (G > 1 && (!d_table || !d_row_base)) return fail(LSTC_ERR_INVALID, "count: batch needs device descriptors");
-    const TuneR tr = resolve_tune(tune);
+    TuneR tr = resolve_tune(tune);
     uint64_t total = 0, total_m = 0;
     for (uint32_t g = 0; g < G; ++g) {
         total += plan_rows(hosts[g]);
         total_m += hosts[g].m;
     }
+    // A launch of few arcs has few items per wave: rows above 128 keys then
+    // share a workgroup table, so their lists spread over four waves (RMAT s16,
+    // 0.9M arcs: 20% faster; no change from 2^23 arcs up, where the default
+    // keeps rows of up to 1024 / table_scale keys wave-private).
+    if ((!tune || !tune->small_max) && total_m < (1ull << 23) && tr.small_max > 128) tr.small_max = 128;
     if (total >= 0xFFFFFFFFull) return fail(LSTC_ERR_INVALID, "count: more than 2^32-1 owned rows in one launch");
     Carver c(ws, ws_bytes);
     if (queue_cap(total, total_m) >= 0xFFFFFFFFull)
