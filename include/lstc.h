@@ -118,7 +118,8 @@ typedef struct lstc_tune {
     uint32_t table_scale;  /* LDS hash capacity >= table_scale * keys (2..16), 0 = 6    */
     uint32_t small_max;    /* switch point: rows with d+ <= small_max get a wave-private
                               1024-slot table, larger rows a workgroup-shared 4096-slot
-                              one (64..1024/table_scale), 0 = 256 (clamped)             */
+                              one (64..1024/table_scale), 0 = 1024/table_scale, or 128
+                              for a launch under 2^23 arcs                              */
     uint32_t small_group;  /* small rows per work ticket (4..256), 0 = planner's choice */
     uint32_t grid_per_cu;  /* persistent workgroups per CU (1..16), 0 = 8              */
     uint32_t unroll;       /* 64-entry chunks in flight per wave (4 or 8), 0 = 4       */
