@@ -2,16 +2,20 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 23]
 
-A step is one exact count (lstc_count) of the resident oriented CSR of a
-synthetic RMAT graph (scale 23, edge factor 16, a/b/c/d .57/.19/.19/.05,
-seed 1), generated, cleaned and oriented on the GPU before timing.
+A step is one exact count (lstc_count_split: plan + count kernel) of the
+resident oriented CSR of a synthetic RMAT graph (scale 23, edge factor 16,
+a/b/c/d .57/.19/.19/.05, seed 1), generated, cleaned and oriented on the GPU
+before timing, with the in-lists and segment words its preprocessing emits.
 GTEPS = undirected edges after cleaning / seconds per count / 1e9.
+
+Besides the headline the line carries the partitioned workload (8 halo parts
+in one batched launch), the mixed-shape batch (batched vs per-graph), the
+serving stream (p50/p99, every count checked) and a serving load sweep.
 Prints one JSON line.
 """
 from __future__ import annotations
 
 import argparse
-import ctypes as C
 import json
 import os
 import sys
@@ -21,6 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+GOLD = os.path.join(ROOT, "tests", "golden")
 
 
 def log(*a):
@@ -36,44 +41,46 @@ def graph_stats(g):
             "sum_out_sq": int((d * d).sum().item()), "sum_in_out": int((indeg * d).sum().item())}
 
 
-def b_alg(st):
-    """Merge-model algorithmic bytes: 8(n+1) + 4m + sum_{(u,v) in E+} [16 + 4(d+(u) + d+(v))]."""
+def merge_model_bytes(st):
+    """Merge model of the reference's kernel: 8(n+1) + 4m + sum_{(u,v) in E+} [16 + 4(d+(u) + d+(v))]."""
     return 8 * (st["n"] + 1) + 4 * st["m"] + 16 * st["m"] + 4 * (st["sum_out_sq"] + st["sum_in_out"])
 
 
 def committed_traffic(scale):
-    """HBM bytes per k_count launch from the committed rocprofv3 PMC summary of this workload, or None.
-
-    The summary (profiles/k_count_rmat_s<scale>_pmc.txt) holds FETCH_SIZE and WRITE_SIZE per
-    dispatch, each scaled by what the counter reads for a known-size copy kernel in the same run.
-    """
+    """HBM-side bytes per count-kernel launch from the committed rocprofv3 PMC summary of this
+    workload (profiles/k_count_rmat_s<scale>_pmc.txt: FETCH_SIZE and WRITE_SIZE per dispatch,
+    each scaled by what the counter reads for a known-size copy kernel in the same run), or None."""
     p = os.path.join(ROOT, "profiles", f"k_count_rmat_s{scale}_pmc.txt")
     try:
         with open(p) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
-                    return int(line.split(":", 1)[1])
+                    return int(line.split(":", 1)[1]), os.path.relpath(p, ROOT)
     except OSError:
         pass
-    return None
+    return None, None
 
 
 def timed_counts(T, g, steps, warmup, stream, ws, split=True):
+    """Step time (plan + count kernel) from hipEvents on the launch stream, and the count
+    kernel's own time from the events the library records around each launch."""
     import torch
     out = torch.zeros(1, dtype=torch.int64, device="cuda")
     for _ in range(warmup):
-        T.count(g, out=out, ws=ws, sync=False, split=split)
+        T.count(g, out=out, ws=ws, sync=False, split=split, stream=stream)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(steps):
-        T.count(g, out=out, ws=ws, sync=False, split=split)
-    e1.record(stream)
-    torch.cuda.synchronize()
+    with T.KernelTimer() as kt:
+        e0.record(stream)
+        for _ in range(steps):
+            T.count(g, out=out, ws=ws, sync=False, split=split, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1) / steps
-    return int(out[0].item()), ms, wall
+    assert kt.launches == steps, (kt.launches, steps)
+    return int(out[0].item()), ms, kt.ms / steps, wall
 
 
 def _cpu_model():
@@ -111,25 +118,32 @@ def cpu_baseline(g, run_s=2.0, runs=5):
         stride = max(1, int(round(stride * max(dt, 1e-4) / target_s)))
         rows = np.arange(0, n, stride, dtype=np.uint32)
         edges = int((off[rows.astype(np.int64) + 1] - off[rows.astype(np.int64)]).sum())
-        O.count_rows(off, col, rows, nthreads)  # warm-up
+        tri = O.count_rows(off, col, rows, nthreads)  # warm-up
         times = []
         for _ in range(reps):
             t0 = time.perf_counter()
             O.count_rows(off, col, rows, nthreads)
             times.append(time.perf_counter() - t0)
         med = statistics.median(times)
-        return edges / med / 1e9, stride, len(rows), edges, med
+        return edges / med / 1e9, stride, rows, edges, med, tri
 
-    v_all, st_all, nr_all, e_all, t_all = rate(threads, run_s, runs, 1)
-    v_one, st_one, nr_one, e_one, t_one = rate(1, run_s, 3, 16)
+    v_all, st_all, rows_all, e_all, t_all, tri_all = rate(threads, run_s, runs, 1)
+    v_one, st_one, rows_one, e_one, t_one, _ = rate(1, run_s, 3, 16)
     return {"value": round(v_all, 5), "unit": "GTEPS", "cores": threads, "kind": "port",
-            "sample": f"every {st_all}th of {n} rows of the same oriented CSR ({nr_all} rows, {e_all} oriented "
+            "sample": f"every {st_all}th of {n} rows of the same oriented CSR ({len(rows_all)} rows, {e_all} oriented "
                       f"edges) counted by the oracle merge counter, median of {runs} runs {t_all:.2f} s",
             "single_thread_value": round(v_one, 6),
-            "single_thread_sample": f"every {st_one}th row ({nr_one} rows, {e_one} oriented edges), median of 3 "
+            "single_thread_sample": f"every {st_one}th row ({len(rows_one)} rows, {e_one} oriented edges), median of 3 "
                                     f"runs {t_one:.2f} s",
             "host": {"nproc": os.cpu_count(), "cpu_model": _cpu_model(),
-                     "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}}
+                     "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}}, (rows_all, tri_all)
+
+
+def sample_matches_gpu(T, g, rows, tri):
+    """The CPU sample's count equals the sum of the GPU's per-row partials g_sum[u] over the same rows."""
+    import numpy as np
+    _, per_row = T.count(g, per_row=True, split=False)
+    return int(per_row[rows.astype(np.int64)].sum(dtype=np.uint64)) == tri
 
 
 def _replay_ms(T, graphs, steps):
@@ -150,16 +164,25 @@ def _replay_ms(T, graphs, steps):
     return ms, per
 
 
+def _gold(name):
+    try:
+        return json.load(open(os.path.join(GOLD, name)))
+    except OSError:
+        return None
+
+
 def partitioned(T, g, k, seed, steps, ws):
     """RMAT halo parts (k-way seeded random assignment) counted in one persistent launch.
 
     Each part carries owned-source in-lists (split counting); the same parts
-    without them (plain CSR counting) are timed beside.
+    without them (plain CSR counting) are timed beside.  Part counts are checked
+    against the oracle's committed counts when they exist for this workload.
     """
     import torch
     part = T.partition_random(g.n, k, seed=seed)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    parts = [T.halo_extract(g, part, p, mode=0, ws=ws) for p in range(k)]
+    parts = T.halo_extract_parts(g, part, k, mode=0, ws=ws)
     torch.cuda.synchronize()
     t_halo = time.perf_counter() - t0
     t0 = time.perf_counter()
@@ -173,43 +196,31 @@ def partitioned(T, g, k, seed, steps, ws):
     ms_plain, per_plain = _replay_ms(T, parts, max(1, steps // 2))
     if per_plain != per:
         raise SystemExit(f"split part counts {per} differ from plain {per_plain}")
+    gold = _gold(f"halo_s{g.meta.get('scale')}_k{k}.json")
+    checked = None
+    if gold and gold.get("seed") == g.meta.get("seed") and gold.get("partition_seed") == seed:
+        checked = per == [q["triangles"] for q in gold["parts"]]
+        if not checked:
+            raise SystemExit(f"part counts {per} differ from the oracle's {[q['triangles'] for q in gold['parts']]}")
     return {"k": k, "mode": "P u N(P) (reference un-pruned halo)", "ms": round(ms, 3),
             "gteps": round(g.m / (ms / 1e3) / 1e9, 4), "sum_parts": sum(per),
-            "plain_csr_ms": round(ms_plain, 3), "halo_extract_s": round(t_halo, 3),
-            "in_lists_s": round(t_inl, 3), "part_rows": [p.n for p in parts],
+            "per_part_oracle_checked": checked,
+            "plain_csr_ms": round(ms_plain, 3), "halo_extract_s": round(t_halo, 4),
+            "in_lists_s": round(t_inl, 4), "part_rows": [p.n for p in parts],
             "part_owned": [p.owned for p in parts], "part_arcs": [p.m for p in parts]}
 
 
-def mixed_workload(T, seed, ws):
-    """Config: 2000x2000 grid + p=0.3 diagonals, Chung-Lu (1M, 2.1, 50, 5e4), 64 ER (n log-uniform)."""
-    import numpy as np
-    import torch
-    graphs, names = [], []
-    n, s, d = T.generate_grid(2000, 2000, 0.3, seed=seed)
-    graphs.append(T.clean_orient(n, s, d, ws=ws, in_lists=True)); names.append("grid2000")
-    del s, d
-    n, s, d = T.generate_chung_lu(1_000_000, 2.1, 50.0, 5e4, seed=seed, ws=ws)
-    graphs.append(T.clean_orient(n, s, d, ws=ws, in_lists=True)); names.append("chung_lu_1M")
-    del s, d
-    rng = np.random.default_rng(seed)
-    for i in range(64):
-        nn = int(round(10 ** rng.uniform(3, 5)))
-        n, s, d = T.generate_er(nn, 20.0, seed=seed * 1000 + i)
-        graphs.append(T.clean_orient(n, s, d, ws=ws, in_lists=True)); names.append(f"er{i}_{nn}")
-    torch.cuda.synchronize()
-    return graphs, names
-
-
-def _per_graph_calls(T, graphs, ws):
+def _per_graph_calls(T, graphs, ws, tune=None):
     """Pre-bound C calls of the per-graph path: graph i counts into res[i] (lstc_count_split
     when it carries in-lists, else lstc_count), sharing one workspace sized for the largest."""
     import ctypes as C
     import torch
     L = T.load()
     need = 0
+    tp = C.byref(tune) if tune is not None else None
     for g in graphs:
         nb = C.c_size_t()
-        T.check(L.lstc_count_workspace(g.n, g.owned, g.m, None, C.byref(nb)), "lstc_count_workspace")
+        T.check(L.lstc_count_workspace(g.n, g.owned, g.m, tp, C.byref(nb)), "lstc_count_workspace")
         need = max(need, nb.value)
     wp, wb = ws.get(need)
     res = torch.zeros(len(graphs), dtype=torch.int64, device="cuda")
@@ -221,39 +232,47 @@ def _per_graph_calls(T, graphs, ws):
             seg = [C.c_void_p(t.data_ptr()) if t is not None else None for t in (g.col_seg, g.in_seg)]
             calls.append((L.lstc_count_split, (C.c_void_p(g.off.data_ptr()), C.c_void_p(g.col.data_ptr()),
                                                C.c_void_p(g.in_off.data_ptr()), C.c_void_p(g.in_col.data_ptr()),
-                                               seg[0], seg[1], g.n, g.owned, g.m, out, None, wp, wb, sh)))
+                                               seg[0], seg[1], g.n, g.owned, g.m, out, tp, wp, wb, sh)))
         else:
             calls.append((L.lstc_count, (C.c_void_p(g.off.data_ptr()), C.c_void_p(g.col.data_ptr()), g.n, g.owned,
-                                         g.m, out, None, None, wp, wb, sh)))
+                                         g.m, out, None, tp, wp, wb, sh)))
     return calls, res
 
 
-def batched_vs_per_graph(T, graphs, reps, ws):
-    """Whole mixed workload, wall clock: the per-graph path (the reference's loop: per graph one
-    count launch, then a synchronising copy of its count to the host; default tune, calls
-    pre-bound so no Python-side allocation is timed) vs one persistent batched launch with the
-    tune the autotuner picks for this batch, replayed from a hipGraph, counts copied back.
-    Also reports the per-graph path's summed device time (no host overhead at all)."""
+def _per_graph(T, graphs, ws, reps, tune=None):
+    """Wall time of the per-graph loop (one launch, then a synchronising copy of its count, per
+    graph) and its summed device time; returns (wall_ms, device_ms, counts)."""
     import torch
-    calls, res = _per_graph_calls(T, graphs, ws)
+    calls, res = _per_graph_calls(T, graphs, ws, tune)
     for fn, args in calls[:3]:
         T.check(fn(*args), "count")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
-        per_graph = []
+        counts = []
         for i, (fn, args) in enumerate(calls):
             T.check(fn(*args), "count")
-            per_graph.append(int(res[i].item()))  # synchronising 8-byte copy, as cudaMemcpy in the loop
-    t_plain = (time.perf_counter() - t0) / reps
+            counts.append(int(res[i].item()))  # synchronising 8-byte copy, as cudaMemcpy in the loop
+    wall = (time.perf_counter() - t0) / reps
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in calls]
     for (fn, args), (e0, e1) in zip(calls, ev):
         e0.record()
         T.check(fn(*args), "count")
         e1.record()
     torch.cuda.synchronize()
-    dev_ms = sum(e0.elapsed_time(e1) for e0, e1 in ev)
+    return wall * 1e3, sum(e0.elapsed_time(e1) for e0, e1 in ev), counts
+
+
+def batched_vs_per_graph(T, graphs, reps, ws):
+    """Whole mixed workload, wall clock: the per-graph path (the reference's loop: per graph one
+    count launch, then a synchronising copy of its count to the host; calls pre-bound so no
+    Python-side allocation is timed) vs one persistent batched launch replayed from a hipGraph,
+    counts copied back.  Both with the tune the autotuner picks for the batch, and the per-graph
+    path also with the defaults.  Counts are checked against the oracle's committed counts."""
+    import torch
     tune, tune_ms, _cached = T.autotune(graphs, reps=3)
+    pg_ms, pg_dev_ms, per_graph = _per_graph(T, graphs, ws, reps, tune=tune)
+    pg0_ms, pg0_dev_ms, per_graph0 = _per_graph(T, graphs, ws, reps, tune=None)
     plan = T.BatchPlan(graphs, tune=tune)
     plan.launch_graph()
     torch.cuda.synchronize()
@@ -269,83 +288,112 @@ def batched_vs_per_graph(T, graphs, reps, ws):
     torch.cuda.synchronize()
     batch_dev_ms = e0.elapsed_time(e1)
     plan.close()
+    gold = _gold("mixed_counts.json")
+    checked = None
+    if gold:
+        checked = batched == [q["triangles"] for q in gold["graphs"]]
     return {"graphs": len(graphs), "edges": int(sum(g.m for g in graphs)),
-            "per_graph_ms": round(t_plain * 1e3, 3), "batched_ms": round(t_batch * 1e3, 3),
-            "speedup": round(t_plain / t_batch, 3), "identical": per_graph == batched,
-            "per_graph_device_ms": round(dev_ms, 3), "batched_device_ms": round(batch_dev_ms, 3),
-            "device_speedup": round(dev_ms / batch_dev_ms, 3),
-            "triangles": int(sum(batched)), "tune": tune.as_dict()}
+            "tune": tune.as_dict(),
+            "per_graph_ms": round(pg_ms, 3), "batched_ms": round(t_batch * 1e3, 3),
+            "speedup": round(pg_ms / (t_batch * 1e3), 3),
+            "per_graph_device_ms": round(pg_dev_ms, 3), "batched_device_ms": round(batch_dev_ms, 3),
+            "device_speedup": round(pg_dev_ms / batch_dev_ms, 3),
+            "per_graph_default_tune_ms": round(pg0_ms, 3),
+            "speedup_vs_default_tune_per_graph": round(pg0_ms / (t_batch * 1e3), 3),
+            "identical": per_graph == batched == per_graph0, "oracle_checked": checked,
+            "triangles": int(sum(batched))}
 
 
-def serving(T, n_req, rate, budget_gb, seed, verify):
-    """Serving loop: a Poisson stream of count requests through lstc_server.
-
-    Request i: raw edge count E_i log-uniform in [1e3, 5e6]; 70% RMAT (a/b/c/d
-    .57/.19/.19/.05, scale = max(4, round(log2(E_i / 16)))), 30% Erdos-Renyi
-    (n = E_i / 10, mean degree 20), each with its own seed.  Arrivals are a
-    Poisson process of `rate` requests/s; each request is submitted at its
-    arrival time and its latency runs from submission to the count being
-    available on the host.  The server admits requests against an HBM budget of
-    `budget_gb`, generates and cleans them on one stream and counts ready ones in
-    persistent batches on another.
-
-    verify: "sample" checks every request with E_i <= 2e5 plus every 50th larger
-    one against the CPU oracle (generate, clean, orient, merge count); "all"
-    checks all of them; "none" skips the check.
-    """
-    import concurrent.futures as cf
+def _serve(T, reqs, budget_gb, t_scale=1.0):
+    """Submits ``reqs`` at their arrival times (scaled by t_scale; 0 = all at once) and waits for all."""
     import numpy as np
-    rng = np.random.default_rng(seed)
-    E = np.exp(rng.uniform(np.log(1e3), np.log(5e6), n_req)).astype(np.int64)
-    is_rmat = rng.uniform(size=n_req) < 0.7
-    arrive = np.cumsum(rng.exponential(1.0 / rate, n_req))
-    specs = []
-    for i in range(n_req):
-        e, s = int(E[i]), seed * 1_000_003 + i
-        if is_rmat[i]:
-            specs.append(("rmat", max(4, int(round(np.log2(e / 16)))), e, s))
-        else:
-            specs.append(("er", max(16, e // 10), e, s))
     srv = T.Server(int(budget_gb * (1 << 30)))
     ids = []
     t0 = time.perf_counter()
-    for i, (kind, a, e, s) in enumerate(specs):
-        while time.perf_counter() - t0 < arrive[i]:
+    for r in reqs:
+        at = r.arrive_s * t_scale
+        while time.perf_counter() - t0 < at:
             pass
-        ids.append(srv.submit_rmat(a, e, s) if kind == "rmat" else srv.submit_er(a, e, s))
-    res = [srv.wait(r) for r in ids]
+        ids.append(srv.submit_rmat(r.a, r.m_raw, r.seed) if r.kind == "rmat" else srv.submit_er(r.a, r.m_raw, r.seed))
+    res = [srv.wait(i) for i in ids]
     wall = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
-    lat = np.array([r["latency_s"] for r in res])
-    edges = int(sum(r["edges"] for r in res))
-    out = {"requests": n_req, "rate_per_s": rate, "hbm_budget_gb": budget_gb,
+    lat = np.array([x["latency_s"] for x in res])
+    edges = int(sum(x["edges"] for x in res))
+    return res, {"wall_s": round(wall, 3), "edges": edges, "edges_per_s": round(edges / wall, 1),
+                 "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                 "mean_ms": round(float(lat.mean()) * 1e3, 3), "max_ms": round(float(lat.max()) * 1e3, 3),
+                 "batches": int(st["batches"]), "graphs_per_launch": round(len(reqs) / max(1, st["batches"]), 2),
+                 "max_batch_graphs": int(st["max_batch_graphs"]), "rejected": int(st["rejected"])}
+
+
+def serving(T, n_req, rate, budget_gb, seed, verify):
+    """Serving loop: the seeded Poisson stream of count requests through lstc_server.
+
+    Request i: raw edge count E_i log-uniform in [1e3, 5e6]; 70% RMAT, 30% ER
+    (trianglecount_amd.workloads.serving_stream).  Latency runs from submission
+    to the count being available on the host.
+
+    verify: "all" checks every count and edge total, against the oracle's committed
+    counts (tests/golden/serving_counts.json) when they cover this stream, else by
+    running the CPU oracle on every request; "none" skips the check.
+    """
+    from trianglecount_amd.workloads import serving_stream
+    reqs = serving_stream(n_req, rate, seed)
+    res, out = _serve(T, reqs, budget_gb)
+    out = {"requests": n_req, "rate_per_s": rate, "offered_edges_per_s": None, "hbm_budget_gb": budget_gb,
            "mix": "70% RMAT / 30% ER, raw edges log-uniform [1e3, 5e6]",
-           "raw_edges": int(E.sum()), "edges": edges, "wall_s": round(wall, 3),
-           "edges_per_s": round(edges / wall, 1), "gteps": round(edges / wall / 1e9, 4),
-           "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
-           "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
-           "mean_ms": round(float(lat.mean()) * 1e3, 3), "max_ms": round(float(lat.max()) * 1e3, 3),
-           "batches": int(st["batches"]), "max_batch_graphs": int(st["max_batch_graphs"]),
-           "rejected": int(st["rejected"])}
-    if verify != "none":
-        from oracle import oracle as O
-        pick = [i for i in range(n_req) if verify == "all" or E[i] <= 200_000 or i % 50 == 0]
+           "raw_edges": int(sum(r.m_raw for r in reqs)), **out}
+    out["offered_edges_per_s"] = round(out["edges"] / reqs[-1].arrive_s, 1)
+    out["gteps"] = round(out["edges_per_s"] / 1e9, 4)
+    if verify == "none":
+        return out
+    gold = _gold("serving_counts.json")
+    if gold and (gold["requests"], gold["rate_per_s"], gold["seed"]) == (n_req, rate, seed):
+        bad = sum((r["count"], r["edges"]) != (gold["triangles"][i], gold["edges"][i]) for i, r in enumerate(res))
+        out["verified"] = n_req
+        out["verified_against"] = "tests/golden/serving_counts.json (oracle counts of every request)"
+        out["mismatches"] = int(bad)
+        return out
+    import concurrent.futures as cf
+    from oracle import oracle as O
 
-        def check(i):
-            kind, a, e, s = specs[i]
-            n, src, dst = O.gen_rmat(a, 16, seed=s, m_raw=e) if kind == "rmat" else O.gen_er(a, 20.0, seed=s, m_raw=e)
-            _, g = O.clean_orient(n, src, dst)
-            return O.count(g, threads=1) == res[i]["count"] and g.m == res[i]["edges"]
+    def check(i):
+        r = reqs[i]
+        n, src, dst = O.gen_rmat(r.a, 16, seed=r.seed, m_raw=r.m_raw) if r.kind == "rmat" else \
+            O.gen_er(r.a, 20.0, seed=r.seed, m_raw=r.m_raw)
+        _, g = O.clean_orient(n, src, dst)
+        return O.count(g, threads=1) == res[i]["count"] and g.m == res[i]["edges"]
 
-        threads = min(16, os.cpu_count() or 1)
-        t1 = time.perf_counter()
-        with cf.ThreadPoolExecutor(threads) as ex:
-            ok = list(ex.map(check, pick))
-        out["cpu_verified"] = len(pick)
-        out["cpu_mismatches"] = int(len(ok) - sum(ok))
-        out["cpu_verify_s"] = round(time.perf_counter() - t1, 1)
+    t1 = time.perf_counter()
+    with cf.ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
+        ok = list(ex.map(check, range(n_req)))
+    out["verified"] = n_req
+    out["verified_against"] = "CPU oracle run on every request"
+    out["mismatches"] = int(len(ok) - sum(ok))
+    out["verify_s"] = round(time.perf_counter() - t1, 1)
     return out
+
+
+def serving_load(T, n_req, rates, budget_gb, seed):
+    """Load sweep: the first n_req requests of the same stream at increasing arrival rates, then
+    all at once (closed burst); each point's throughput, latency and graphs per launch."""
+    from trianglecount_amd.workloads import serving_stream
+    base = serving_stream(n_req, 1000.0, seed)
+    gold = _gold("serving_counts.json")
+    points = []
+    for rate in list(rates) + [None]:
+        res, p = _serve(T, base, budget_gb, t_scale=0.0 if rate is None else 1000.0 / rate)
+        p = {"rate_per_s": rate if rate is not None else "burst", **p}
+        p["offered_edges_per_s"] = None if rate is None else round(p["edges"] / (base[-1].arrive_s * 1000.0 / rate), 1)
+        if gold and gold["seed"] == seed:
+            p["mismatches"] = int(sum(r["count"] != gold["triangles"][i] for i, r in enumerate(res)))
+        points.append(p)
+        log(f"serving load {p}")
+    return {"requests_per_point": n_req, "points": points,
+            "capacity_edges_per_s": max(p["edges_per_s"] for p in points)}
 
 
 def main():
@@ -356,13 +404,15 @@ def main():
     ap.add_argument("--scale", type=int, default=23)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the partitioned and mixed workloads")
+    ap.add_argument("--no-extra", action="store_true", help="skip the partitioned, mixed and serving workloads")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the timed split count (for profiling: no plain, extra or CPU legs)")
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
-    ap.add_argument("--serve-verify", choices=["sample", "all", "none"], default="sample")
+    ap.add_argument("--serve-verify", choices=["all", "none"], default="all")
+    ap.add_argument("--load-requests", type=int, default=2000, help="requests per point of the serving load sweep")
+    ap.add_argument("--load-rates", default="2000,4000,8000", help="arrival rates of the load sweep (+ a burst)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run an int32 elementwise copy of this many MiB (reads N, writes N bytes), "
                          "the known-size kernel that calibrates HBM byte counters")
@@ -373,6 +423,7 @@ def main():
         sys.exit(2)
     import torch
     import trianglecount_amd as T
+    from trianglecount_amd import workloads as W
     T.load()
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
@@ -397,45 +448,73 @@ def main():
     e1.record()
     torch.cuda.synchronize()
     t_prep = e0.elapsed_time(e1) / 1e3
+    g.meta.update(scale=args.scale, seed=args.seed)
     del s, d
     st = graph_stats(g)
     log(f"rmat s{args.scale}: n={g.n} m={g.m} max d+={st['max_out']} gen {t_gen:.2f}s prep {t_prep*1e3:.1f} ms")
 
-    tri, ms, wall = timed_counts(T, g, args.steps, args.warmup, stream, ws)
+    tri, ms, kern_ms, wall = timed_counts(T, g, args.steps, args.warmup, stream, ws)
+    work = T.work_stats([g], ws=ws)[0]
     tri_plain, ms_plain = tri, None
+    inl_ms = None
     if not args.headline_only:
-        tri_plain, ms_plain, _ = timed_counts(T, g, max(1, args.steps // 2), 1, stream, ws, split=False)
+        tri_plain, ms_plain, _, _ = timed_counts(T, g, max(1, args.steps // 2), 1, stream, ws, split=False)
         log(f"plain-CSR count (no in-lists) {ms_plain:.3f} ms/step")
+        # drop-in cost: in-lists and segment words built from a bare oriented CSR
+        bare = T.Graph(g.n, g.m, g.off, g.col)
+        ea, eb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        T.in_lists(bare, ws=ws)
+        torch.cuda.synchronize()
+        ea.record()
+        T.in_lists(bare, ws=ws)
+        eb.record()
+        torch.cuda.synchronize()
+        inl_ms = ea.elapsed_time(eb)
+        del bare
     if tri_plain != tri:
         raise SystemExit(f"split count {tri} differs from the plain-CSR count {tri_plain}")
+    gold = (_gold("rmat_counts.json") or {}).get(str(args.scale))
+    if gold and args.seed == gold["seed"] and tri != gold["triangles"]:
+        raise SystemExit(f"count {tri} differs from the oracle's {gold['triangles']}")
     gteps = g.m / (ms / 1e3) / 1e9
-    ba = b_alg(st)
-    achieved = ba / (ms / 1e3) / 1e9
-    log(f"count {tri} in {ms:.3f} ms/step -> {gteps:.3f} GTEPS; B_alg {ba/1e9:.1f} GB -> {achieved:.0f} GB/s")
+    achieved = work["bytes"] / (kern_ms / 1e3) / 1e9
+    mm = merge_model_bytes(st)
+    traffic, traffic_src = committed_traffic(args.scale)
+    log(f"count {tri} in {ms:.3f} ms/step (kernel {kern_ms:.3f} ms) -> {gteps:.3f} GTEPS; "
+        f"algorithmic {work['bytes']/1e9:.1f} GB -> {achieved:.0f} GB/s")
 
     extra = {}
     if not args.no_extra:
         extra["partitioned"] = partitioned(T, g, 8, args.seed, max(1, args.steps // 2), ws)
-        log(f"partitioned k=8: {extra['partitioned']['ms']} ms -> {extra['partitioned']['gteps']} GTEPS, "
-            f"sum of parts {extra['partitioned']['sum_parts']} (unpartitioned {tri})")
+        log(f"partitioned k=8: {extra['partitioned']}")
         if extra["partitioned"]["sum_parts"] != tri:
             raise SystemExit("partitioned sum differs from the unpartitioned count")
-        mg, _names = mixed_workload(T, args.seed, ws)
+        g.sym_off = g.sym_col = None
+        torch.cuda.empty_cache()
+        mg, _names = W.mixed_graphs(args.seed, ws)
+        torch.cuda.synchronize()
         extra["mixed"] = batched_vs_per_graph(T, mg, 3, ws)
         log(f"mixed workload: {extra['mixed']}")
-        if not extra["mixed"]["identical"]:
-            raise SystemExit("batched counts differ from per-graph counts")
+        if not extra["mixed"]["identical"] or extra["mixed"]["oracle_checked"] is False:
+            raise SystemExit("batched counts differ from per-graph or oracle counts")
         del mg
         torch.cuda.empty_cache()
         extra["serving"] = serving(T, args.serve_requests, args.serve_rate, args.serve_budget_gb, args.seed,
                                    args.serve_verify)
         log(f"serving: {extra['serving']}")
-        if extra["serving"].get("cpu_mismatches"):
-            raise SystemExit("served counts differ from the CPU oracle")
-    cpu = None if args.no_cpu else cpu_baseline(g)
-    if cpu:
+        if extra["serving"].get("mismatches"):
+            raise SystemExit("served counts differ from the oracle")
+        rates = [float(x) for x in args.load_rates.split(",") if x]
+        extra["serving_load"] = serving_load(T, args.load_requests, rates, args.serve_budget_gb, args.seed)
+        if any(p.get("mismatches") for p in extra["serving_load"]["points"]):
+            raise SystemExit("served counts under load differ from the oracle")
+    cpu = None
+    if not args.no_cpu:
+        cpu, (rows, tri_rows) = cpu_baseline(g)
+        cpu["sample_matches_gpu_per_row"] = sample_matches_gpu(T, g, rows, tri_rows)
         log(f"cpu baseline {cpu['value']} GTEPS on {cpu['cores']} threads, {cpu['single_thread_value']} GTEPS on one "
-            f"({cpu['host']['cpu_model']}, nproc {cpu['host']['nproc']})")
+            f"({cpu['host']['cpu_model']}, nproc {cpu['host']['nproc']}); sample == GPU per-row partials: "
+            f"{cpu['sample_matches_gpu_per_row']}")
     res = {
         "metric": "exact-count GTEPS at RMAT s23 ef16; batched vs per-graph launch; p99 latency",
         "value": round(gteps, 4), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
@@ -444,19 +523,27 @@ def main():
         "data": "synthetic (seeded RMAT generated on the GPU)",
         "config": {"workload": f"rmat-s{args.scale}-ef16 unpartitioned exact count", "scale": args.scale,
                    "edge_factor": 16, "abcd": [0.57, 0.19, 0.19, 0.05], "seed": args.seed,
-                   "n": g.n, "m": g.m, "triangles": tri, "prep_ms": round(t_prep * 1e3, 2),
+                   "n": g.n, "m": g.m, "triangles": tri, "oracle_checked": bool(gold),
+                   "prep_ms": round(t_prep * 1e3, 2),
                    "end_to_end_gteps": round(g.m / (t_prep + ms / 1e3) / 1e9, 4),
                    "count_mode": "split (in-lists emitted by the GPU preprocessing)",
+                   "in_lists_from_bare_csr_ms": inl_ms and round(inl_ms, 3),
                    "plain_csr_count_ms": ms_plain and round(ms_plain, 4),
                    "plain_csr_gteps": ms_plain and round(g.m / (ms_plain / 1e3) / 1e9, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": committed_traffic(args.scale)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_count", "kernel_ms": round(kern_ms, 4),
+                     "algorithmic_bytes": work["bytes"], "work": work,
+                     "l2_fabric_gbs": traffic and round(traffic / (kern_ms / 1e3) / 1e9, 1),
+                     "traffic_source": traffic_src,
+                     "merge_model_bytes": mm, "merge_model_gbs": round(mm / (kern_ms / 1e3) / 1e9, 1)},
         "cpu_baseline": cpu,
     }
     if extra:
         res["partitioned_s23_k8"] = extra["partitioned"]
         res["batched_vs_per_graph"] = extra["mixed"]
         res["serving"] = extra["serving"]
+        res["serving_load"] = extra["serving_load"]
     print(json.dumps(res))
 
 

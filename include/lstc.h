@@ -112,14 +112,39 @@ lstc_status lstc_halo_extract(const uint64_t *d_sym_off, const uint32_t *d_sym_c
                               uint32_t *d_col_out, uint32_t *d_l2g_out, uint32_t n_local_cap,
                               uint64_t m_local_cap, void *d_ws, size_t ws_bytes, lstc_stream s);
 
+/* The halo subgraphs of all k parts at once (1 <= k <= 64): part p's output is
+ * exactly what lstc_halo_extract gives for p, but every pass over the
+ * symmetric CSR serves all parts (a per-vertex mask of the parts whose local
+ * set holds it), where the reference builds one part per pass
+ * (round_clean_subgraph.cu:904-976, called once per part).  Vertices with
+ * part[v] >= k are owned by no part.  Both calls synchronise the stream.     */
+typedef struct lstc_halo_out {
+    uint64_t *off;        /* device, n_local + 1                 */
+    uint32_t *col;        /* device, m_local                     */
+    uint32_t *l2g;        /* device, n_local (local -> global), or NULL for every part */
+    uint32_t n_local_cap;
+    uint32_t reserved;
+    uint64_t m_local_cap;
+} lstc_halo_out;
+lstc_status lstc_halo_parts_workspace(uint32_t n, uint32_t k, size_t *bytes);
+/* n_local[k], n_owned[k], m_local[k] */
+lstc_status lstc_halo_parts_sizes(const uint64_t *d_sym_off, const uint32_t *d_sym_col, uint32_t n,
+                                  const uint32_t *d_part, uint32_t k, int mode, uint32_t *n_local,
+                                  uint32_t *n_owned, uint64_t *m_local, void *d_ws, size_t ws_bytes,
+                                  lstc_stream s);
+/* outs[k] (host array of device buffers with their capacities) */
+lstc_status lstc_halo_parts_extract(const uint64_t *d_sym_off, const uint32_t *d_sym_col, uint32_t n,
+                                    const uint32_t *d_part, uint32_t k, int mode, const lstc_halo_out *outs,
+                                    void *d_ws, size_t ws_bytes, lstc_stream s);
+
 /* ------------------------------------------------------------------ (c) counting
  * Tuning knobs (picked by lstc_autotune; zero-initialised = defaults).       */
 typedef struct lstc_tune {
     uint32_t table_scale;  /* LDS hash capacity >= table_scale * keys (2..16), 0 = 6    */
     uint32_t small_max;    /* switch point: rows with d+ <= small_max get a wave-private
                               1024-slot table, larger rows a workgroup-shared 4096-slot
-                              one (64..1024/table_scale), 0 = 1024/table_scale, or 128
-                              for a launch under 2^23 arcs                              */
+                              one (64..1024/table_scale); 0 = min(256, 1024/table_scale),
+                              capped at 128 for a launch under 2^23 arcs              */
     uint32_t small_group;  /* small rows per work ticket (4..256), 0 = planner's choice */
     uint32_t grid_per_cu;  /* persistent workgroups per CU (1..16), 0 = 8              */
     uint32_t unroll;       /* 64-entry chunks in flight per wave (4 or 8), 0 = 4       */
@@ -220,6 +245,40 @@ lstc_status lstc_batch_plan_run(lstc_batch_plan *plan, uint64_t *d_counts, lstc_
 lstc_status lstc_batch_plan_launch_graph(lstc_batch_plan *plan, uint64_t *d_counts, lstc_stream s);
 lstc_status lstc_batch_plan_destroy(lstc_batch_plan *plan);
 
+/* ------------------------------------------------------------------ work and timing
+ * Work of one count of each graph, counted on the device from the graph and
+ * the tuning (no reference counterpart; diagnostic for rooflines): the items
+ * the planner emits, the entries read to build their LDS tables, the lists
+ * located (through segment words, or an id and two row offsets), the list
+ * entries streamed, and from those the algorithmic bytes of the count:
+ *   4 (hashed_keys + streamed) + 8 or 20 per segment + 20 per tiny_segment
+ *   + 16 (items + chunk_items) + 32 tiny_rows + 16 (items + tiny_rows)
+ *   + 32 or 16 per planned row (two planner passes over the row offsets).
+ * out[G]; workspace from lstc_work_stats_workspace; synchronises the stream. */
+typedef struct lstc_work {
+    uint64_t rows;          /* rows the planner visits                              */
+    uint64_t items;         /* hash-path items: source sides + in-list chunks        */
+    uint64_t chunk_items;   /* of which in-list chunks                               */
+    uint64_t tiny_rows;     /* rows counted one lane per arc                         */
+    uint64_t hashed_keys;   /* entries read to build tables (+ tiny rows' lists)     */
+    uint64_t segments;      /* lists located on the hash path                        */
+    uint64_t tiny_segments; /* arcs of the lane-per-arc path                         */
+    uint64_t streamed;      /* list entries streamed through tables or compared      */
+    uint64_t bytes;         /* algorithmic bytes of one count                        */
+} lstc_work;
+lstc_status lstc_work_stats_workspace(uint32_t G, size_t *bytes);
+lstc_status lstc_work_stats(const lstc_graph_desc *graphs, uint32_t G, const lstc_tune *tune,
+                            lstc_work *out, void *d_ws, size_t ws_bytes, lstc_stream s);
+
+/* Count-kernel timing (diagnostic; the reference brackets its kernel with
+ * cudaEvents, src/WithoutPartition/ShareKernelTCV62_update_v1.cu:263-290).
+ * While enabled, every launch of the count kernel on a stream that is not
+ * being captured is bracketed by two hipEvents on that stream.
+ * lstc_timing_read waits for the recorded launches, returns their summed
+ * kernel milliseconds and their number, and forgets them.                    */
+lstc_status lstc_timing_enable(int on);
+lstc_status lstc_timing_read(double *kernel_ms, uint64_t *launches);
+
 /* ------------------------------------------------------------------ autotuning
  * Replaces the reference's compile-time N_THREADS_PER_BLOCK / SHARED_MEM sweep
  * (src/WithoutPartition/ShareKernelTCV62_update_v1.cu:8-9, src/profiling/occup_*.txt).
@@ -229,6 +288,10 @@ lstc_status lstc_batch_plan_destroy(lstc_batch_plan *plan);
  * own scratch and synchronises the stream.  The cache is a TSV of
  * (device, shape class, knobs, ms); the last matching line wins.              */
 lstc_status lstc_shape_class(const lstc_graph_desc *graphs, uint32_t G, uint32_t *shape_class);
+/* The tuning a launch over these graphs runs with: defaults filled in, ranges
+ * clamped, and the small-launch default of small_max applied (host only, reads
+ * only the descriptors' sizes).  A tune from lstc_autotune resolves to itself. */
+lstc_status lstc_tune_resolve(const lstc_graph_desc *graphs, uint32_t G, const lstc_tune *tune, lstc_tune *out);
 lstc_status lstc_autotune(const lstc_graph_desc *graphs, uint32_t G, uint32_t reps, lstc_tune *best,
                           double *best_ms, lstc_stream s);
 lstc_status lstc_tune_cache_lookup(const char *path, uint32_t shape_class, lstc_tune *out);

@@ -110,3 +110,43 @@ def test_batch_and_in_lists_reject_bad_shapes_on_host():
     assert lib.lstc_in_lists(_FAKE, _FAKE, 10, 11, 20, _FAKE, _FAKE, ctypes.byref(k), _FAKE, ctypes.c_size_t(1 << 20),
                              None) == _lib.LSTC_ERR_INVALID
     assert lib.lstc_in_lists_workspace(10, 10, 20, ctypes.byref(nb)) == _lib.LSTC_OK and nb.value > 0
+
+
+def _descs(sizes):
+    arr = (_lib.GraphDesc * len(sizes))()
+    for i, (n, m) in enumerate(sizes):
+        arr[i].n_rows = arr[i].n_owned = n
+        arr[i].m = m
+    return arr
+
+
+def _resolve(lib, descs, tune=None):
+    out = _lib.Tune()
+    t = ctypes.byref(tune) if tune is not None else None
+    assert lib.lstc_tune_resolve(descs, len(descs), t, ctypes.byref(out)) == 0
+    return out.as_dict()
+
+
+def test_tune_resolve_host_only():
+    """The tuning a launch runs with: defaults, clamps and the small-launch cap of small_max."""
+    lib = _lib.load()
+    small = _descs([(1000, 1 << 20), (5000, 1 << 21)])  # 3 * 2^20 arcs < 2^23
+    big = _descs([(1 << 20, 1 << 23)])
+    d = _resolve(lib, small)
+    assert d == {"table_scale": 6, "small_max": 128, "small_group": 0, "grid_per_cu": 8, "unroll": 4,
+                 "tiny_max": 8}
+    assert _resolve(lib, big)["small_max"] == 170  # min(256, 1024 // 6)
+    assert _resolve(lib, big, _lib.Tune(table_scale=2))["small_max"] == 256
+    assert _resolve(lib, big, _lib.Tune(table_scale=16))["small_max"] == 64
+    # an explicit small_max is kept (within its clamp) even for a small launch
+    assert _resolve(lib, small, _lib.Tune(small_max=256))["small_max"] == 170
+    assert _resolve(lib, small, _lib.Tune(small_max=64))["small_max"] == 64
+    # clamps
+    c = _resolve(lib, big, _lib.Tune(table_scale=40, grid_per_cu=99, unroll=5, tiny_max=1, small_group=1))
+    assert (c["table_scale"], c["grid_per_cu"], c["unroll"], c["tiny_max"], c["small_group"]) == (16, 16, 4, 1, 4)
+    # a resolved tune resolves to itself: replaying it runs what was resolved
+    for descs in (small, big):
+        for t in (None, _lib.Tune(table_scale=4), _lib.Tune(small_max=100, unroll=8, tiny_max=4)):
+            r = _resolve(lib, descs, t)
+            assert _resolve(lib, descs, _lib.Tune(**r)) == r
+    assert lib.lstc_tune_resolve(small, 2, None, None) == _lib.LSTC_ERR_INVALID

@@ -95,3 +95,86 @@ def test_halo_split_parts_match_oracle(lstc, oracle, name, builder, k, mode):
         acc += want
     assert acc == whole
     assert lstc.count_batch(parts) == [lstc.count(h) for h in parts]
+
+
+def _parts_equal_oracle(lstc, oracle, g, sym, part_dev, part_host, k, mode, tag):
+    parts = lstc.halo_extract_parts(g, part_dev, k, mode=mode, with_l2g=True)
+    assert len(parts) == k
+    for p, h in enumerate(parts):
+        oh = oracle.halo_extract(sym, part_host, p, mode)
+        assert (h.n_owned, h.n, h.m) == (oh.n_owned, oh.csr.n, oh.csr.m), (tag, p)
+        off, col = h.to_numpy()
+        assert np.array_equal(off, oh.csr.off), (tag, p)
+        assert np.array_equal(col, oh.csr.col), (tag, p)
+        assert np.array_equal(h.meta["l2g"].cpu().numpy().view(np.uint32), oh.l2g), (tag, p)
+    return parts
+
+
+@pytest.mark.parametrize("name,builder,k", [
+    ("multigraph", lambda O: G.random_multigraph(2000, 12000, seed=3), 3),
+    ("rmat_s12", lambda O: O.gen_rmat(12, 16, seed=2), 8),
+    ("rmat_s12_k64", lambda O: O.gen_rmat(12, 16, seed=2), 64),
+    ("rmat_s13_k13", lambda O: O.gen_rmat(13, 8, seed=6), 13),
+    ("grid", lambda O: O.gen_grid(30, 30, 0.3, 4), 4),
+    ("K100", lambda O: G.complete(100), 2),
+    ("K100_k1", lambda O: G.complete(100), 1),
+    ("star", lambda O: G.star(300), 5),
+])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_halo_parts_bit_exact(lstc, oracle, name, builder, k, mode):
+    """All k parts from one call equal the oracle's part-by-part extraction."""
+    n, s, d = builder(oracle)
+    g = _device_graph(lstc, n, s, d)
+    sym, _ = oracle.clean_orient(n, s, d)
+    part = lstc.partition_random(g.n, k, seed=5)
+    opart = oracle.partition_random(sym.n, k, seed=5)
+    _parts_equal_oracle(lstc, oracle, g, sym, part, opart, k, mode, name)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_halo_parts_unowned_and_empty_parts(lstc, oracle, mode):
+    """Vertices assigned to a part >= k are owned by no part but may be halo vertices; empty parts
+    give empty owned blocks.  Same result as the single-part call and the oracle."""
+    n, s, d = oracle.gen_rmat(11, 16, seed=8)
+    g = _device_graph(lstc, n, s, d)
+    sym, _ = oracle.clean_orient(n, s, d)
+    rng = np.random.default_rng(4)
+    k = 6
+    host = rng.choice(np.array([0, 2, 3, 5, 7, 99], np.uint32), size=sym.n)  # parts 1 and 4 empty, 7 and 99 unowned
+    dev = torch.from_numpy(host.view(np.int32)).cuda()
+    parts = _parts_equal_oracle(lstc, oracle, g, sym, dev, host, k, mode, "unowned")
+    for p in range(k):
+        one = lstc.halo_extract(g, dev, p, mode=mode)
+        assert one.n == parts[p].n and one.m == parts[p].m
+        assert all(np.array_equal(a, b) for a, b in zip(one.to_numpy(), parts[p].to_numpy()))
+    assert parts[1].owned == 0 and parts[4].owned == 0
+
+
+def test_halo_parts_sum_and_capacity_errors(lstc, oracle):
+    from trianglecount_amd._lib import HaloOut
+    import ctypes as C
+    n, s, d = lstc.generate_rmat(15, 16, seed=3)
+    g = lstc.clean_orient(n, s, d, keep_symmetric=True)
+    total = lstc.count(g)
+    part = lstc.partition_random(g.n, 8, seed=1)
+    parts = lstc.halo_extract_parts(g, part, 8, mode=0, split=True)
+    assert sum(lstc.count_batch(parts)) == total
+    # a part whose output capacity is too small is refused before anything is written
+    L = lstc.load()
+    outs = (HaloOut * 8)()
+    bufs = []
+    for p, h in enumerate(parts):
+        off = torch.full((h.n + 1,), -7, dtype=torch.int64, device="cuda")
+        col = torch.empty(max(h.m, 1), dtype=torch.int32, device="cuda")
+        bufs.append((off, col))
+        outs[p].off, outs[p].col = off.data_ptr(), col.data_ptr()
+        outs[p].n_local_cap, outs[p].m_local_cap = h.n, h.m - (1 if p == 3 else 0)
+    nb = C.c_size_t()
+    lstc.check(L.lstc_halo_parts_workspace(g.n, 8, C.byref(nb)))
+    ws = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+    st = L.lstc_halo_parts_extract(C.c_void_p(g.sym_off.data_ptr()), C.c_void_p(g.sym_col.data_ptr()), g.n,
+                                   C.c_void_p(part.data_ptr()), 8, 0, outs, C.c_void_p(ws.data_ptr()), nb.value,
+                                   C.c_void_p(lstc.stream_handle()))
+    assert st == -1  # LSTC_ERR_INVALID
+    assert all(int(off[0].item()) == -7 for off, _ in bufs)
+    assert L.lstc_halo_parts_workspace(g.n, 65, C.byref(nb)) == -1

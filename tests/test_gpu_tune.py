@@ -24,6 +24,10 @@ def test_autotune_keeps_counts_and_caches(lstc, oracle, tmp_path):
     d = t.as_dict()
     assert d["table_scale"] in (2, 4, 6, 8, 16) and d["unroll"] in (4, 8)
     assert lstc.count_batch(gs, tune=t) == want
+    # the batch is under 2^23 arcs, so a default small_max runs capped at 128; the returned
+    # tune must name what was timed, so replaying it resolves to itself
+    assert lstc.tune_resolve(gs, t).as_dict() == d
+    assert lstc.tune_resolve(gs).as_dict()["small_max"] == 128
     t2, ms2, hit2 = lstc.autotune(gs, reps=2, cache=cache)
     assert hit2 and ms2 is None and t2.as_dict() == d
     # another shape class misses the cache

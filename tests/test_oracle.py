@@ -293,3 +293,32 @@ def test_rmat_golden_counts_reproduce(oracle):
         n, src, dst = oracle.gen_rmat(s, 16, seed=1)
         _, g = oracle.clean_orient(n, src, dst)
         assert (g.n, g.m, oracle.count(g)) == (gold[str(s)]["n"], gold[str(s)]["m"], gold[str(s)]["triangles"])
+
+
+def test_count_merge_list_is_row_subset_of_brute_force(oracle):
+    """lo_count_merge_list (the CPU baseline's sampled counter): the count over a row list equals
+    sum over those rows u of sum_{w in N+(u)} |N+(u) n N+(w)| by Python sets, for random subsets."""
+    import numpy as np
+    n, s, d = oracle.gen_rmat(10, 16, seed=12)
+    _, g = oracle.clean_orient(n, s, d)
+    nbr = [set(g.col[g.off[u]:g.off[u + 1]].tolist()) for u in range(g.n)]
+    per = [sum(len(nbr[u] & nbr[w]) for w in nbr[u]) for u in range(g.n)]
+    rng = np.random.default_rng(3)
+    for size in (1, 7, 100, g.n):
+        rows = np.sort(rng.choice(g.n, size=size, replace=False)).astype(np.uint32)
+        assert oracle.count_rows(g.off, g.col, rows, threads=2) == sum(per[int(r)] for r in rows)
+    assert oracle.count_rows(g.off, g.col, np.zeros(0, np.uint32)) == 0
+    # every row once is the whole count
+    assert oracle.count_rows(g.off, g.col, np.arange(g.n, dtype=np.uint32)) == oracle.count(g)
+
+
+def test_stats_match_numpy(oracle):
+    """lo_stats (max d+, sum d+^2, sum d- d+, rows with d+ >= 2) against numpy on the same CSR."""
+    import numpy as np
+    for n, s, d in (oracle.gen_rmat(11, 16, seed=2), oracle.gen_er(3000, 10, seed=1)):
+        _, g = oracle.clean_orient(n, s, d)
+        dout = np.diff(g.off.astype(np.int64))
+        din = np.bincount(g.col.astype(np.int64), minlength=g.n)
+        st = oracle.stats(g)
+        assert st == {"n": g.n, "m": g.m, "max_out": int(dout.max()), "sum_out_sq": int((dout * dout).sum()),
+                      "sum_in_out": int((din * dout).sum()), "rows_ge2": int((dout >= 2).sum())}

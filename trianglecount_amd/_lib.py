@@ -47,6 +47,12 @@ class GraphDesc(C.Structure):
                 ("col_seg", C.c_void_p), ("in_seg", C.c_void_p)]
 
 
+class HaloOut(C.Structure):
+    """lstc_halo_out (include/lstc.h)."""
+    _fields_ = [("off", C.c_void_p), ("col", C.c_void_p), ("l2g", C.c_void_p), ("n_local_cap", C.c_uint32),
+                ("reserved", C.c_uint32), ("m_local_cap", C.c_uint64)]
+
+
 class Request(C.Structure):
     """lstc_request (include/lstc.h)."""
     _fields_ = [("kind", C.c_uint32), ("scale", C.c_uint32), ("n", C.c_uint32), ("n_rows", C.c_uint32),
@@ -57,6 +63,15 @@ class Request(C.Structure):
 class Result(C.Structure):
     """lstc_result (include/lstc.h)."""
     _fields_ = [("count", C.c_uint64), ("edges", C.c_uint64), ("latency_s", C.c_double)]
+
+
+class WorkStats(C.Structure):
+    """lstc_work (include/lstc.h)."""
+    _fields_ = [(k, C.c_uint64) for k in ("rows", "items", "chunk_items", "tiny_rows", "hashed_keys", "segments",
+                                          "tiny_segments", "streamed", "bytes")]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
 class ServerStats(C.Structure):
@@ -93,6 +108,9 @@ SIGNATURES = {
     "lstc_halo_workspace": (ST, [U32, PSZ]),
     "lstc_halo_sizes": (ST, [VP, VP, U32, VP, U32, I32, PU32, PU32, PU64, VP, SZ, VP]),
     "lstc_halo_extract": (ST, [VP, VP, U32, VP, U32, I32, VP, VP, VP, U32, U64, VP, SZ, VP]),
+    "lstc_halo_parts_workspace": (ST, [U32, U32, PSZ]),
+    "lstc_halo_parts_sizes": (ST, [VP, VP, U32, VP, U32, I32, PU32, PU32, PU64, VP, SZ, VP]),
+    "lstc_halo_parts_extract": (ST, [VP, VP, U32, VP, U32, I32, C.POINTER(HaloOut), VP, SZ, VP]),
     "lstc_count_workspace": (ST, [U32, U32, U64, C.POINTER(Tune), PSZ]),
     "lstc_count": (ST, [VP, VP, U32, U32, U64, VP, VP, C.POINTER(Tune), VP, SZ, VP]),
     "lstc_count_split": (ST, [VP, VP, VP, VP, VP, VP, U32, U32, U64, VP, C.POINTER(Tune), VP, SZ, VP]),
@@ -106,6 +124,11 @@ SIGNATURES = {
     "lstc_batch_plan_launch_graph": (ST, [VP, VP, VP]),
     "lstc_batch_plan_destroy": (ST, [VP]),
     "lstc_shape_class": (ST, [C.POINTER(GraphDesc), U32, PU32]),
+    "lstc_tune_resolve": (ST, [C.POINTER(GraphDesc), U32, C.POINTER(Tune), C.POINTER(Tune)]),
+    "lstc_work_stats_workspace": (ST, [U32, PSZ]),
+    "lstc_work_stats": (ST, [C.POINTER(GraphDesc), U32, C.POINTER(Tune), C.POINTER(WorkStats), VP, SZ, VP]),
+    "lstc_timing_enable": (ST, [I32]),
+    "lstc_timing_read": (ST, [C.POINTER(C.c_double), PU64]),
     "lstc_autotune": (ST, [C.POINTER(GraphDesc), U32, U32, C.POINTER(Tune), C.POINTER(C.c_double), VP]),
     "lstc_tune_cache_lookup": (ST, [C.c_char_p, U32, C.POINTER(Tune)]),
     "lstc_tune_cache_store": (ST, [C.c_char_p, U32, C.POINTER(Tune), DBL]),

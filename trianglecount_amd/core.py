@@ -306,6 +306,42 @@ def halo_extract(g: Graph, part: torch.Tensor, p: int, mode: int = 0, with_l2g: 
     return h
 
 
+def halo_extract_parts(g: Graph, part: torch.Tensor, k: int, mode: int = 0, with_l2g: bool = False,
+                       ws: Workspace | None = None, stream=None, split: bool = False) -> list:
+    """Halo subgraphs of all ``k`` parts (lstc_halo_parts_*): the same graphs as
+    ``[halo_extract(g, part, p, mode) for p in range(k)]``, built by passes that serve every part."""
+    from ._lib import HaloOut
+    if g.sym_off is None:
+        raise ValueError("halo_extract_parts needs the symmetric CSR: clean_orient(..., keep_symmetric=True)")
+    L = load()
+    k = int(k)
+    nb = C.c_size_t()
+    check(L.lstc_halo_parts_workspace(g.n, k, C.byref(nb)), "lstc_halo_parts_workspace")
+    wp, wb = (ws or _default_ws).get(nb.value)
+    sh = C.c_void_p(stream_handle(stream))
+    nl, no, ml = (C.c_uint32 * k)(), (C.c_uint32 * k)(), (C.c_uint64 * k)()
+    check(L.lstc_halo_parts_sizes(_ptr(g.sym_off), _ptr(g.sym_col), g.n, _ptr(part), k, mode, nl, no, ml, wp, wb,
+                                  sh), "lstc_halo_parts_sizes")
+    outs = (HaloOut * k)()
+    parts = []
+    for p in range(k):
+        h = Graph(nl[p], ml[p], empty_u64(nl[p] + 1), empty_u32(ml[p]), n_owned=no[p])
+        if with_l2g:
+            h.meta["l2g"] = empty_u32(nl[p])
+        outs[p].off, outs[p].col = h.off.data_ptr(), h.col.data_ptr()
+        outs[p].l2g = h.meta["l2g"].data_ptr() if with_l2g else None
+        outs[p].n_local_cap, outs[p].m_local_cap = nl[p], ml[p]
+        parts.append(h)
+    check(L.lstc_halo_parts_extract(_ptr(g.sym_off), _ptr(g.sym_col), g.n, _ptr(part), k, mode, outs, wp, wb, sh),
+          "lstc_halo_parts_extract")
+    for h in parts:
+        if with_l2g:
+            h.meta["l2g"] = h.meta["l2g"][: h.n]
+        if split:
+            in_lists(h, ws=ws, stream=stream)
+    return parts
+
+
 # ------------------------------------------------------------------ batching
 def _descs(graphs, per_row=None):
     from ._lib import GraphDesc
@@ -468,6 +504,52 @@ def shape_class(graphs: list) -> int:
     sc = C.c_uint32()
     check(load().lstc_shape_class(_descs(graphs), len(graphs), C.byref(sc)), "lstc_shape_class")
     return sc.value
+
+
+def work_stats(graphs: list, tune: Tune | None = None, ws: Workspace | None = None, stream=None) -> list[dict]:
+    """Per-graph work of one count and its algorithmic bytes (lstc_work_stats)."""
+    from ._lib import WorkStats
+    L = load()
+    G = len(graphs)
+    nb = C.c_size_t()
+    check(L.lstc_work_stats_workspace(G, C.byref(nb)), "lstc_work_stats_workspace")
+    wp, wb = (ws or _default_ws).get(nb.value)
+    out = (WorkStats * max(G, 1))()
+    check(L.lstc_work_stats(_descs(graphs), G, C.byref(tune) if tune is not None else None, out, wp, wb,
+                            C.c_void_p(stream_handle(stream))), "lstc_work_stats")
+    return [out[i].as_dict() for i in range(G)]
+
+
+class KernelTimer:
+    """Count-kernel time from hipEvents the library records around each launch (lstc_timing_*).
+
+        with KernelTimer() as t:
+            ... counts ...
+        t.ms, t.launches      # summed kernel milliseconds and launches inside the block
+    """
+
+    def __enter__(self):
+        L = load()
+        check(L.lstc_timing_read(None, None), "lstc_timing_read")  # drop earlier records
+        check(L.lstc_timing_enable(1), "lstc_timing_enable")
+        self.ms, self.launches = 0.0, 0
+        return self
+
+    def __exit__(self, *exc):
+        L = load()
+        check(L.lstc_timing_enable(0), "lstc_timing_enable")
+        ms, n = C.c_double(), C.c_uint64()
+        check(L.lstc_timing_read(C.byref(ms), C.byref(n)), "lstc_timing_read")
+        self.ms, self.launches = ms.value, n.value
+        return False
+
+
+def tune_resolve(graphs: list, tune: Tune | None = None) -> Tune:
+    """The tuning a launch over ``graphs`` runs with (lstc_tune_resolve; host only)."""
+    out = Tune()
+    check(load().lstc_tune_resolve(_descs(graphs), len(graphs), C.byref(tune) if tune is not None else None,
+                                   C.byref(out)), "lstc_tune_resolve")
+    return out
 
 
 def autotune(graphs: list, reps: int = 3, cache: str | None = None, stream=None):

@@ -5,6 +5,20 @@
 
 namespace lstc {
 
+// A row's in-list is split into kind B items of kInChunk sources, so that one
+// hub with a short list and a huge in-list (power-law graphs) becomes many
+// items instead of one straggler.  Larger chunks only when the chunk index
+// would not fit its 20-bit field.
+static constexpr uint32_t kInChunk = 512;
+static constexpr uint32_t kMaxChunks = (1u << 20) - 1;
+
+__host__ __device__ __forceinline__ uint32_t in_chunk(uint64_t in_degree) {
+    const uint64_t need = (in_degree + kMaxChunks - 1) / kMaxChunks;
+    return need <= kInChunk ? kInChunk : (uint32_t)((need + 63u) & ~63ull);
+}
+
+static constexpr uint32_t kTinyMax = 8;  // largest d+ of the lane-per-arc path
+
 // One graph as the kernels see it.
 struct GraphDev {
     const uint64_t *off;
@@ -67,6 +81,10 @@ struct TuneR {
     uint32_t tiny_max;      // rows with 2 <= d+ <= tiny_max (graphs with in-lists): lane-per-arc path
 };
 TuneR resolve_tune(const lstc_tune *t);
+// The tuning a launch of total_m arcs runs with: resolve_tune plus the
+// small-launch default of small_max.  The autotuner reports this, so a
+// returned tune replays exactly the configuration that was timed.
+TuneR effective_tune(const lstc_tune *t, uint64_t total_m);
 
 lstc_status validate_graph(const HostGraph &h, const char *who);
 
@@ -84,5 +102,13 @@ constexpr uint32_t kMaxGraphs = 2047;
 lstc_status count_launch(const HostGraph *hosts, uint32_t G, const GraphDev *d_table, const uint64_t *d_row_base,
                          unsigned long long *d_counts, const lstc_tune *tune, void *ws, size_t ws_bytes,
                          hipStream_t st);
+
+// Kernel timing (lstc_timing_enable): when enabled and the stream is not being
+// captured, timing_begin / timing_end record a hipEvent pair around a launch.
+struct TimingMark {
+    hipEvent_t e0, e1;
+};
+TimingMark timing_begin(hipStream_t st);
+void timing_end(const TimingMark &m, hipStream_t st);
 
 }  // namespace lstc

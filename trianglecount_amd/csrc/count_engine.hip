@@ -50,21 +50,9 @@ static constexpr uint32_t kSlice = 1u << kSliceLog2;  // wave-private slice (sma
 static constexpr uint32_t kBigLog2 = 12;
 static constexpr uint32_t kBigCap = 1u << kBigLog2;   // shared table (big items)
 static constexpr uint32_t kBigTile = kBigCap / 2;     // keys per table fill of a big row
-// A row's in-list is split into kind B items of kInChunk sources, so that one
-// hub with a short list and a huge in-list (power-law graphs) becomes many
-// items instead of one straggler.  Larger chunks only when the chunk index
-// would not fit its 20-bit field.
-static constexpr uint32_t kInChunk = 512;
-static constexpr uint32_t kMaxChunks = (1u << 20) - 1;
-
-__host__ __device__ __forceinline__ uint32_t in_chunk(uint64_t in_degree) {
-    const uint64_t need = (in_degree + kMaxChunks - 1) / kMaxChunks;
-    return need <= kInChunk ? kInChunk : (uint32_t)((need + 63u) & ~63ull);
-}
 static constexpr int kBins = 6;                       // hash-path bins
 static constexpr int kTinyBin = kBins;                // the lane-per-arc rows
 static constexpr int kSlots = kBins + 1;              // queue regions
-static constexpr uint32_t kTinyMax = 8;               // largest d+ of the lane-per-arc path
 static constexpr uint32_t kBigGroup = 2;              // big items per ticket
 
 // ctl (uint32) layout
@@ -105,6 +93,17 @@ TuneR resolve_tune(const lstc_tune *t) {
     r.grid_per_cu = hmax(1u, hmin(16u, r.grid_per_cu));
     r.tiny_max = hmin(kTinyMax, r.tiny_max);
     if (r.tiny_max < 2) r.tiny_max = 0;
+    return r;
+}
+
+// A launch of few arcs has few items per wave: rows above 128 keys then
+// share a workgroup table, so their lists spread over four waves (RMAT s16,
+// 0.9M arcs: 20% faster; no change from 2^23 arcs up, where the default keeps
+// rows of up to 1024 / table_scale keys wave-private).  Only a default
+// small_max is capped; an explicit one is kept.
+TuneR effective_tune(const lstc_tune *t, uint64_t total_m) {
+    TuneR r = resolve_tune(t);
+    if ((!t || !t->small_max) && total_m < (1ull << 23) && r.small_max > 128) r.small_max = 128;
     return r;
 }
 
@@ -346,8 +345,11 @@ __device__ __forceinline__ void table_insert(uint32_t *tab, uint32_t *ovf, uint3
         const uint32_t key = gload(col + b0 + j);
         const uint32_t home = hquad(key, qshift);
         uint32_t h = home;
-        // a slot is tried only once the ones before it in the bucket are taken
-        for (;;) {
+        // A slot is tried only once the ones before it in the bucket are taken.
+        // At most 1/2 of the slots are filled, so a free slot lies within one
+        // lap of the buckets; the lap bound makes the loop finite by
+        // construction as well.
+        for (uint32_t step = 0; step <= qmask; ++step) {
             uint32_t *b = tab + 4 * h;
             if (atomicCAS(b, kEmpty, key) == kEmpty || atomicCAS(b + 1, kEmpty, key) == kEmpty ||
                 atomicCAS(b + 2, kEmpty, key) == kEmpty || atomicCAS(b + 3, kEmpty, key) == kEmpty)
@@ -476,7 +478,12 @@ __device__ __forceinline__ void drain_queue(const uint4 *buckets, uint32_t qmask
     const uint32_t xv = act ? qx[lane] : kNoId;
     uint32_t q = (hquad(xv, qshift) + 1u) & qmask;
     uint64_t live = ballot(act);
-    while (live != 0) {
+    // Every chain ends at a hit or at a bucket whose last slot is empty: slots
+    // fill in order, so a bucket with its last slot taken is full, and with at
+    // most 1/2 of the slots filled at least half of the buckets end chains.
+    // A chain therefore ends within one lap; the lap bound keeps the loop
+    // finite by construction as well.
+    for (uint32_t step = 0; live != 0 && step < qmask; ++step) {
         const uint4 w = buckets[q];
         const uint64_t H = bucket_hits(w, xv) & live;
         add_lane_bit(lane_hits, H);
@@ -902,17 +909,12 @@ lstc_status count_launch(const HostGraph *hosts, uint32_t G, const GraphDev *d_t
     if (G > kMaxGraphs) return fail(LSTC_ERR_INVALID, "count: at most %u graphs per launch", kMaxGraphs);
     if (!d_counts) return fail(LSTC_ERR_INVALID, "count: NULL counts");
     if (G > 1 && (!d_table || !d_row_base)) return fail(LSTC_ERR_INVALID, "count: batch needs device descriptors");
-    TuneR tr = resolve_tune(tune);
     uint64_t total = 0, total_m = 0;
     for (uint32_t g = 0; g < G; ++g) {
         total += plan_rows(hosts[g]);
         total_m += hosts[g].m;
     }
-    // A launch of few arcs has few items per wave: rows above 128 keys then
-    // share a workgroup table, so their lists spread over four waves (RMAT s16,
-    // 0.9M arcs: 20% faster; no change from 2^23 arcs up, where the default
-    // keeps rows of up to 1024 / table_scale keys wave-private).
-    if ((!tune || !tune->small_max) && total_m < (1ull << 23) && tr.small_max > 128) tr.small_max = 128;
+    const TuneR tr = effective_tune(tune, total_m);
     if (total >= 0xFFFFFFFFull) return fail(LSTC_ERR_INVALID, "count: more than 2^32-1 owned rows in one launch");
     Carver c(ws, ws_bytes);
     if (queue_cap(total, total_m) >= 0xFFFFFFFFull)
@@ -952,8 +954,10 @@ lstc_status count_launch(const HostGraph *hosts, uint32_t G, const GraphDev *d_t
         for (uint32_t g = 0; g < G; ++g) narrow = narrow && hosts[g].m <= (1ull << 30);
         auto kern = tr.unroll == 8 ? (narrow ? k_count<8, true> : k_count<8, false>)
                                    : (narrow ? k_count<4, true> : k_count<4, false>);
+        const TimingMark tm = timing_begin(st);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, st, gs, queue, ctl, d_counts, tr.table_scale);
         LSTC_LAUNCHED("k_count");
+        timing_end(tm, st);
     }
     return LSTC_OK;
 }

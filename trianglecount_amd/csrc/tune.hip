@@ -50,6 +50,23 @@ extern "C" lstc_status lstc_shape_class(const lstc_graph_desc *graphs, uint32_t 
     return LSTC_OK;
 }
 
+extern "C" lstc_status lstc_tune_resolve(const lstc_graph_desc *graphs, uint32_t G, const lstc_tune *tune,
+                                         lstc_tune *out) {
+    if (!out || (G && !graphs)) return fail(LSTC_ERR_INVALID, "lstc_tune_resolve: NULL argument");
+    uint64_t arcs = 0;
+    for (uint32_t g = 0; g < G; ++g) arcs += graphs[g].m;
+    const TuneR r = effective_tune(tune, arcs);
+    lstc_tune t{};
+    t.table_scale = r.table_scale;
+    t.small_max = r.small_max;
+    t.small_group = r.small_group;
+    t.grid_per_cu = r.grid_per_cu;
+    t.unroll = r.unroll;
+    t.tiny_max = r.tiny_max ? r.tiny_max : 1u;
+    *out = t;
+    return LSTC_OK;
+}
+
 extern "C" lstc_status lstc_autotune(const lstc_graph_desc *graphs, uint32_t G, uint32_t reps, lstc_tune *best,
                                      double *best_ms, lstc_stream s) {
     hipStream_t st = (hipStream_t)s;
@@ -121,15 +138,9 @@ extern "C" lstc_status lstc_autotune(const lstc_graph_desc *graphs, uint32_t G, 
     (void)hipEventDestroy(e1);
     (void)hipFree(d_counts);
     if (rc != LSTC_OK) return rc;
-    // record the resolved values so a cache line is explicit
-    const TuneR r = resolve_tune(&cur);
-    best->table_scale = r.table_scale;
-    best->small_max = r.small_max;
-    best->small_group = r.small_group;
-    best->grid_per_cu = r.grid_per_cu;
-    best->unroll = r.unroll;
-    best->tiny_max = r.tiny_max ? r.tiny_max : 1u;
-    memset(best->reserved, 0, sizeof(best->reserved));
+    // record the values the winning launches ran with, so a cache line is
+    // explicit and replays the timed configuration
+    LSTC_TRY(lstc_tune_resolve(graphs, G, &cur, best));
     if (best_ms) *best_ms = cur_ms;
     return LSTC_OK;
 }
