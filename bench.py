@@ -309,16 +309,18 @@ def _serve(T, reqs, budget_gb, t_scale=1.0):
     import numpy as np
     srv = T.Server(int(budget_gb * (1 << 30)))
     ids = []
-    t0 = time.perf_counter()
-    for r in reqs:
-        at = r.arrive_s * t_scale
-        while time.perf_counter() - t0 < at:
-            pass
-        ids.append(srv.submit_rmat(r.a, r.m_raw, r.seed) if r.kind == "rmat" else srv.submit_er(r.a, r.m_raw, r.seed))
-    res = [srv.wait(i) for i in ids]
-    wall = time.perf_counter() - t0
-    st = srv.stats()
-    srv.close()
+    with T.KernelTimer() as kt:  # the server's count kernels, on its count stream
+        t0 = time.perf_counter()
+        for r in reqs:
+            at = r.arrive_s * t_scale
+            while time.perf_counter() - t0 < at:
+                pass
+            ids.append(srv.submit_rmat(r.a, r.m_raw, r.seed) if r.kind == "rmat" else
+                       srv.submit_er(r.a, r.m_raw, r.seed))
+        res = [srv.wait(i) for i in ids]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
     lat = np.array([x["latency_s"] for x in res])
     edges = int(sum(x["edges"] for x in res))
     return res, {"wall_s": round(wall, 3), "edges": edges, "edges_per_s": round(edges / wall, 1),
@@ -326,6 +328,7 @@ def _serve(T, reqs, budget_gb, t_scale=1.0):
                  "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
                  "mean_ms": round(float(lat.mean()) * 1e3, 3), "max_ms": round(float(lat.max()) * 1e3, 3),
                  "batches": int(st["batches"]), "graphs_per_launch": round(len(reqs) / max(1, st["batches"]), 2),
+                 "count_kernel_busy": round(kt.ms / 1e3 / wall, 3),
                  "max_batch_graphs": int(st["max_batch_graphs"]), "rejected": int(st["rejected"])}
 
 
@@ -381,7 +384,7 @@ def serving_load(T, n_req, rates, budget_gb, seed):
     """Load sweep: the first n_req requests of the same stream at increasing arrival rates, then
     all at once (closed burst); each point's throughput, latency and graphs per launch."""
     from trianglecount_amd.workloads import serving_stream
-    base = serving_stream(n_req, 1000.0, seed)
+    base = serving_stream(10000, 1000.0, seed)[:n_req]  # the head of the committed stream
     gold = _gold("serving_counts.json")
     points = []
     for rate in list(rates) + [None]:

@@ -4,19 +4,26 @@
 // (README.md:57-91).  Here a server owns one device slab of `budget` bytes and a
 // worker thread that runs the loop
 //   admit   : FIFO requests whose working set fits the free slab space get
-//             memory and their preparation (seeded generation + clean/orient)
-//             enqueued on the prep stream;
+//             memory and their preparation (seeded generation + clean/orient +
+//             in-lists + segment words) enqueued on one of kPrepStreams prep
+//             streams, round robin, so a large request does not hold up the
+//             small ones behind it;
 //   collect : prepared requests (prep event done) release their transient
-//             memory and wait for counting; their CSR stays resident;
-//   launch  : when the count stream is idle, every prepared request is counted
-//             by ONE persistent batched launch (count_engine.hip), results are
-//             copied to pinned host memory behind an event;
-//   finish  : when that event completes, counts and latencies are published,
-//             resident memory is freed and waiters are woken.
-// Requests that arrive while a batch is on the GPU are prepared concurrently on
-// the prep stream and join the next launch, so preparation overlaps counting
-// and the batch grows with the load (continuous batching).  Nothing on the
-// device waits on host flags: every kernel runs to completion on its own.
+//             memory, have their out-of-range flag checked and wait for
+//             counting; their CSR stays resident;
+//   launch  : while one of the kSlots batch slots is free, every prepared
+//             request is counted by ONE persistent batched launch
+//             (count_engine.hip) on the count stream, results copied to pinned
+//             host memory behind an event;
+//   finish  : when a slot's event completes, counts and latencies are
+//             published, resident memory is freed and waiters are woken.
+// Requests that arrive while batches are on the GPU are prepared concurrently
+// and join the next launch, so the batch grows with the load (continuous
+// batching).  With two slots the next batch is already queued behind the one
+// that runs, so the device does not idle while the host notices a finished
+// batch.  The loop is a host thread on purpose: nothing on the device waits on
+// host flags, so every kernel runs to completion on its own and a lost or
+// stalled host never leaves a spinning kernel on the GPU.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -112,6 +119,7 @@ struct Req {
     uint64_t m_cap = 0;
     hipEvent_t prep_done = nullptr;
     const uint32_t *d_bad = nullptr;
+    int flag_slot = -1;  // pinned host word that receives *d_bad at the end of the preparation
 };
 
 struct Result {
@@ -129,6 +137,21 @@ __global__ void k_gather_edges(const GraphDev *__restrict__ desc, uint32_t G, un
 
 static size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
 
+static constexpr int kPrepStreams = 3;  // preparation streams (round robin over requests)
+static constexpr int kSlots = 2;        // batches that may be queued on the count stream at once
+static constexpr int kFlagSlots = 4096; // requests in preparation at once (pinned flag words)
+
+// One batch in flight: its share of the reserve and its pinned staging.
+struct BatchSlot {
+    char *reserve = nullptr;
+    GraphDev *h_desc = nullptr;
+    uint64_t *h_base = nullptr;
+    unsigned long long *h_counts = nullptr;
+    unsigned long long *h_edges = nullptr;
+    hipEvent_t done = nullptr;
+    std::vector<Req *> batch;  // empty = free
+};
+
 }  // namespace lstc
 
 using namespace lstc;
@@ -141,15 +164,14 @@ struct lstc_server {
     bool has_tune = false;
     char *slab = nullptr;
     SlabPool pool;             // request working sets (generation, cleaning, resident CSRs)
-    char *reserve = nullptr;   // batch descriptors, counts and work queue (outside the pool)
-    size_t reserve_bytes = 0;
-    hipStream_t prep_stream = nullptr, count_stream = nullptr;
-    // pinned host staging for descriptors and results of the batch in flight
-    GraphDev *h_desc = nullptr;
-    uint64_t *h_base = nullptr;
-    unsigned long long *h_counts = nullptr;
-    unsigned long long *h_edges = nullptr;
-    hipEvent_t batch_done = nullptr;
+    size_t reserve_bytes = 0;  // per batch slot: descriptors, counts and work queue (outside the pool)
+    hipStream_t prep_stream[kPrepStreams] = {};
+    hipStream_t count_stream = nullptr;
+    uint32_t next_prep = 0;
+    BatchSlot slot[kSlots];
+    uint32_t next_slot = 0;    // the slot the next launch takes (slots complete in launch order)
+    uint32_t *h_flags = nullptr;  // pinned out-of-range flags of requests in preparation
+    std::vector<int> free_flags;
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -164,8 +186,8 @@ struct lstc_server {
     std::thread worker;
 
     // worker-private state
-    std::deque<Req *> pending, preparing, ready;
-    std::vector<Req *> in_flight;
+    std::deque<Req *> pending, ready;
+    std::vector<Req *> preparing;
 };
 
 namespace lstc {
@@ -203,6 +225,7 @@ static lstc_status admit(lstc_server *S) {
             S->ready.push_back(r);
             continue;
         }
+        if (S->free_flags.empty()) break;  // enough requests in preparation for now
         const uint32_t n = request_rows(r->spec);
         size_t ws = 0;
         const size_t tb = prep_bytes(n, r->spec.m_raw, &ws), rb = resident_bytes(n, r->spec.m_raw);
@@ -226,7 +249,11 @@ static lstc_status admit(lstc_server *S) {
         r->in_seg = (uint64_t *)(res + 2 * half + round256(8 * (r->spec.m_raw ? r->spec.m_raw : 1)));
         r->n_rows = n;
         r->m_cap = r->spec.m_raw;
-        hipStream_t st = S->prep_stream;
+        r->flag_slot = S->free_flags.back();
+        S->free_flags.pop_back();
+        S->preparing.push_back(r);  // from here on fail_all releases it
+        hipStream_t st = S->prep_stream[S->next_prep];
+        S->next_prep = (S->next_prep + 1) % kPrepStreams;
         lstc_status rc;
         if (r->spec.kind == LSTC_REQ_RMAT)
             rc = lstc_gen_rmat(r->spec.scale, r->spec.m_raw, r->spec.a, r->spec.b, r->spec.c, r->spec.seed, r->src,
@@ -239,26 +266,49 @@ static lstc_status admit(lstc_server *S) {
                                      r->in_col, r->ws, r->ws_bytes, st, nullptr, &r->d_bad));
         LSTC_TRY(segments_launch_dev(r->off, r->col, r->in_off, r->in_col, n, r->spec.m_raw, r->col_seg, r->in_seg,
                                      st));
+        // the out-of-range flag of the cleaning comes back with the preparation
+        LSTC_HIP(hipMemcpyAsync(S->h_flags + r->flag_slot, r->d_bad, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         LSTC_HIP(hipEventCreateWithFlags(&r->prep_done, hipEventDisableTiming));
         LSTC_HIP(hipEventRecord(r->prep_done, st));
-        S->preparing.push_back(r);
     }
     return LSTC_OK;
 }
 
+static void publish_failed(lstc_server *S, Req *r, lstc_status st) {
+    const double lat = std::chrono::duration<double>(Clock::now() - r->t_submit).count();
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->results[r->id] = Result{st, 0, 0, lat};
+    S->n_completed++;
+}
+
+// Requests on different prep streams finish in any order: every one whose
+// event has completed is collected.
 static lstc_status collect_prepared(lstc_server *S) {
-    while (!S->preparing.empty()) {
-        Req *r = S->preparing.front();
+    bool failed_any = false;
+    for (size_t i = 0; i < S->preparing.size();) {
+        Req *r = S->preparing[i];
         hipError_t e = hipEventQuery(r->prep_done);
-        if (e == hipErrorNotReady) break;
+        if (e == hipErrorNotReady) { ++i; continue; }
         if (e != hipSuccess) return hip_fail(e, "server: prep event");
         (void)hipEventDestroy(r->prep_done);
         r->prep_done = nullptr;
         S->pool.release(r->transient);
         r->transient = nullptr;
-        S->preparing.pop_front();
+        const bool bad = S->h_flags[r->flag_slot] != 0;
+        S->free_flags.push_back(r->flag_slot);
+        r->flag_slot = -1;
+        S->preparing[i] = S->preparing.back();
+        S->preparing.pop_back();
+        if (bad) {  // an id out of range: the graph is not counted
+            S->pool.release(r->resident);
+            publish_failed(S, r, LSTC_ERR_INVALID);
+            delete r;
+            failed_any = true;
+            continue;
+        }
         S->ready.push_back(r);
     }
+    if (failed_any) S->cv_done.notify_all();
     return LSTC_OK;
 }
 
@@ -272,8 +322,9 @@ size_t batch_head_bytes(uint32_t G) {
 }
 
 static lstc_status launch_batch(lstc_server *S) {
-    if (!S->in_flight.empty() || S->ready.empty()) return LSTC_OK;
-    // every ready request joins, as long as the work queue fits the reserve
+    BatchSlot &B = S->slot[S->next_slot];
+    if (!B.batch.empty() || S->ready.empty()) return LSTC_OK;
+    // every ready request joins, as long as the work queue fits the slot's reserve
     std::vector<Req *> batch;
     uint64_t rows = 0, arcs = 0;
     while (!S->ready.empty() && batch.size() < S->max_batch) {
@@ -294,57 +345,62 @@ static lstc_status launch_batch(lstc_server *S) {
         Req *r = batch[g];
         hosts[g] = HostGraph{r->off, r->col, nullptr, r->n_rows, r->n_rows, r->m_cap, r->in_off, r->in_col,
                              r->col_seg, r->in_seg};
-        S->h_desc[g] = to_dev(hosts[g]);
-        S->h_base[g] = acc;
+        B.h_desc[g] = to_dev(hosts[g]);
+        B.h_base[g] = acc;
         acc += r->n_rows;
     }
-    S->h_base[G] = acc;
+    B.h_base[G] = acc;
     const size_t head = batch_head_bytes(G);
-    Carver d(S->reserve, S->reserve_bytes);
+    Carver d(B.reserve, S->reserve_bytes);
     GraphDev *d_desc = d.take<GraphDev>(G);
     uint64_t *d_base = d.take<uint64_t>((size_t)G + 1);
     unsigned long long *d_counts = d.take<unsigned long long>(G);
     unsigned long long *d_edges = d.take<unsigned long long>(G);
     hipStream_t st = S->count_stream;
-    LSTC_HIP(hipMemcpyAsync(d_desc, S->h_desc, sizeof(GraphDev) * G, hipMemcpyHostToDevice, st));
-    LSTC_HIP(hipMemcpyAsync(d_base, S->h_base, sizeof(uint64_t) * (G + 1), hipMemcpyHostToDevice, st));
+    B.batch = batch;  // from here on the slot owns the requests (fail_all releases them)
+    LSTC_HIP(hipMemcpyAsync(d_desc, B.h_desc, sizeof(GraphDev) * G, hipMemcpyHostToDevice, st));
+    LSTC_HIP(hipMemcpyAsync(d_base, B.h_base, sizeof(uint64_t) * (G + 1), hipMemcpyHostToDevice, st));
     LSTC_TRY(count_launch(hosts.data(), G, G > 1 ? d_desc : nullptr, G > 1 ? d_base : nullptr, d_counts,
-                          S->has_tune ? &S->tune : nullptr, S->reserve + head, S->reserve_bytes - head, st));
+                          S->has_tune ? &S->tune : nullptr, B.reserve + head, S->reserve_bytes - head, st));
     hipLaunchKernelGGL(k_gather_edges, dim3(1), dim3(256), 0, st, d_desc, G, d_edges);
     LSTC_LAUNCHED("k_gather_edges");
-    LSTC_HIP(hipMemcpyAsync(S->h_counts, d_counts, sizeof(unsigned long long) * G, hipMemcpyDeviceToHost, st));
-    LSTC_HIP(hipMemcpyAsync(S->h_edges, d_edges, sizeof(unsigned long long) * G, hipMemcpyDeviceToHost, st));
-    LSTC_HIP(hipEventRecord(S->batch_done, st));
-    S->in_flight = batch;
+    LSTC_HIP(hipMemcpyAsync(B.h_counts, d_counts, sizeof(unsigned long long) * G, hipMemcpyDeviceToHost, st));
+    LSTC_HIP(hipMemcpyAsync(B.h_edges, d_edges, sizeof(unsigned long long) * G, hipMemcpyDeviceToHost, st));
+    LSTC_HIP(hipEventRecord(B.done, st));
+    S->next_slot = (S->next_slot + 1) % kSlots;
     return LSTC_OK;
 }
 
+// Slots complete in launch order (one count stream): the oldest one is checked.
 static lstc_status finish_batch(lstc_server *S) {
-    if (S->in_flight.empty()) return LSTC_OK;
-    hipError_t e = hipEventQuery(S->batch_done);
-    if (e == hipErrorNotReady) return LSTC_OK;
-    if (e != hipSuccess) return hip_fail(e, "server: batch event");
-    const auto now = Clock::now();
-    {
-        std::lock_guard<std::mutex> lk(S->mu);
-        for (size_t g = 0; g < S->in_flight.size(); ++g) {
-            Req *r = S->in_flight[g];
-            const double lat = std::chrono::duration<double>(now - r->t_submit).count();
-            S->results[r->id] = Result{LSTC_OK, (uint64_t)S->h_counts[g], (uint64_t)S->h_edges[g], lat};
-            S->latencies.push_back(lat);
-            S->n_completed++;
-            S->edges_done += S->h_edges[g];
-            S->triangles_done += S->h_counts[g];
+    for (int k = 0; k < kSlots; ++k) {
+        BatchSlot &B = S->slot[(S->next_slot + k) % kSlots];
+        if (B.batch.empty()) continue;
+        hipError_t e = hipEventQuery(B.done);
+        if (e == hipErrorNotReady) return LSTC_OK;
+        if (e != hipSuccess) return hip_fail(e, "server: batch event");
+        const auto now = Clock::now();
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            for (size_t g = 0; g < B.batch.size(); ++g) {
+                Req *r = B.batch[g];
+                const double lat = std::chrono::duration<double>(now - r->t_submit).count();
+                S->results[r->id] = Result{LSTC_OK, (uint64_t)B.h_counts[g], (uint64_t)B.h_edges[g], lat};
+                S->latencies.push_back(lat);
+                S->n_completed++;
+                S->edges_done += B.h_edges[g];
+                S->triangles_done += B.h_counts[g];
+            }
+            S->n_batches++;
+            S->max_batch_seen = std::max<uint64_t>(S->max_batch_seen, B.batch.size());
         }
-        S->n_batches++;
-        S->max_batch_seen = std::max<uint64_t>(S->max_batch_seen, S->in_flight.size());
+        for (Req *r : B.batch) {
+            S->pool.release(r->resident);
+            delete r;
+        }
+        B.batch.clear();
+        S->cv_done.notify_all();
     }
-    for (Req *r : S->in_flight) {
-        S->pool.release(r->resident);
-        delete r;
-    }
-    S->in_flight.clear();
-    S->cv_done.notify_all();
     return LSTC_OK;
 }
 
@@ -358,10 +414,19 @@ static void fail_all(lstc_server *S, lstc_status st) {
     for (Req *r : S->pending) drop(r);
     for (Req *r : S->preparing) drop(r);
     for (Req *r : S->ready) drop(r);
-    for (Req *r : S->in_flight) drop(r);
+    for (BatchSlot &B : S->slot) {
+        for (Req *r : B.batch) drop(r);
+        B.batch.clear();
+    }
     for (Req *r : S->submitted) drop(r);
-    S->pending.clear(); S->preparing.clear(); S->ready.clear(); S->in_flight.clear(); S->submitted.clear();
+    S->pending.clear(); S->preparing.clear(); S->ready.clear(); S->submitted.clear();
     S->cv_done.notify_all();
+}
+
+static bool slots_busy(const lstc_server *S) {
+    for (const BatchSlot &B : S->slot)
+        if (!B.batch.empty()) return true;
+    return false;
 }
 
 static void worker_main(lstc_server *S) {
@@ -369,7 +434,7 @@ static void worker_main(lstc_server *S) {
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            const bool idle = S->pending.empty() && S->preparing.empty() && S->ready.empty() && S->in_flight.empty();
+            const bool idle = S->pending.empty() && S->preparing.empty() && S->ready.empty() && !slots_busy(S);
             if (idle && S->submitted.empty()) {
                 if (S->stop) break;
                 S->cv_work.wait_for(lk, std::chrono::milliseconds(2));
@@ -382,9 +447,9 @@ static void worker_main(lstc_server *S) {
         lstc_status rc = finish_batch(S);
         if (rc == LSTC_OK) rc = collect_prepared(S);
         if (rc == LSTC_OK) rc = admit(S);
-        if (rc == LSTC_OK) rc = launch_batch(S);
+        for (int k = 0; rc == LSTC_OK && k < kSlots; ++k) rc = launch_batch(S);
         if (rc != LSTC_OK) { fail_all(S, rc); break; }
-        const bool busy = !S->in_flight.empty() || !S->preparing.empty();
+        const bool busy = slots_busy(S) || !S->preparing.empty();
         if (busy) std::this_thread::sleep_for(std::chrono::microseconds(50));
     }
 }
@@ -407,22 +472,31 @@ extern "C" lstc_status lstc_server_create(size_t hbm_budget_bytes, uint32_t max_
         delete S;
         return fail(LSTC_ERR_OOM, "lstc_server_create: cannot allocate %zu bytes of HBM", hbm_budget_bytes);
     }
-    // batch work queues live in a fixed reserve so resident CSRs can never starve the count
-    S->reserve_bytes = hmin<size_t>(hmax<size_t>(hbm_budget_bytes / 8, (size_t)16 << 20), (size_t)4 << 30);
-    S->reserve_bytes = hmin<size_t>(S->reserve_bytes, hbm_budget_bytes / 2);
-    S->reserve = S->slab + (hbm_budget_bytes - S->reserve_bytes);
-    S->pool.init(S->slab, hbm_budget_bytes - S->reserve_bytes);
-    e = hipStreamCreateWithFlags(&S->prep_stream, hipStreamNonBlocking);
+    // batch work queues live in a fixed reserve (one share per batch slot), so resident
+    // CSRs can never starve the count
+    size_t reserve = hmin<size_t>(hmax<size_t>(hbm_budget_bytes / 8, (size_t)32 << 20), (size_t)4 << 30);
+    reserve = hmin<size_t>(reserve, hbm_budget_bytes / 2);
+    S->reserve_bytes = (reserve / kSlots) & ~size_t(255);
+    char *res0 = S->slab + (hbm_budget_bytes - (size_t)kSlots * S->reserve_bytes);
+    S->pool.init(S->slab, (size_t)(res0 - S->slab) & ~size_t(255));
+    for (int i = 0; i < kPrepStreams && e == hipSuccess; ++i)
+        e = hipStreamCreateWithFlags(&S->prep_stream[i], hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->count_stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_desc, sizeof(GraphDev) * max_batch);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_base, sizeof(uint64_t) * (max_batch + 1));
-    if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_counts, sizeof(unsigned long long) * max_batch);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_edges, sizeof(unsigned long long) * max_batch);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&S->batch_done, hipEventDisableTiming);
+    for (int k = 0; k < kSlots && e == hipSuccess; ++k) {
+        BatchSlot &B = S->slot[k];
+        B.reserve = res0 + (size_t)k * S->reserve_bytes;
+        e = hipHostMalloc((void **)&B.h_desc, sizeof(GraphDev) * max_batch);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&B.h_base, sizeof(uint64_t) * (max_batch + 1));
+        if (e == hipSuccess) e = hipHostMalloc((void **)&B.h_counts, sizeof(unsigned long long) * max_batch);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&B.h_edges, sizeof(unsigned long long) * max_batch);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&B.done, hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_flags, sizeof(uint32_t) * kFlagSlots);
     if (e != hipSuccess) {
         lstc_server_destroy(S);
         return hip_fail(e, "lstc_server_create");
     }
+    for (int i = kFlagSlots - 1; i >= 0; --i) S->free_flags.push_back(i);
     S->worker = std::thread(worker_main, S);
     *out = S;
     return LSTC_OK;
@@ -540,13 +614,17 @@ extern "C" lstc_status lstc_server_destroy(lstc_server *S) {
         S->cv_work.notify_all();
         S->worker.join();
     }
-    if (S->prep_stream) { (void)hipStreamSynchronize(S->prep_stream); (void)hipStreamDestroy(S->prep_stream); }
+    for (hipStream_t &ps : S->prep_stream)
+        if (ps) { (void)hipStreamSynchronize(ps); (void)hipStreamDestroy(ps); }
     if (S->count_stream) { (void)hipStreamSynchronize(S->count_stream); (void)hipStreamDestroy(S->count_stream); }
-    if (S->batch_done) (void)hipEventDestroy(S->batch_done);
-    if (S->h_desc) (void)hipHostFree(S->h_desc);
-    if (S->h_base) (void)hipHostFree(S->h_base);
-    if (S->h_counts) (void)hipHostFree(S->h_counts);
-    if (S->h_edges) (void)hipHostFree(S->h_edges);
+    for (BatchSlot &B : S->slot) {
+        if (B.done) (void)hipEventDestroy(B.done);
+        if (B.h_desc) (void)hipHostFree(B.h_desc);
+        if (B.h_base) (void)hipHostFree(B.h_base);
+        if (B.h_counts) (void)hipHostFree(B.h_counts);
+        if (B.h_edges) (void)hipHostFree(B.h_edges);
+    }
+    if (S->h_flags) (void)hipHostFree(S->h_flags);
     if (S->slab) (void)hipFree(S->slab);
     delete S;
     return LSTC_OK;
