@@ -182,6 +182,20 @@ lstc_status lstc_count_split(const uint64_t *d_off, const uint32_t *d_col, const
                              uint32_t n_rows, uint32_t n_owned, uint64_t m, uint64_t *d_count,
                              const lstc_tune *tune, void *d_ws, size_t ws_bytes, lstc_stream s);
 
+/* lstc_count_split without a workspace argument, for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid allocation).  No counterpart in the reference.  The
+ * engine's scratch is a static device array of lstc_count_split_static_limit()
+ * bytes (32 MiB) per GPU, held until the process ends; a graph whose scratch
+ * need (lstc_count_workspace) exceeds it is refused with LSTC_ERR_INVALID.
+ * The call allocates nothing.  Calls that overlap on one GPU share the array
+ * and race: keep them on one stream or serialise them.                      */
+size_t lstc_count_split_static_limit(void);
+lstc_status lstc_count_split_static(const uint64_t *d_off, const uint32_t *d_col, const uint64_t *d_in_off,
+                                    const uint32_t *d_in_col, const uint64_t *d_col_seg, const uint64_t *d_in_seg,
+                                    uint32_t n_rows, uint32_t n_owned, uint64_t m, uint64_t *d_count,
+                                    const lstc_tune *tune, lstc_stream s);
+
 /* Target-side in-lists of an oriented CSR for split counting: for every row w,
  * the owned sources u < n_owned of arcs u -> w with d+(u) <= d+(w) (row lengths
  * of this CSR), ascending.  d_in_off[n_rows+1], d_in_col[m] (at most m entries);

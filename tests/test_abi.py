@@ -150,3 +150,18 @@ def test_tune_resolve_host_only():
             r = _resolve(lib, descs, t)
             assert _resolve(lib, descs, _lib.Tune(**r)) == r
     assert lib.lstc_tune_resolve(small, 2, None, None) == _lib.LSTC_ERR_INVALID
+
+
+def test_static_count_refuses_before_any_device_call():
+    """lstc_count_split_static checks its arguments and the scratch limit on the host first."""
+    lib = _lib.load()
+    assert lib.lstc_count_split_static_limit() == 32 << 20
+    fake = ctypes.c_void_p(0x1000)
+    # scratch need above the limit
+    st = lib.lstc_count_split_static(fake, fake, fake, fake, None, None, 1 << 23, 1 << 23, 1 << 20, fake, None, None)
+    assert st == _lib.LSTC_ERR_INVALID and b"static scratch" in lib.lstc_last_error()
+    # missing in-lists / output
+    assert lib.lstc_count_split_static(fake, fake, None, None, None, None, 10, 10, 20, fake, None, None) == \
+        _lib.LSTC_ERR_INVALID
+    assert lib.lstc_count_split_static(fake, fake, fake, fake, None, None, 10, 10, 20, None, None, None) == \
+        _lib.LSTC_ERR_INVALID

@@ -99,9 +99,11 @@ constexpr uint32_t kMaxGraphs = 2047;
 // already hold the same descriptors and the owned-row prefix on the device.
 // d_counts[G] are overwritten.  Only kernels are enqueued (no copies, no
 // synchronisation), so the sequence can be captured into a hipGraph.
+// timed = false keeps the launch free of the optional kernel timing (which may
+// create events), for callers that must not allocate.
 lstc_status count_launch(const HostGraph *hosts, uint32_t G, const GraphDev *d_table, const uint64_t *d_row_base,
                          unsigned long long *d_counts, const lstc_tune *tune, void *ws, size_t ws_bytes,
-                         hipStream_t st);
+                         hipStream_t st, bool timed = true);
 
 // Kernel timing (lstc_timing_enable): when enabled and the stream is not being
 // captured, timing_begin / timing_end record a hipEvent pair around a launch.

@@ -904,7 +904,7 @@ size_t count_ws_bytes(uint32_t G, uint64_t total_rows, uint64_t total_m) {
 
 lstc_status count_launch(const HostGraph *hosts, uint32_t G, const GraphDev *d_table, const uint64_t *d_row_base,
                          unsigned long long *d_counts, const lstc_tune *tune, void *ws, size_t ws_bytes,
-                         hipStream_t st) {
+                         hipStream_t st, bool timed) {
     if (G == 0) return LSTC_OK;
     if (G > kMaxGraphs) return fail(LSTC_ERR_INVALID, "count: at most %u graphs per launch", kMaxGraphs);
     if (!d_counts) return fail(LSTC_ERR_INVALID, "count: NULL counts");
@@ -954,7 +954,7 @@ lstc_status count_launch(const HostGraph *hosts, uint32_t G, const GraphDev *d_t
         for (uint32_t g = 0; g < G; ++g) narrow = narrow && hosts[g].m <= (1ull << 30);
         auto kern = tr.unroll == 8 ? (narrow ? k_count<8, true> : k_count<8, false>)
                                    : (narrow ? k_count<4, true> : k_count<4, false>);
-        const TimingMark tm = timing_begin(st);
+        const TimingMark tm = timed ? timing_begin(st) : TimingMark{nullptr, nullptr};
         hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, st, gs, queue, ctl, d_counts, tr.table_scale);
         LSTC_LAUNCHED("k_count");
         timing_end(tm, st);
