@@ -146,6 +146,27 @@ def sample_matches_gpu(T, g, rows, tri):
     return int(per_row[rows.astype(np.int64)].sum(dtype=np.uint64)) == tri
 
 
+def autotuned(T, g, ws, stream, steps=5):
+    """lstc_autotune on this graph: the tune it picks and the count's time with it vs the defaults
+    (both timed the same way, plan + count kernel per step)."""
+    tune, _, _ = T.autotune([g], reps=3)
+    out = {"tune": tune.as_dict()}
+    _, ms0, k0, _ = timed_counts(T, g, steps, 1, stream, ws)
+    import torch
+    o = torch.zeros(1, dtype=torch.int64, device="cuda")
+    for _ in range(2):
+        T.count(g, out=o, ws=ws, sync=False, tune=tune, stream=stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        T.count(g, out=o, ws=ws, sync=False, tune=tune, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    out.update(default_ms=round(ms0, 4), tuned_ms=round(e0.elapsed_time(e1) / steps, 4))
+    return out
+
+
 def _replay_ms(T, graphs, steps):
     """Device time of one persistent batched launch of ``graphs`` (hipGraph replay); returns (ms, counts)."""
     import torch
@@ -488,6 +509,14 @@ def main():
 
     extra = {}
     if not args.no_extra:
+        tuned = {f"s{args.scale}": autotuned(T, g, ws, stream)}
+        n21, s21, d21 = T.generate_rmat(21, 16, seed=args.seed)
+        g21 = T.clean_orient(n21, s21, d21, ws=ws, in_lists=True)
+        del s21, d21
+        tuned["s21"] = autotuned(T, g21, ws, stream)
+        del g21
+        extra["autotune"] = tuned
+        log(f"autotune: {tuned}")
         extra["partitioned"] = partitioned(T, g, 8, args.seed, max(1, args.steps // 2), ws)
         log(f"partitioned k=8: {extra['partitioned']}")
         if extra["partitioned"]["sum_parts"] != tri:
@@ -547,6 +576,7 @@ def main():
         res["batched_vs_per_graph"] = extra["mixed"]
         res["serving"] = extra["serving"]
         res["serving_load"] = extra["serving_load"]
+        res["autotune"] = extra["autotune"]
     print(json.dumps(res))
 
 

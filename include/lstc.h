@@ -72,6 +72,15 @@ lstc_status lstc_gen_grid(uint32_t rows, uint32_t cols, double p_diag, uint64_t 
  * Halo_creation_preprocess.py:288,325,362.                                   */
 lstc_status lstc_partition_random(uint32_t n, uint32_t k, uint64_t seed, uint32_t *d_part,
                                   lstc_stream s);
+/* Contiguous blocks, Halo_creation_preprocess.py:292-300: l = n / k, part i is
+ * [i*l, (i+1)*l), the remainder (everything when n < k) stays in part 0.     */
+lstc_status lstc_partition_contig(uint32_t n, uint32_t k, uint32_t *d_part, lstc_stream s);
+/* An externally computed assignment (e.g. METIS) from a node-parts text file:
+ * one integer per vertex, the format Halo_creation_preprocess.py:308-312
+ * writes and round_clean_subgraph.cu:964-968 reads.  The first n values are
+ * uploaded into d_part[n]; a short file or a value >= k is refused with
+ * LSTC_ERR_INVALID.  Synchronises the stream.                                */
+lstc_status lstc_partition_read(const char *path, uint32_t n, uint32_t k, uint32_t *d_part, lstc_stream s);
 
 /* ------------------------------------------------------------------ (a) clean + orient
  * Replaces Med_dataset/without_part_preprocess.py:193-208 (DGL remove_self_loop,
@@ -146,12 +155,14 @@ typedef struct lstc_tune {
                               one (64..1024/table_scale); 0 = min(256, 1024/table_scale),
                               capped at 128 for a launch under 2^23 arcs              */
     uint32_t small_group;  /* small rows per work ticket (4..256), 0 = planner's choice */
-    uint32_t grid_per_cu;  /* persistent workgroups per CU (1..16), 0 = 8              */
+    uint32_t grid_per_cu;  /* persistent workgroups per CU (1..32), 0 = 32 / waves     */
     uint32_t unroll;       /* 64-entry chunks in flight per wave (4 or 8), 0 = 4       */
     uint32_t tiny_max;     /* graphs with in-lists: rows with 2 <= d+ <= tiny_max are
                               counted one lane per arc, comparing the two lists (each
                               at most tiny_max <= 8 entries) in registers; 1 = off, 0 = 8 */
-    uint32_t reserved[2];
+    uint32_t waves;        /* waves per workgroup of the count kernel: 2, 4 or 8 (workgroup
+                              size 128, 256 or 512), 0 = 4                              */
+    uint32_t reserved[1];
 } lstc_tune;
 
 /* Exact count over owned rows [0, n_owned) of an oriented CSR: the sum over

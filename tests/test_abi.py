@@ -134,7 +134,11 @@ def test_tune_resolve_host_only():
     big = _descs([(1 << 20, 1 << 23)])
     d = _resolve(lib, small)
     assert d == {"table_scale": 6, "small_max": 128, "small_group": 0, "grid_per_cu": 8, "unroll": 4,
-                 "tiny_max": 8}
+                 "tiny_max": 8, "waves": 4}
+    # the workgroup size: 2, 4 or 8 waves; the default grid keeps 32 waves per CU
+    for w, want_w, want_grid in ((2, 2, 16), (8, 8, 4), (3, 2, 16), (16, 8, 4)):
+        r = _resolve(lib, big, _lib.Tune(waves=w))
+        assert (r["waves"], r["grid_per_cu"]) == (want_w, want_grid), w
     assert _resolve(lib, big)["small_max"] == 170  # min(256, 1024 // 6)
     assert _resolve(lib, big, _lib.Tune(table_scale=2))["small_max"] == 256
     assert _resolve(lib, big, _lib.Tune(table_scale=16))["small_max"] == 64
@@ -143,10 +147,11 @@ def test_tune_resolve_host_only():
     assert _resolve(lib, small, _lib.Tune(small_max=64))["small_max"] == 64
     # clamps
     c = _resolve(lib, big, _lib.Tune(table_scale=40, grid_per_cu=99, unroll=5, tiny_max=1, small_group=1))
-    assert (c["table_scale"], c["grid_per_cu"], c["unroll"], c["tiny_max"], c["small_group"]) == (16, 16, 4, 1, 4)
+    assert (c["table_scale"], c["grid_per_cu"], c["unroll"], c["tiny_max"], c["small_group"]) == (16, 32, 4, 1, 4)
     # a resolved tune resolves to itself: replaying it runs what was resolved
     for descs in (small, big):
-        for t in (None, _lib.Tune(table_scale=4), _lib.Tune(small_max=100, unroll=8, tiny_max=4)):
+        for t in (None, _lib.Tune(table_scale=4), _lib.Tune(small_max=100, unroll=8, tiny_max=4),
+                  _lib.Tune(waves=8, grid_per_cu=3)):
             r = _resolve(lib, descs, t)
             assert _resolve(lib, descs, _lib.Tune(**r)) == r
     assert lib.lstc_tune_resolve(small, 2, None, None) == _lib.LSTC_ERR_INVALID

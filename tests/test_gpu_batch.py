@@ -90,7 +90,8 @@ def test_split_plan_run_and_graph_replay(lstc, oracle):
 
 @pytest.mark.parametrize("tune", [dict(table_scale=2), dict(table_scale=8), dict(small_max=64, small_group=4),
                                   dict(unroll=4, grid_per_cu=2), dict(small_max=512, table_scale=2),
-                                  dict(table_scale=16)])
+                                  dict(table_scale=16), dict(waves=2), dict(waves=8),
+                                  dict(waves=8, grid_per_cu=32), dict(waves=2, table_scale=2, small_max=512)])
 def test_tuning_knobs_do_not_change_counts(lstc, oracle, tune):
     gs, want = _mixed(lstc, oracle)
     t = lstc.Tune(**tune)

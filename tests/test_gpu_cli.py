@@ -94,3 +94,23 @@ def test_halo_and_partitioned_count(lstc, oracle, rmat_files, prune):
         out = _run([os.path.join(BIN, "lstc_part_tc"), "g_part.csr_node_parts_random_4_output.csr", "4"] + extra,
                    str(wd))
         assert int(re.search(r"Total Triangle : (\d+)", out).group(1)) == want
+
+
+@pytest.mark.parametrize("ext", [".txt", ".mtx"])
+def test_preprocess_snap_and_matrix_market_inputs(lstc, oracle, tmp_path, ext):
+    """lstc_preprocess reads SNAP .txt (4 header lines) and 1-based .mtx inputs to the same
+    <name>_output.csr as the oracle's cleaning of the edges they hold."""
+    n, s, d = oracle.gen_rmat(10, 8, seed=11)
+    e = np.stack([s, d], 1).astype(np.int64)
+    if ext == ".txt":
+        body = "# Undirected graph: g.txt\n# synthetic\n# Nodes: x Edges: y\n# FromNodeId\tToNodeId\n"
+        body += "".join(f"{a}\t{b}\n" for a, b in e)
+    else:
+        body = f"%%MatrixMarket matrix coordinate pattern general\n% synthetic\n{n} {n} {len(e)}\n"
+        body += "".join(f"{a + 1} {b + 1}\n" for a, b in e)
+    (tmp_path / f"g{ext}").write_text(body)
+    _run([os.path.join(BIN, "lstc_preprocess"), f"g{ext}"], str(tmp_path))
+    sym, g = oracle.clean_orient(int(e.max()) + 1, s, d)
+    v = _ints(str(tmp_path / "g_output.csr"))
+    assert v[:4] == [g.n, 2 * g.m, g.n + 1, g.m]
+    assert v[4:4 + g.n + 1] == g.off.tolist() and v[4 + g.n + 1:] == g.col.tolist()

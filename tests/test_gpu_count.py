@@ -113,7 +113,10 @@ def test_split_count_matches_oracle(lstc, oracle, name, builder):
     assert lstc.count(g) == want, name
     for t in (lstc.Tune(table_scale=2), lstc.Tune(table_scale=4, small_max=64), lstc.Tune(unroll=8),
               lstc.Tune(tiny_max=1), lstc.Tune(tiny_max=2), lstc.Tune(tiny_max=5, small_group=4),
-              lstc.Tune(table_scale=16, small_max=32)):
+              lstc.Tune(table_scale=16, small_max=32),
+              # workgroup size 128 / 512: big rows tile by 1024 / 4096 keys
+              lstc.Tune(waves=2), lstc.Tune(waves=8), lstc.Tune(waves=2, table_scale=2, unroll=8),
+              lstc.Tune(waves=8, small_max=64, grid_per_cu=1)):
         assert lstc.count(g, tune=t) == want, (name, t.as_dict())
     # a batch mixing split and plain graphs
     plain = lstc.Graph.from_numpy(og.off, og.col)

@@ -178,3 +178,62 @@ def test_halo_parts_sum_and_capacity_errors(lstc, oracle):
     assert st == -1  # LSTC_ERR_INVALID
     assert all(int(off[0].item()) == -7 for off, _ in bufs)
     assert L.lstc_halo_parts_workspace(g.n, 65, C.byref(nb)) == -1
+
+
+@pytest.mark.parametrize("n,k", [(1000, 3), (1000, 8), (7, 8), (1, 1), (4096, 64)])
+def test_partition_contig(lstc, n, k):
+    """Contiguous blocks (Halo_creation_preprocess.py:292-300): n // k vertices per part,
+    the remainder (everything when n < k) in part 0."""
+    want = np.zeros(n, np.uint32)
+    l = n // k
+    for i in range(k):
+        want[i * l:(i + 1) * l] = i
+    got = lstc.partition_contig(n, k).cpu().numpy().view(np.uint32)
+    assert np.array_equal(got, want)
+
+
+def test_partition_from_file_drives_halo(lstc, oracle, tmp_path):
+    """An externally computed assignment (here: blocks of a BFS order, as a METIS-like stand-in)
+    read from a node-parts file gives the same halo parts as the oracle with that assignment."""
+    n, s, d = oracle.gen_rmat(12, 16, seed=14)
+    g = _device_graph(lstc, n, s, d)
+    sym, _ = oracle.clean_orient(n, s, d)
+    # BFS order from vertex 0 over the symmetric CSR, cut into k blocks
+    k = 5
+    seen = np.zeros(sym.n, bool)
+    order = []
+    for root in range(sym.n):
+        if seen[root]:
+            continue
+        seen[root] = True
+        q = [root]
+        while q:
+            v = q.pop(0)
+            order.append(v)
+            for w in sym.col[sym.off[v]:sym.off[v + 1]]:
+                if not seen[w]:
+                    seen[w] = True
+                    q.append(int(w))
+    host = np.zeros(sym.n, np.uint32)
+    host[np.array(order)] = (np.arange(sym.n) * k // sym.n).astype(np.uint32)
+    path = tmp_path / "g_part.csr_node_parts_metis_5"
+    path.write_text(" ".join(str(int(x)) for x in host) + "\n")
+    dev = lstc.partition_from_file(str(path), sym.n, k)
+    assert np.array_equal(dev.cpu().numpy().view(np.uint32), host)
+    _parts_equal_oracle(lstc, oracle, g, sym, dev, host, k, 0, "metis-like")
+    # the parts still sum to the whole
+    parts = lstc.halo_extract_parts(g, dev, k)
+    assert sum(lstc.count_batch(parts)) == lstc.count(g)
+
+
+def test_partition_file_errors(lstc, tmp_path):
+    short = tmp_path / "short"
+    short.write_text("0 1 2\n")
+    with pytest.raises(lstc.LstcError, match="holds 3 values"):
+        lstc.partition_from_file(str(short), 10, 4)
+    big = tmp_path / "big"
+    big.write_text("0 1 7 2\n")
+    with pytest.raises(lstc.LstcError, match="part 7 >= k"):
+        lstc.partition_from_file(str(big), 4, 4)
+    with pytest.raises(lstc.LstcError, match="cannot open"):
+        lstc.partition_from_file(str(tmp_path / "missing"), 4, 4)

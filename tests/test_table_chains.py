@@ -32,7 +32,8 @@ PHI = 0x9E3779B1
 PHI_INV = pow(PHI, -1, 1 << 32)
 EMPTY = 0xFFFFFFFF
 NO_ID = 0xFFFFFFFE  # the id that past-the-end positions probe
-SLICE_LOG2, BIG_LOG2, BIG_TILE = 10, 12, 2048
+SLICE_LOG2 = 10
+BIG_LOG2 = {2: 11, 4: 12, 8: 13}  # workgroup table of 1024 slots per wave, by waves per workgroup
 
 
 def hquad(x, tl):
@@ -153,7 +154,7 @@ def check_table(tl, keys, threads, rng):
     return worst_insert
 
 
-@pytest.mark.parametrize("tl", [6, 7, 8, 9, 10, 11, 12])
+@pytest.mark.parametrize("tl", [6, 7, 8, 9, 10, 11, 12, 13])
 def test_chains_end_and_membership_exact(tl):
     rng = random.Random(tl)
     slots = 1 << tl
@@ -167,7 +168,7 @@ def test_chains_end_and_membership_exact(tl):
 def test_full_home_chain_wraps_around():
     """Half the slots, every key on the last bucket: the chain runs over the end of the table."""
     rng = random.Random(7)
-    for tl in (6, 10, 12):
+    for tl in (6, 10, 13):
         nb = 1 << (tl - 2)
         keys = keys_with_home(nb - 1, tl, 2 * nb, rng)
         worst = check_table(tl, keys, 64, rng)
@@ -177,15 +178,17 @@ def test_full_home_chain_wraps_around():
 def test_kernel_loads_stay_at_or_below_half():
     """Every table the kernel builds is at most half full, for every row length and table_scale
     the tuning allows (resolve_tune clamps table_scale to 2..16 and small_max to 64..1024/scale;
-    small rows take a 1024-slot slice, big rows 2048-key tiles of a 4096-slot table)."""
+    small rows take a 1024-slot slice, big rows tiles of half the workgroup's table of 1024
+    slots per wave)."""
     for scale in range(2, 17):
         small_max = max(64, 1024 // scale)
         for k in range(2, small_max + 1):
             tl = table_log2_for(k, scale, SLICE_LOG2)
             assert 2 * k <= (1 << tl), (scale, k, tl)
-        for tk in range(1, BIG_TILE + 1):
-            tl = table_log2_for(tk, scale, BIG_LOG2)
-            assert 2 * tk <= (1 << tl), (scale, tk, tl)
+        for waves, big in BIG_LOG2.items():
+            for tk in range(1, (1 << big) // 2 + 1):  # big rows are tiled by half the table
+                tl = table_log2_for(tk, scale, big)
+                assert 2 * tk <= (1 << tl), (scale, waves, tk, tl)
 
 
 def test_hash_matches_kernel_arithmetic():

@@ -8,6 +8,7 @@ from ._lib import LstcError, Tune, GraphDesc, load, check, header_functions  # n
 from .core import (BatchPlan, Graph, Server, Workspace, clean_orient, count, count_batch,  # noqa: F401
                    generate_chung_lu,
                    generate_er, generate_grid, generate_rmat, halo_extract, halo_extract_parts, partition_random,
+                   partition_contig, partition_from_file,
                    stream_handle, autotune, shape_class, tune_resolve, work_stats, KernelTimer, selftest, in_lists, segments)
 
 __version__ = "1.0.0"

@@ -33,7 +33,8 @@ class Tune(C.Structure):
     """lstc_tune (include/lstc.h)."""
     _fields_ = [("table_scale", C.c_uint32), ("small_max", C.c_uint32),
                 ("small_group", C.c_uint32), ("grid_per_cu", C.c_uint32),
-                ("unroll", C.c_uint32), ("tiny_max", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+                ("unroll", C.c_uint32), ("tiny_max", C.c_uint32), ("waves", C.c_uint32),
+                ("reserved", C.c_uint32 * 1)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "reserved"}
@@ -103,6 +104,8 @@ SIGNATURES = {
     "lstc_gen_chung_lu": (ST, [U32, DBL, DBL, DBL, U64, U64, VP, VP, VP, SZ, VP]),
     "lstc_gen_grid": (ST, [U32, U32, DBL, U64, VP, VP, VP]),
     "lstc_partition_random": (ST, [U32, U32, U64, VP, VP]),
+    "lstc_partition_contig": (ST, [U32, U32, VP, VP]),
+    "lstc_partition_read": (ST, [C.c_char_p, U32, U32, VP, VP]),
     "lstc_clean_orient_workspace": (ST, [U32, U64, PSZ]),
     "lstc_clean_orient": (ST, [VP, VP, U64, U32, I32, VP, VP, VP, VP, VP, VP, PU32, PU64, VP, SZ, VP]),
     "lstc_halo_workspace": (ST, [U32, PSZ]),

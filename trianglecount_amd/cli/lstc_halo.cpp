@@ -35,39 +35,57 @@ int main(int argc, char **argv) {
             fprintf(stderr, "in_deg[%llu] differs from its row length: unsupported input\n", (unsigned long long)v);
             return 1;
         }
-    std::vector<uint32_t> part;
-    if (!lstc_text::read_node_parts(argv[3], g.n, part, err)) {
-        fprintf(stderr, "Error opening file: %s (%s)\n", argv[3], err.c_str());
-        return 1;
-    }
     const double t0 = now_s();
     uint64_t *d_off = upload(g.off);
     uint32_t *d_col = upload(g.col);
-    uint32_t *d_part = upload(part);
+    uint32_t *d_part = nullptr;
+    CLI_HIP(hipMalloc((void **)&d_part, sizeof(uint32_t) * (g.n ? g.n : 1)));
+    if (lstc_partition_read(argv[3], (uint32_t)g.n, (uint32_t)k, d_part, nullptr) != LSTC_OK) {
+        fprintf(stderr, "Error opening file: %s (%s)\n", argv[3], lstc_last_error());
+        return 1;
+    }
     Workspace ws;
     size_t need = 0;
-    CLI_CHECK(lstc_halo_workspace((uint32_t)g.n, &need));
-    void *w = ws.get(need);
+    std::vector<uint32_t> nl(k), no(k);
+    std::vector<uint64_t> ml(k);
+    std::vector<lstc_halo_out> outs(k);
+    auto alloc_out = [&](uint64_t p) {
+        lstc_halo_out &o = outs[p];
+        o = lstc_halo_out{};
+        CLI_HIP(hipMalloc((void **)&o.off, sizeof(uint64_t) * ((size_t)nl[p] + 1)));
+        CLI_HIP(hipMalloc((void **)&o.col, sizeof(uint32_t) * (ml[p] ? ml[p] : 1)));
+        o.n_local_cap = nl[p];
+        o.m_local_cap = ml[p];
+    };
+    if (k <= 64) {  // every part at once: each pass over the graph serves all k parts
+        CLI_CHECK(lstc_halo_parts_workspace((uint32_t)g.n, (uint32_t)k, &need));
+        void *w = ws.get(need);
+        CLI_CHECK(lstc_halo_parts_sizes(d_off, d_col, (uint32_t)g.n, d_part, (uint32_t)k, mode, nl.data(), no.data(),
+                                        ml.data(), w, ws.bytes, nullptr));
+        for (uint64_t p = 0; p < k; ++p) alloc_out(p);
+        CLI_CHECK(lstc_halo_parts_extract(d_off, d_col, (uint32_t)g.n, d_part, (uint32_t)k, mode, outs.data(), w,
+                                          ws.bytes, nullptr));
+    } else {  // one part per call
+        CLI_CHECK(lstc_halo_workspace((uint32_t)g.n, &need));
+        void *w = ws.get(need);
+        for (uint64_t p = 0; p < k; ++p) {
+            CLI_CHECK(lstc_halo_sizes(d_off, d_col, (uint32_t)g.n, d_part, (uint32_t)p, mode, &nl[p], &no[p], &ml[p],
+                                      w, ws.bytes, nullptr));
+            alloc_out(p);
+            CLI_CHECK(lstc_halo_extract(d_off, d_col, (uint32_t)g.n, d_part, (uint32_t)p, mode, outs[p].off,
+                                        outs[p].col, nullptr, nl[p], ml[p], w, ws.bytes, nullptr));
+        }
+    }
     std::vector<lstc_text::PartCsr> parts(k);
     for (uint64_t p = 0; p < k; ++p) {
-        uint32_t nl = 0, no = 0;
-        uint64_t ml = 0;
-        CLI_CHECK(lstc_halo_sizes(d_off, d_col, (uint32_t)g.n, d_part, (uint32_t)p, mode, &nl, &no, &ml, w, ws.bytes,
-                                  nullptr));
-        uint64_t *o = nullptr;
-        uint32_t *c = nullptr;
-        CLI_HIP(hipMalloc((void **)&o, sizeof(uint64_t) * ((size_t)nl + 1)));
-        CLI_HIP(hipMalloc((void **)&c, sizeof(uint32_t) * (ml ? ml : 1)));
-        CLI_CHECK(lstc_halo_extract(d_off, d_col, (uint32_t)g.n, d_part, (uint32_t)p, mode, o, c, nullptr, nl, ml, w,
-                                    ws.bytes, nullptr));
-        parts[p].n_local = nl;
-        parts[p].n_owned = no;
-        parts[p].off = download(o, (size_t)nl + 1);
-        parts[p].col = download(c, ml);
-        (void)hipFree(o);
-        (void)hipFree(c);
-        printf("%llu th Partitions: owned %u, with halo %u, arcs %llu\n", (unsigned long long)p, no, nl,
-               (unsigned long long)ml);
+        parts[p].n_local = nl[p];
+        parts[p].n_owned = no[p];
+        parts[p].off = download(outs[p].off, (size_t)nl[p] + 1);
+        parts[p].col = download(outs[p].col, ml[p]);
+        (void)hipFree(outs[p].off);
+        (void)hipFree(outs[p].col);
+        printf("%llu th Partitions: owned %u, with halo %u, arcs %llu\n", (unsigned long long)p, no[p], nl[p],
+               (unsigned long long)ml[p]);
     }
     const double t1 = now_s();
     const std::string out = std::string(argv[3]) + "_output.csr";

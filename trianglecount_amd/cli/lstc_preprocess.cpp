@@ -98,10 +98,8 @@ int main(int argc, char **argv) {
             const std::vector<uint32_t> part = download(d_part, n_out);
             const std::string pr = o2 + "_node_parts_random_" + std::to_string(k);
             lstc_text::write_node_parts(pr.c_str(), part.data(), n_out);
-            std::vector<uint32_t> contig(n_out, 0);
-            const uint64_t l = n_out / k;
-            for (uint64_t i = 0; i < k && l; ++i)
-                for (uint64_t v = i * l; v < (i + 1) * l; ++v) contig[v] = (uint32_t)i;
+            CLI_CHECK(lstc_partition_contig(n_out, k, d_part, nullptr));
+            const std::vector<uint32_t> contig = download(d_part, n_out);
             const std::string pc = o2 + "_node_parts_contig_" + std::to_string(k);
             lstc_text::write_node_parts(pc.c_str(), contig.data(), n_out);
             printf("wrote %s, %s\n", pr.c_str(), pc.c_str());

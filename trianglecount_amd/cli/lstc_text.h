@@ -12,8 +12,8 @@
 //   partitioned CSR "<node_parts>_output.csr": `V E`, then per part
 //                 `Num_Nodes Num_Nodes+1 Num_Edges |P|` / row_ptr / col
 //                 (writer round_clean_subgraph.cu:371,850-865; reader src/WithPartition/without_sorting_tc.cu:242-286)
-//   edge lists    .tsv (GraphChallenge: src dst weight), .txt (SNAP, '#' comments),
-//                 .mtx (Matrix Market coordinate, 1-based)
+//   edge lists    .tsv (GraphChallenge: src dst weight), .txt (SNAP, 4 header lines),
+//                 .mtx / .mmio (Matrix Market coordinate, 1-based)
 //                 (loaders Med_dataset/without_part_preprocess.py:126-177)
 #pragma once
 
@@ -211,25 +211,43 @@ inline bool write_partitioned(const char *path, uint64_t V, uint64_t E, const st
     return true;
 }
 
-// Raw edge list -> (n, src, dst).  .mtx is 1-based (Matrix Market); .tsv/.txt are 0-based.
+// Raw edge list -> (n, src, dst), as the reference's loaders read it
+// (Med_dataset/without_part_preprocess.py:126-177):
+//   .tsv          GraphChallenge "src<TAB>dst<TAB>weight", ids as written (:135-144)
+//   .txt          SNAP: the first 4 lines are skipped (pandas skiprows=4, :145-154),
+//                 then "src<TAB>dst", ids as written
+//   .mtx / .mmio  Matrix Market coordinate: '%' lines, one size line
+//                 "rows cols nnz", then 1-based "row col [value]" (scipy mmread ->
+//                 0-based COO, :126-134, :155-163)
+// n = largest id + 1, as dgl.graph((u, v)) counts nodes.  Lines starting with
+// '#' or '%' are skipped in every format.  Other extensions are refused, as the
+// reference refuses them (:173-175).
 inline bool read_edge_list(const char *path, uint64_t &n, std::vector<uint32_t> &src, std::vector<uint32_t> &dst,
                            std::string &err) {
     std::string p(path);
     const size_t dot = p.find_last_of('.');
     const std::string ext = dot == std::string::npos ? "" : p.substr(dot);
+    const bool mtx = ext == ".mtx" || ext == ".mmio", txt = ext == ".txt", tsv = ext == ".tsv";
+    if (!mtx && !txt && !tsv) {
+        err = "unsupported file type " + ext + ": only .tsv (GraphChallenge), .txt (SNAP), .mtx / .mmio (Matrix Market)";
+        return false;
+    }
     FILE *f = fopen(path, "r");
     if (!f) { err = "cannot open " + p; return false; }
     char line[4096];
-    uint64_t maxid = 0;
-    bool first = true, any = false;
-    const bool mtx = ext == ".mtx" || ext == ".mmio";
+    uint64_t maxid = 0, lineno = 0;
+    bool size_line = mtx, any = false;
     while (fgets(line, sizeof(line), f)) {
-        if (line[0] == '%' || line[0] == '#' || line[0] == '\n') continue;
+        if (txt && lineno++ < 4) continue;  // SNAP header (skiprows=4)
+        if (line[0] == '%' || line[0] == '#' || line[0] == '\n' || line[0] == '\r') continue;
         unsigned long long a, b;
         if (sscanf(line, "%llu %llu", &a, &b) != 2) continue;
-        if (mtx && first) { first = false; continue; }  // size line: rows cols nnz
-        first = false;
-        if (mtx) { if (a == 0 || b == 0) { err = "0 index in .mtx"; fclose(f); return false; } --a; --b; }
+        if (size_line) { size_line = false; continue; }  // rows cols nnz
+        if (mtx) {
+            if (a == 0 || b == 0) { err = "0 index in a 1-based Matrix Market file"; fclose(f); return false; }
+            --a;
+            --b;
+        }
         if (a >= 0xFFFFFFFEull || b >= 0xFFFFFFFEull) { err = "vertex id too large"; fclose(f); return false; }
         src.push_back((uint32_t)a);
         dst.push_back((uint32_t)b);

@@ -176,6 +176,21 @@ def partition_random(n: int, k: int, seed: int = 1, stream=None) -> torch.Tensor
     return part[:n]
 
 
+def partition_contig(n: int, k: int, stream=None) -> torch.Tensor:
+    """Contiguous blocks of n // k vertices, the remainder in part 0 (lstc_partition_contig)."""
+    part = empty_u32(n)
+    check(load().lstc_partition_contig(n, k, _ptr(part), C.c_void_p(stream_handle(stream))), "lstc_partition_contig")
+    return part[:n]
+
+
+def partition_from_file(path: str, n: int, k: int, stream=None) -> torch.Tensor:
+    """An externally computed assignment (e.g. METIS) from a node-parts file (lstc_partition_read)."""
+    part = empty_u32(n)
+    check(load().lstc_partition_read(str(path).encode(), n, k, _ptr(part), C.c_void_p(stream_handle(stream))),
+          "lstc_partition_read")
+    return part[:n]
+
+
 # ------------------------------------------------------------------ (a) clean + orient
 def clean_orient(n: int, src: torch.Tensor, dst: torch.Tensor, compact: bool = True,
                  keep_symmetric: bool = False, in_lists: bool = False, ws: Workspace | None = None,

@@ -79,6 +79,7 @@ struct TuneR {
     uint32_t grid_per_cu;   // persistent workgroups per CU
     uint32_t unroll;        // chunks in flight per wave (4 or 8)
     uint32_t tiny_max;      // rows with 2 <= d+ <= tiny_max (graphs with in-lists): lane-per-arc path
+    uint32_t waves;         // waves per workgroup of the count kernel (2, 4 or 8)
 };
 TuneR resolve_tune(const lstc_tune *t);
 // The tuning a launch of total_m arcs runs with: resolve_tune plus the

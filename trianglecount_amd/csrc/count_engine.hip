@@ -22,11 +22,12 @@
 // smaller ones form "small" tickets of small_group items (by default chosen by
 // the planner from the small items per workgroup).  k_count reads it from ctl.
 //
-// Count (k_count, one persistent launch): each workgroup of kWaves waves takes
+// Count (k_count, one persistent launch): each workgroup of WAVES waves (2, 4 or
+// 8: the workgroup size is a tuning knob) takes
 // tickets from an atomic head.
-//   big item  : the whole workgroup builds one LDS hash table (up to 4096 slots,
-//               load <= 1/2; rows above 2048 keys are tiled) and the waves split
-//               the 64-segment windows round-robin;
+//   big item  : the whole workgroup builds one LDS hash table (up to 1024 slots
+//               per wave: 4096 with 4 waves; load <= 1/2, longer rows are tiled)
+//               and the waves split the 64-segment windows round-robin;
 //   small item: one wave, a private 1024-slot slice of the same LDS.
 // Walk of one window: one segment per lane (id, start, length), inclusive scan
 // of lengths, empty segments compacted out; then the flat sequence of entries
@@ -37,6 +38,9 @@
 // loaded (global loads, never FLAT) before round r is probed, lockstep over its
 // UNROLL chunks.  Hits are counted per lane and reduced across the wave per
 // item; the workgroup issues one uint64 atomic per graph it counted.
+#include <mutex>
+#include <unordered_map>
+
 #include "count_engine.h"
 #include "wave.h"
 
@@ -44,12 +48,14 @@ namespace lstc {
 
 static constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 static constexpr uint32_t kNoId = 0xFFFFFFFEu;  // never a vertex id (ids < 2^32 - 2), never kEmpty
-static constexpr int kWaves = 4;
 static constexpr uint32_t kSliceLog2 = 10;
 static constexpr uint32_t kSlice = 1u << kSliceLog2;  // wave-private slice (small items)
-static constexpr uint32_t kBigLog2 = 12;
-static constexpr uint32_t kBigCap = 1u << kBigLog2;   // shared table (big items)
-static constexpr uint32_t kBigTile = kBigCap / 2;     // keys per table fill of a big row
+// LDS table of a workgroup: one slice per wave; a big item uses all of it as one
+// table (2^big_log2 slots), filled with at most half as many keys per tile.
+template <int WAVES>
+constexpr uint32_t big_log2() { return kSliceLog2 + (WAVES >= 8 ? 3 : WAVES >= 4 ? 2 : 1); }
+template <int WAVES>
+constexpr uint32_t tab_slots() { return 1u << big_log2<WAVES>(); }
 static constexpr int kBins = 6;                       // hash-path bins
 static constexpr int kTinyBin = kBins;                // the lane-per-arc rows
 static constexpr int kSlots = kBins + 1;              // queue regions
@@ -75,7 +81,7 @@ TuneR resolve_tune(const lstc_tune *t) {
     // defaults: the fastest point of a (table_scale, small_max, grid_per_cu, unroll)
     // sweep at RMAT scales 16, 21 and 23 on MI355X (table_scale 6: rows of up to
     // 170 keys keep a wave-private table)
-    TuneR r{6, 256, 0, 0, 4, kTinyMax};  // small_group 0: chosen by the planner
+    TuneR r{6, 256, 0, 0, 4, kTinyMax, 4};  // small_group 0: chosen by the planner
     if (t) {
         if (t->tiny_max) r.tiny_max = t->tiny_max == 1 ? 0u : t->tiny_max;  // 1 = off
         if (t->table_scale) r.table_scale = t->table_scale;
@@ -83,14 +89,17 @@ TuneR resolve_tune(const lstc_tune *t) {
         if (t->small_group) r.small_group = t->small_group;
         if (t->grid_per_cu) r.grid_per_cu = t->grid_per_cu;
         if (t->unroll) r.unroll = t->unroll;
+        if (t->waves) r.waves = t->waves;
     }
     r.table_scale = hmax(2u, hmin(16u, r.table_scale));
     // a small row must fit its slice at the chosen load factor
     r.small_max = hmax(64u, hmin(kSlice / r.table_scale, r.small_max));
     if (r.small_group) r.small_group = hmax(4u, hmin(256u, r.small_group));
     r.unroll = r.unroll >= 8 ? 8 : 4;
-    if (r.grid_per_cu == 0) r.grid_per_cu = 8;  // k_count<4> fits 8 workgroups per CU (LDS and VGPRs)
-    r.grid_per_cu = hmax(1u, hmin(16u, r.grid_per_cu));
+    r.waves = r.waves >= 8 ? 8 : r.waves >= 4 ? 4 : 2;
+    // default: 32 waves per CU (k_count fits them in LDS and VGPRs), 8 workgroups of 4 waves
+    if (r.grid_per_cu == 0) r.grid_per_cu = 32 / r.waves;
+    r.grid_per_cu = hmax(1u, hmin(32u, r.grid_per_cu));
     r.tiny_max = hmin(kTinyMax, r.tiny_max);
     if (r.tiny_max < 2) r.tiny_max = 0;
     return r;
@@ -277,7 +286,7 @@ __global__ __launch_bounds__(256) void k_plan_rows(GraphSet gs, Bounds bd, uint3
 // 32-64 small tickets per workgroup (measured on RMAT s21 and s23: fewer,
 // larger tickets leave a ragged tail, more of them queue on the ticket head).
 __global__ __launch_bounds__(1024) void k_plan_scan(uint32_t *__restrict__ wcnt, uint32_t *__restrict__ ctl,
-                                                    uint32_t small_group, uint32_t grid) {
+                                                    uint32_t small_group, uint32_t grid, uint32_t waves) {
     constexpr uint32_t kPer = kPlanEntries / 1024;
     static_assert(kPlanEntries % 1024 == 0, "plan entries per thread");
     __shared__ uint32_t s_wave[16];
@@ -319,7 +328,7 @@ __global__ __launch_bounds__(1024) void k_plan_scan(uint32_t *__restrict__ wcnt,
         }
         ctl[C_SGROUP] = sg;
         const uint32_t small_t = (n_small + sg - 1) / sg;
-        const uint32_t tiny_t = (n_tiny + kWaves * 64 - 1) / (kWaves * 64);
+        const uint32_t tiny_t = (n_tiny + waves * 64 - 1) / (waves * 64);  // 64 rows per wave
         ctl[C_NBIG] = n_big;
         ctl[C_NSMALL] = n_small;
         ctl[C_NSMALLT] = small_t;
@@ -759,19 +768,22 @@ __device__ __forceinline__ uint32_t tiny_walk(const GraphDev &G, bool mine, bool
 
 // ------------------------------------------------------------------ persistent count kernel
 // NARROW: every graph of the launch has m <= 2^30 arcs.
-template <int UNROLL, bool NARROW>
-__global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(GraphSet gs, const unsigned long long *__restrict__ queue,
+// Occupancy target (the second launch bound is waves per SIMD): 8 with UNROLL 4,
+// 6 with UNROLL 8, whatever the workgroup size.
+template <int UNROLL, bool NARROW, int WAVES>
+__global__ __launch_bounds__(WAVES * 64, UNROLL == 4 ? 8 : 6) void k_count(GraphSet gs, const unsigned long long *__restrict__ queue,
                                                        uint32_t *__restrict__ ctl,
                                                        unsigned long long *__restrict__ counts, uint32_t scale) {
-    __shared__ __align__(16) uint32_t s_tab[kBigCap];
-    __shared__ uint32_t s_ovf[kBigCap / 128];  // overflow bits, one per 4-slot bucket
-    __shared__ uint32_t s_base[kWaves][64];
-    __shared__ uint32_t s_end[kWaves][64];
-    __shared__ __align__(16) uint32_t s_flag[kWaves][16];  // 64 position flags per wave (tiny rows)
-    __shared__ uint8_t s_src[kWaves][64];
+    constexpr uint32_t kTab = tab_slots<WAVES>();
+    __shared__ __align__(16) uint32_t s_tab[kTab];
+    __shared__ uint32_t s_ovf[kTab / 128];  // overflow bits, one per 4-slot bucket
+    __shared__ uint32_t s_base[WAVES][64];
+    __shared__ uint32_t s_end[WAVES][64];
+    __shared__ __align__(16) uint32_t s_flag[WAVES][16];  // 64 position flags per wave (tiny rows)
+    __shared__ uint8_t s_src[WAVES][64];
     __shared__ uint32_t s_ticket;
-    __shared__ unsigned long long s_acc[kWaves];
-    __shared__ uint32_t s_g[kWaves];
+    __shared__ unsigned long long s_acc[WAVES];
+    __shared__ uint32_t s_g[WAVES];
 
     // item and graph fields are wave-uniform: they live in scalar registers
     const uint32_t tid = threadIdx.x, wv = uniform(tid >> 6), lane = tid & 63;
@@ -800,17 +812,18 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(Grap
             const uint32_t K = (uint32_t)(sload(G.off + u + 1) - rb);
             const Segs sg = item_segs(G, kind, chunk, u, rb, K);
             unsigned long long hits = 0;
+            constexpr uint32_t kBigTile = kTab / 2;  // keys per table fill of a big row
             for (uint32_t t0 = 0; t0 < K; t0 += kBigTile) {
                 const uint32_t tk = min(K - t0, kBigTile);
-                const uint32_t tl = table_log2_for(tk, scale, kBigLog2);
-                for (uint32_t i = tid; i < (1u << tl); i += kWaves * 64) s_tab[i] = kEmpty;
-                if (tid < kBigCap / 128) s_ovf[tid] = 0;
+                const uint32_t tl = table_log2_for(tk, scale, big_log2<WAVES>());
+                for (uint32_t i = tid; i < (1u << tl); i += WAVES * 64) s_tab[i] = kEmpty;
+                if (tid < kTab / 128) s_ovf[tid] = 0;
                 __syncthreads();
-                table_insert(s_tab, s_ovf, tl, G.col, rb + t0, tk, tid, kWaves * 64);
+                table_insert(s_tab, s_ovf, tl, G.col, rb + t0, tk, tid, WAVES * 64);
                 __syncthreads();
-                table_mark(s_tab, s_ovf, tl, tid, kWaves * 64);
+                table_mark(s_tab, s_ovf, tl, tid, WAVES * 64);
                 __syncthreads();
-                hits += walk<UNROLL, NARROW>(G.off, G.col, G.n_rows, sg.ids, sg.seg, sg.n, sg.below, wv * 64, kWaves * 64, s_tab, tl,
+                hits += walk<UNROLL, NARROW>(G.off, G.col, G.n_rows, sg.ids, sg.seg, sg.n, sg.below, wv * 64, WAVES * 64, s_tab, tl,
                                      s_base[wv], s_end[wv], lane);
                 __syncthreads();
             }
@@ -824,7 +837,7 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(Grap
           }
         } else if (t >= small_end) {
             // tiny rows: 64 per wave, lane-per-arc; the lanes are split by graph
-            const uint32_t e0 = tiny0 + (t - small_end) * (kWaves * 64) + wv * 64;
+            const uint32_t e0 = tiny0 + (t - small_end) * (WAVES * 64) + wv * 64;
             const uint32_t e = e0 + lane;
             const bool valid = e < tiny_end;
             Item itm{0, 0, 0, 0};
@@ -845,11 +858,11 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(Grap
                 acc += hits;
             }
         } else {
-            // small items: each wave takes every kWaves-th item of the group
+            // small items: each wave takes every WAVES-th item of the group
             const uint32_t first = n_big + (t - n_bigt) * small_group;
             const uint32_t last = min(first + small_group, n_big + n_small);
             uint32_t *tab = s_tab + wv * kSlice;
-            for (uint32_t q = first + wv; q < last; q += kWaves) {
+            for (uint32_t q = first + wv; q < last; q += WAVES) {
                 const Item itm = decode(sload(queue + q));
                 const uint32_t u = itm.row, chunk = itm.chunk, kind = itm.kind, g = itm.g;
                 const GraphDev G = graph_at<true>(gs, g);
@@ -879,10 +892,10 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(Grap
     if (lane == 0) { s_acc[wv] = acc; s_g[wv] = cur_g; }
     __syncthreads();
     if (tid == 0) {
-        for (int i = 0; i < kWaves; ++i) {
+        for (int i = 0; i < WAVES; ++i) {
             if (s_g[i] == kEmpty) continue;
             unsigned long long sum = s_acc[i];
-            for (int j = i + 1; j < kWaves; ++j)
+            for (int j = i + 1; j < WAVES; ++j)
                 if (s_g[j] == s_g[i]) { sum += s_acc[j]; s_g[j] = kEmpty; }
             if (sum) atomicAdd(counts + s_g[i], sum);
         }
@@ -890,6 +903,21 @@ __global__ __launch_bounds__(kWaves * 64, UNROLL == 4 ? 8 : 6) void k_count(Grap
 }
 
 // ------------------------------------------------------------------ host side
+// Workgroups of `kern` that fit on one CU at once (registers, LDS), cached per
+// kernel; 0 when the runtime cannot tell.
+static uint32_t resident_blocks(const void *kern, int threads) {
+    static std::mutex mu;
+    static std::unordered_map<const void *, uint32_t> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(kern);
+    if (it != cache.end()) return it->second;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kern, threads, 0) != hipSuccess || blocks < 0) blocks = 0;
+    (void)hipGetLastError();
+    cache[kern] = (uint32_t)blocks;
+    return (uint32_t)blocks;
+}
+
 // queue entries: per row one kind A item and at most 1 + in-degree / kInChunk kind B items
 static uint64_t queue_cap(uint64_t total_rows, uint64_t total_m) { return 2 * total_rows + total_m / kInChunk + 1; }
 
@@ -943,19 +971,28 @@ lstc_status count_launch(const HostGraph *hosts, uint32_t G, const GraphDev *d_t
         hipLaunchKernelGGL(k_plan_rows<0>, dim3(kPlanBlocks), dim3(256), 0, st, gs, bd, wcnt, queue);
         LSTC_LAUNCHED("k_plan_rows<0>");
         hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(1024), 0, st, wcnt, ctl, tr.small_group,
-                           (uint32_t)cus * tr.grid_per_cu);
+                           (uint32_t)cus * tr.grid_per_cu, tr.waves);
         LSTC_LAUNCHED("k_plan_scan");
         hipLaunchKernelGGL(k_plan_rows<1>, dim3(kPlanBlocks), dim3(256), 0, st, gs, bd, wcnt, queue);
         LSTC_LAUNCHED("k_plan_rows<1>");
     }
     if (total) {
-        const uint32_t grid = (uint32_t)cus * tr.grid_per_cu;
+        uint32_t grid = (uint32_t)cus * tr.grid_per_cu;
         bool narrow = true;
         for (uint32_t g = 0; g < G; ++g) narrow = narrow && hosts[g].m <= (1ull << 30);
-        auto kern = tr.unroll == 8 ? (narrow ? k_count<8, true> : k_count<8, false>)
-                                   : (narrow ? k_count<4, true> : k_count<4, false>);
+        using Kern = void (*)(GraphSet, const unsigned long long *, uint32_t *, unsigned long long *, uint32_t);
+        // [unroll 4/8][narrow][waves 2/4/8]
+        static const Kern kerns[2][2][3] = {
+            {{k_count<4, false, 2>, k_count<4, false, 4>, k_count<4, false, 8>},
+             {k_count<4, true, 2>, k_count<4, true, 4>, k_count<4, true, 8>}},
+            {{k_count<8, false, 2>, k_count<8, false, 4>, k_count<8, false, 8>},
+             {k_count<8, true, 2>, k_count<8, true, 4>, k_count<8, true, 8>}}};
+        const Kern kern = kerns[tr.unroll == 8][narrow][tr.waves == 2 ? 0 : tr.waves == 4 ? 1 : 2];
+        // a persistent grid beyond what fits at once would only wait for the queue to drain
+        const uint32_t fit = resident_blocks((const void *)kern, tr.waves * 64);
+        if (fit && grid > (uint32_t)cus * fit) grid = (uint32_t)cus * fit;
         const TimingMark tm = timed ? timing_begin(st) : TimingMark{nullptr, nullptr};
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, st, gs, queue, ctl, d_counts, tr.table_scale);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(tr.waves * 64), 0, st, gs, queue, ctl, d_counts, tr.table_scale);
         LSTC_LAUNCHED("k_count");
         timing_end(tm, st);
     }

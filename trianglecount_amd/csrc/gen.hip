@@ -1,7 +1,10 @@
-// gen.hip -- seeded synthetic graph generators and the random k-way partition.
+// gen.hip -- seeded synthetic graph generators and the k-way vertex partitions
+// (seeded random, contiguous blocks, read from a node-parts file).
 // Counter-based (SplitMix64 finaliser) so every sample is a pure function of
 // (seed, stream, index); the CPU oracle implements the same definitions.
 #include <math.h>
+#include <stdio.h>
+
 #include <vector>
 
 #include "lstc_internal.h"
@@ -196,6 +199,57 @@ extern "C" lstc_status lstc_gen_grid(uint32_t rows, uint32_t cols, double p_diag
     hipLaunchKernelGGL(k_gen_grid, dim3(gen_grid_dim(nv)), dim3(256), 0, (hipStream_t)s, rows, cols,
                        prob_to_u64(p_diag), stream_key(seed, 4), d_src, d_dst);
     LSTC_LAUNCHED("k_gen_grid");
+    return LSTC_OK;
+}
+
+// Contiguous blocks (preprocess_GPU/generate_clean_undir_dataset/
+// Halo_creation_preprocess.py:292-300): l = n // k, part i holds [i*l, (i+1)*l),
+// the remainder (and everything when n < k) stays in part 0.
+__global__ void k_partition_contig(uint32_t n, uint32_t k, uint32_t l, uint32_t *__restrict__ part) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = l ? v / l : 0;
+        part[v] = i < k ? (uint32_t)i : 0u;
+    }
+}
+
+extern "C" lstc_status lstc_partition_contig(uint32_t n, uint32_t k, uint32_t *d_part, lstc_stream s) {
+    if (k == 0) return fail(LSTC_ERR_INVALID, "lstc_partition_contig: k must be > 0");
+    if (n == 0) return LSTC_OK;
+    if (!d_part) return fail(LSTC_ERR_INVALID, "lstc_partition_contig: NULL output");
+    hipLaunchKernelGGL(k_partition_contig, dim3(gen_grid_dim(n)), dim3(256), 0, (hipStream_t)s, n, k, n / k, d_part);
+    LSTC_LAUNCHED("k_partition_contig");
+    return LSTC_OK;
+}
+
+// A node-parts file (one integer per vertex, whitespace-separated: the format
+// Halo_creation_preprocess.py:308-312 writes and round_clean_subgraph.cu:964-968
+// reads), e.g. an assignment computed by METIS outside this library.  The first
+// n values are read (as the reference reads them); a value >= k is refused,
+// since that vertex would belong to no part.
+extern "C" lstc_status lstc_partition_read(const char *path, uint32_t n, uint32_t k, uint32_t *d_part,
+                                           lstc_stream s) {
+    if (!path) return fail(LSTC_ERR_INVALID, "lstc_partition_read: NULL path");
+    if (k == 0) return fail(LSTC_ERR_INVALID, "lstc_partition_read: k must be > 0");
+    if (n && !d_part) return fail(LSTC_ERR_INVALID, "lstc_partition_read: NULL output");
+    FILE *f = fopen(path, "r");
+    if (!f) return fail(LSTC_ERR_INVALID, "lstc_partition_read: cannot open %s", path);
+    std::vector<uint32_t> part(n);
+    for (uint32_t v = 0; v < n; ++v) {
+        unsigned long long x = 0;
+        if (fscanf(f, "%llu", &x) != 1) {
+            fclose(f);
+            return fail(LSTC_ERR_INVALID, "lstc_partition_read: %s holds %u values, %u vertices expected", path, v, n);
+        }
+        if (x >= k) {
+            fclose(f);
+            return fail(LSTC_ERR_INVALID, "lstc_partition_read: vertex %u has part %llu >= k = %u", v, x, k);
+        }
+        part[v] = (uint32_t)x;
+    }
+    fclose(f);
+    if (n == 0) return LSTC_OK;
+    LSTC_HIP(hipMemcpyAsync(d_part, part.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, (hipStream_t)s));
+    LSTC_HIP(hipStreamSynchronize((hipStream_t)s));  // the host buffer goes out of scope
     return LSTC_OK;
 }
 

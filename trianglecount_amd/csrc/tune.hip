@@ -63,6 +63,7 @@ extern "C" lstc_status lstc_tune_resolve(const lstc_graph_desc *graphs, uint32_t
     t.grid_per_cu = r.grid_per_cu;
     t.unroll = r.unroll;
     t.tiny_max = r.tiny_max ? r.tiny_max : 1u;
+    t.waves = r.waves;
     *out = t;
     return LSTC_OK;
 }
@@ -109,19 +110,26 @@ extern "C" lstc_status lstc_autotune(const lstc_graph_desc *graphs, uint32_t G, 
     const uint32_t cand_scale[] = {4, 6, 8, 16};
     const uint32_t cand_small[] = {64, 128, 256};
     const uint32_t cand_group[] = {4, 16, 32, 64};
-    const uint32_t cand_grid[] = {4, 6, 8};
+    const uint32_t cand_grid[] = {4, 6, 8};  // for 4 waves; scaled by 4 / waves below
     const uint32_t cand_unroll[] = {4, 8};
     const uint32_t cand_tiny[] = {1, 4, 8};  // 1 = lane-per-arc path off
+    const uint32_t cand_waves[] = {2, 4, 8}; // workgroup size 128 / 256 / 512 (the reference swept 64..512)
     struct Knob { uint32_t lstc_tune::*field; const uint32_t *vals; int nvals; };
-    const Knob knobs[] = {{&lstc_tune::table_scale, cand_scale, 4}, {&lstc_tune::small_max, cand_small, 3},
-                          {&lstc_tune::small_group, cand_group, 4}, {&lstc_tune::grid_per_cu, cand_grid, 3},
-                          {&lstc_tune::unroll, cand_unroll, 2}, {&lstc_tune::tiny_max, cand_tiny, 3}};
+    // waves first: grid_per_cu is then searched for the chosen workgroup size
+    const Knob knobs[] = {{&lstc_tune::waves, cand_waves, 3},      {&lstc_tune::table_scale, cand_scale, 4},
+                          {&lstc_tune::small_max, cand_small, 3},  {&lstc_tune::small_group, cand_group, 4},
+                          {&lstc_tune::grid_per_cu, cand_grid, 3}, {&lstc_tune::unroll, cand_unroll, 2},
+                          {&lstc_tune::tiny_max, cand_tiny, 3}};
     for (const Knob &k : knobs) {
         if (rc != LSTC_OK) break;
         lstc_tune base = cur;
         for (int i = 0; i < k.nvals && rc == LSTC_OK; ++i) {
             lstc_tune t = base;
             t.*(k.field) = k.vals[i];
+            if (k.field == &lstc_tune::grid_per_cu) {  // same waves per CU whatever the workgroup size
+                const uint32_t w = t.waves ? t.waves : 4;
+                t.grid_per_cu = hmax(1u, k.vals[i] * 4 / w);
+            }
             if (memcmp(&t, &cur, sizeof(t)) == 0) continue;
             double ms = 0;
             rc = measure(t, &ms, got);
@@ -156,12 +164,13 @@ extern "C" lstc_status lstc_tune_cache_lookup(const char *path, uint32_t shape_c
     lstc_tune t{};
     while (fgets(line, sizeof(line), f)) {
         char d[128];
-        unsigned cls, a, b, c, g, u, y;
+        unsigned cls, a, b, c, g, u, y, w;
         double ms;
-        if (sscanf(line, "%127s %u %u %u %u %u %u %u %lf", d, &cls, &a, &b, &c, &g, &u, &y, &ms) != 9) continue;
+        if (sscanf(line, "%127s %u %u %u %u %u %u %u %u %lf", d, &cls, &a, &b, &c, &g, &u, &y, &w, &ms) != 10) continue;
         if (strcmp(d, dev) != 0 || cls != shape_class) continue;
         t = lstc_tune{};
         t.table_scale = a; t.small_max = b; t.small_group = c; t.grid_per_cu = g; t.unroll = u; t.tiny_max = y;
+        t.waves = w;
         found = true;  // the last matching line wins
     }
     fclose(f);
@@ -176,8 +185,8 @@ extern "C" lstc_status lstc_tune_cache_store(const char *path, uint32_t shape_cl
     device_key(dev, sizeof(dev));
     FILE *f = fopen(path, "a");
     if (!f) return fail(LSTC_ERR_INVALID, "lstc_tune_cache_store: cannot open %s", path);
-    fprintf(f, "%s\t%u\t%u\t%u\t%u\t%u\t%u\t%u\t%.6f\n", dev, shape_class, t->table_scale, t->small_max,
-            t->small_group, t->grid_per_cu, t->unroll, t->tiny_max, ms);
+    fprintf(f, "%s\t%u\t%u\t%u\t%u\t%u\t%u\t%u\t%u\t%.6f\n", dev, shape_class, t->table_scale, t->small_max,
+            t->small_group, t->grid_per_cu, t->unroll, t->tiny_max, t->waves, ms);
     fclose(f);
     return LSTC_OK;
 }
