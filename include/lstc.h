@@ -207,6 +207,17 @@ lstc_status lstc_count_split_static(const uint64_t *d_off, const uint32_t *d_col
                                     uint32_t n_rows, uint32_t n_owned, uint64_t m, uint64_t *d_count,
                                     const lstc_tune *tune, lstc_stream s);
 
+/* Per-edge triangle support and per-vertex triangle counts (local clustering,
+ * k-truss), the step after the reference's per-source partials g_sum[u]
+ * (src/WithoutPartition/ShareKernelTCV62_update_v1.cu:175-186).  Over the
+ * triangles whose source row is owned (u < n_owned; all of them when
+ * n_owned = n_rows): d_support[m] (uint32, may be NULL) receives for arc j of
+ * the oriented CSR the number of triangles containing that edge, d_per_vertex
+ * [n_rows] (uint64, may be NULL) the number containing each vertex.  Each sums
+ * to 3x the count.  Stream-ordered, no workspace.                            */
+lstc_status lstc_triangle_support(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows, uint32_t n_owned,
+                                  uint64_t m, uint32_t *d_support, uint64_t *d_per_vertex, lstc_stream s);
+
 /* Target-side in-lists of an oriented CSR for split counting: for every row w,
  * the owned sources u < n_owned of arcs u -> w with d+(u) <= d+(w) (row lengths
  * of this CSR), ascending.  d_in_off[n_rows+1], d_in_col[m] (at most m entries);

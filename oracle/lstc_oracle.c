@@ -336,6 +336,39 @@ uint64_t lo_count_merge(uint32_t n_rows, uint32_t n_owned, const uint64_t *off,
     return total;
 }
 
+/* Per-edge support and per-vertex triangle counts (the outputs the reference's
+ * per-source partials g_sum[u] lead to, ShareKernelTCV62_update_v1.cu:175-186):
+ * every triangle u -> w -> x with u -> x and u < n_owned (found, as in
+ * lo_count_merge, by merging N+(u) with N+(w)) adds one to the support of its
+ * three arcs u->w, u->x, w->x (indexed by their positions in col) and one to
+ * the count of each of its three vertices.  Serial: the outputs are plain sums. */
+void lo_triangle_support(uint32_t n_rows, uint32_t n_owned, const uint64_t *off, const uint32_t *col,
+                         uint32_t *support, uint64_t *per_vertex)
+{
+    const uint64_t m = off[n_rows];
+    if (support) memset(support, 0, sizeof(uint32_t) * m);
+    if (per_vertex) memset(per_vertex, 0, sizeof(uint64_t) * n_rows);
+    for (uint32_t u = 0; u < n_owned; ++u) {
+        const uint64_t b0 = off[u], e0 = off[u + 1];
+        for (uint64_t j = b0; j < e0; ++j) {
+            const uint32_t w = col[j];
+            uint64_t p = b0, q = off[w];
+            const uint64_t qe = off[w + 1];
+            while (p < e0 && q < qe) {
+                const uint32_t x = col[p], y = col[q];
+                if (x < y) ++p;
+                else if (y < x) ++q;
+                else {
+                    if (support) { support[j]++; support[p]++; support[q]++; }
+                    if (per_vertex) { per_vertex[u]++; per_vertex[w]++; per_vertex[x]++; }
+                    ++p;
+                    ++q;
+                }
+            }
+        }
+    }
+}
+
 /* The same merge counter over an explicit list of rows (a bounded sample). */
 uint64_t lo_count_merge_list(const uint64_t *off, const uint32_t *col, const uint32_t *rows,
                              uint64_t nrows, int threads)

@@ -21,6 +21,9 @@
  *                         (two different algorithms), closed forms.
  *   lo_halo_extract     : brute-force definition on small graphs, and the
  *                         invariant sum over parts == unpartitioned count.
+ *   lo_triangle_support : support of edge {u,w} = |N(u) n N(w)| over the symmetric
+ *                         graph, per-vertex counts = networkx triangles, closed forms
+ *                         (K_n, wheel), sums = 3 x lo_count_merge.
  *   lo_partition_random : determinism, range, balance.
  */
 #ifndef LSTC_ORACLE_H
@@ -76,6 +79,12 @@ uint64_t lo_count_merge_list(const uint64_t *off, const uint32_t *col, const uin
  * (tiles of 128 staged entries, endpoint-then-bisection search).          */
 uint64_t lo_count_rows_k1(uint32_t n_rows, uint32_t n_owned, const uint64_t *off,
                           const uint32_t *col, uint64_t *g_sum);
+
+/* Per-edge support (support[m], by arc position) and per-vertex triangle counts
+ * (per_vertex[n_rows]) over the triangles whose source row is owned; either
+ * output may be NULL.                                                      */
+void lo_triangle_support(uint32_t n_rows, uint32_t n_owned, const uint64_t *off, const uint32_t *col,
+                         uint32_t *support, uint64_t *per_vertex);
 
 /* ---- partition and (b) halo extraction ---- */
 void lo_partition_random(uint32_t n, uint32_t k, uint64_t seed, uint32_t *part);

@@ -70,6 +70,8 @@ def lib():
         L.lo_count_merge_list.argtypes = [_u64p, _u32p, _u32p, C.c_uint64, C.c_int]
         L.lo_count_rows_k1.restype = C.c_uint64
         L.lo_count_rows_k1.argtypes = [C.c_uint32, C.c_uint32, _u64p, _u32p, C.c_void_p]
+        L.lo_triangle_support.restype = None
+        L.lo_triangle_support.argtypes = [C.c_uint32, C.c_uint32, _u64p, _u32p, C.c_void_p, C.c_void_p]
         L.lo_partition_random.restype = None
         L.lo_partition_random.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, _u32p]
         L.lo_halo_extract.restype = None
@@ -190,6 +192,15 @@ def count_k1(g: CSR, n_owned: int | None = None):
     gs = np.zeros(max(no, 1), np.uint64)
     t = int(lib().lo_count_rows_k1(g.n, no, g.off, g.col, _ptr(gs)))
     return t, gs[:no]
+
+
+def triangle_support(g: CSR, n_owned: int | None = None):
+    """lo_triangle_support: (support[m] uint32 by arc position, per_vertex[n] uint64)."""
+    no = g.n if n_owned is None else int(n_owned)
+    sup = np.zeros(max(g.m, 1), np.uint32)
+    pv = np.zeros(max(g.n, 1), np.uint64)
+    lib().lo_triangle_support(g.n, no, g.off, g.col, _ptr(sup), _ptr(pv))
+    return sup[: g.m], pv[: g.n]
 
 
 # ---------------------------------------------------------------- partition/halo

@@ -322,3 +322,43 @@ def test_stats_match_numpy(oracle):
         st = oracle.stats(g)
         assert st == {"n": g.n, "m": g.m, "max_out": int(dout.max()), "sum_out_sq": int((dout * dout).sum()),
                       "sum_in_out": int((din * dout).sum()), "rows_ge2": int((dout >= 2).sum())}
+
+
+def _support_by_common_neighbours(sym, g):
+    """support of every oriented arc (u, w) = |N(u) n N(w)| in the symmetric graph (textbook identity)."""
+    nbr = [set(sym.col[sym.off[v]:sym.off[v + 1]].tolist()) for v in range(sym.n)]
+    src = np.repeat(np.arange(g.n), np.diff(g.off.astype(np.int64)))
+    return np.array([len(nbr[u] & nbr[w]) for u, w in zip(src.tolist(), g.col.tolist())], np.uint32), nbr
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_triangle_support_matches_common_neighbours_and_networkx(oracle, seed):
+    n, s, d = G.random_multigraph(300, 2500, seed=seed)
+    sym, g = oracle.clean_orient(n, s, d)
+    sup, pv = oracle.triangle_support(g)
+    want, nbr = _support_by_common_neighbours(sym, g)
+    assert np.array_equal(sup, want)
+    nxg = nx.Graph()
+    nxg.add_nodes_from(range(sym.n))
+    nxg.add_edges_from((u, w) for u in range(sym.n) for w in nbr[u] if u < w)
+    tri = nx.triangles(nxg)
+    assert pv.tolist() == [tri[v] for v in range(sym.n)]
+    total = oracle.count(g)
+    assert int(sup.sum()) == 3 * total and int(pv.sum()) == 3 * total
+
+
+@pytest.mark.parametrize("k", [3, 4, 9, 30])
+def test_triangle_support_closed_forms(oracle, k):
+    _, g = oracle.clean_orient(*G.complete(k))
+    sup, pv = oracle.triangle_support(g)
+    assert set(sup.tolist()) == {k - 2} and set(pv.tolist()) == {(k - 1) * (k - 2) // 2}
+    # wheel with r rim vertices: spokes carry 2 triangles, rim edges 1; the hub is in r, rim vertices in 2
+    r = max(k, 4)
+    sym, g = oracle.clean_orient(*G.wheel(r))
+    sup, pv = oracle.triangle_support(g)
+    deg = np.diff(sym.off.astype(np.int64))
+    hub = int(np.argmax(deg))
+    src = np.repeat(np.arange(g.n), np.diff(g.off.astype(np.int64)))
+    spoke = (src == hub) | (g.col == hub)
+    assert set(sup[spoke].tolist()) == {2} and set(sup[~spoke].tolist()) == {1}
+    assert pv[hub] == r and set(np.delete(pv, hub).tolist()) == {2}

@@ -119,6 +119,7 @@ SIGNATURES = {
     "lstc_count_split": (ST, [VP, VP, VP, VP, VP, VP, U32, U32, U64, VP, C.POINTER(Tune), VP, SZ, VP]),
     "lstc_count_split_static_limit": (SZ, []),
     "lstc_count_split_static": (ST, [VP, VP, VP, VP, VP, VP, U32, U32, U64, VP, C.POINTER(Tune), VP]),
+    "lstc_triangle_support": (ST, [VP, VP, U32, U32, U64, VP, VP, VP]),
     "lstc_in_lists_workspace": (ST, [U32, U32, U64, PSZ]),
     "lstc_in_lists": (ST, [VP, VP, U32, U32, U64, VP, VP, PU64, VP, SZ, VP]),
     "lstc_segments": (ST, [VP, VP, U32, U64, VP, U64, VP, VP, VP]),

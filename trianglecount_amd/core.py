@@ -289,6 +289,19 @@ def count(g: Graph, n_owned: int | None = None, per_row: bool = False, tune: Tun
     return total
 
 
+def triangle_support(g: Graph, n_owned: int | None = None, per_edge: bool = True, per_vertex: bool = True,
+                     stream=None):
+    """Per-edge support (uint32 per arc of g.col) and per-vertex triangle counts (uint64 per row),
+    over the triangles whose source row is owned (lstc_triangle_support).  Returns device tensors
+    (support, per_vertex), either None when not asked for."""
+    no = g.owned if n_owned is None else int(n_owned)
+    sup = empty_u32(g.m) if per_edge else None
+    pv = empty_u64(g.n) if per_vertex else None
+    check(load().lstc_triangle_support(_ptr(g.off), _ptr(g.col), g.n, no, g.m, _ptr(sup), _ptr(pv),
+                                       C.c_void_p(stream_handle(stream))), "lstc_triangle_support")
+    return (sup[: g.m] if sup is not None else None), (pv[: g.n] if pv is not None else None)
+
+
 # ------------------------------------------------------------------ (b) halo
 def halo_extract(g: Graph, part: torch.Tensor, p: int, mode: int = 0, with_l2g: bool = False,
                  ws: Workspace | None = None, stream=None, split: bool = False) -> Graph:
