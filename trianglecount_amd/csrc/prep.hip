@@ -8,12 +8,16 @@
 //   keys  : each raw pair (s, d), s != d, emits (s<<b | d) and (d<<b | s);
 //           loops and out-of-range ids emit a sentinel that sorts last
 //   sort  : LSD radix sort over 2b bits (prims.hip)
-//   unique: flag first-of-run, scan, scatter  -> symmetric arc list (sorted)
-//   degree: per-source atomic count           -> deg[v]
+//   then three compactions of the sorted list, each a count pass (per tile of
+//   4096 keys) + a scan of the tile counts + a scatter pass that recomputes the
+//   predicate and writes the kept entries at their final positions (no flag or
+//   position arrays); the scatter also records each row's first / end position:
+//   unique: first of each run          -> symmetric arc list, row bounds, deg[v]
 //   relabel (compact): id' = rank of v among vertices with deg > 0 (monotone)
 //   symmetric CSR: offsets = scan(deg) compacted, columns relabelled
-//   orient: keep s->d iff deg s < deg d or (equal and d < s); flag, scan,
-//           scatter keeps each row ascending; offsets = scan of kept per row.
+//   orient: keep s->d iff deg s < deg d or (equal and d < s)
+//           -> oriented columns (rows ascending), d+(v), offsets = scan(d+)
+//   in    : reverse keys of kept arcs with d+(d) <= d+(s) -> target-side in-lists
 #include "lstc_internal.h"
 #include "wave.h"
 
@@ -47,41 +51,173 @@ __global__ void k_make_keys(const uint32_t *__restrict__ src, const uint32_t *__
     }
 }
 
-__global__ void k_unique_flags(const uint64_t *__restrict__ k, uint64_t nk, uint32_t *__restrict__ f) {
-    GRID_STRIDE(i, nk) {
-        const uint64_t x = k[i];
-        f[i] = (x != kSentinel && (i == 0 || k[i - 1] != x)) ? 1u : 0u;
+// ------------------------------------------------------------------ compactions
+// Tile = 256 threads x 16 rounds = 4096 keys; key (round r, thread t) is tile
+// position r * 256 + t, so loads are coalesced and the order inside the tile is
+// (round, wave, lane).  The scatter pass ranks the kept keys of a tile in that
+// order from one ballot per round and a 64-entry prefix over (round, wave).
+static constexpr uint32_t kCT = 256;
+static constexpr uint32_t kCR = 16;
+static constexpr uint64_t kCTile = (uint64_t)kCT * kCR;
+
+template <class Op>
+__global__ __launch_bounds__(kCT) void k_compact_count(Op op, uint32_t *__restrict__ tile_cnt) {
+    __shared__ uint32_t s_w[kCT / 64];
+    const uint64_t n = op.size();
+    const uint64_t tile0 = (uint64_t)blockIdx.x * kCTile;
+    uint32_t c = 0;
+    if (tile0 < n) {
+#pragma unroll 4
+        for (uint32_t r = 0; r < kCR; ++r) {
+            const uint64_t i = tile0 + (uint64_t)r * kCT + threadIdx.x;
+            if (i < n && op.pred(i)) ++c;
+        }
+    }
+    c = wave_sum32(c);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+template <class Op>
+__global__ __launch_bounds__(kCT) void k_compact_scatter(Op op, const uint64_t *__restrict__ tile_off) {
+    __shared__ uint32_t s_pre[kCR * (kCT / 64)];  // kept keys per (round, wave), then their exclusive prefix
+    const uint64_t n = op.size();
+    const uint64_t tile0 = (uint64_t)blockIdx.x * kCTile;
+    if (tile0 >= n) return;  // the whole block leaves together
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint64_t kept[kCR];
+#pragma unroll
+    for (uint32_t r = 0; r < kCR; ++r) {
+        const uint64_t i = tile0 + (uint64_t)r * kCT + threadIdx.x;
+        kept[r] = ballot(i < n && op.pred(i));
+        if (lane == 0) s_pre[r * (kCT / 64) + wv] = (uint32_t)__popcll(kept[r]);
+    }
+    __syncthreads();
+    if (wv == 0) {
+        const uint32_t v = s_pre[lane];
+        const uint32_t incl = wave_incl_scan(v, lane);
+        s_pre[lane] = incl - v;
+    }
+    __syncthreads();
+    const uint64_t base = tile_off[blockIdx.x];
+#pragma unroll
+    for (uint32_t r = 0; r < kCR; ++r) {
+        const uint64_t i = tile0 + (uint64_t)r * kCT + threadIdx.x;
+        if (i < n) op.emit(i, lane_in(kept[r]), base + s_pre[r * (kCT / 64) + wv] + mask_rank(kept[r]));
     }
 }
 
-__global__ void k_unique_scatter(const uint64_t *__restrict__ k, uint64_t nk,
-                                 const uint32_t *__restrict__ f, const uint64_t *__restrict__ pos,
-                                 uint64_t *__restrict__ uk) {
-    GRID_STRIDE(i, nk) if (f[i]) uk[pos[i]] = k[i];
+// The row of key i starts (ends) there when the previous (next) key has another row.
+__device__ __forceinline__ bool row_first(const uint64_t *k, uint64_t i, uint32_t b, uint64_t s) {
+    return i == 0 || (k[i - 1] >> b) != s;
+}
+__device__ __forceinline__ bool row_last(const uint64_t *k, uint64_t i, uint64_t n, uint32_t b, uint64_t s) {
+    return i + 1 == n || (k[i + 1] >> b) != s;
 }
 
-// Row boundaries of the sorted symmetric arc list: first[s], last[s] (exclusive).
-// Vertices without arcs keep first == last == 0 (pre-filled).  No atomics: a hub
-// row is one run of the sorted list, so its degree is last - first.
-__global__ void k_row_bounds(const uint64_t *__restrict__ uk, const uint64_t *__restrict__ d_m2, uint32_t b,
-                             uint64_t *__restrict__ first, uint64_t *__restrict__ last) {
-    const uint64_t m2 = *d_m2;
-    GRID_STRIDE(i, m2) {
-        const uint64_t s = uk[i] >> b;
-        if (i == 0 || (uk[i - 1] >> b) != s) first[s] = i;
-        if (i + 1 == m2 || (uk[i + 1] >> b) != s) last[s] = i + 1;
+// Unique keys of the sorted list (sentinels dropped), with each row's bounds
+// [first, last) in the unique list.
+struct UniqueOp {
+    const uint64_t *keys;
+    uint64_t n;
+    uint32_t b;
+    uint64_t *uk, *first, *last;
+    __device__ uint64_t size() const { return n; }
+    __device__ bool pred(uint64_t i) const {
+        const uint64_t k = keys[i];
+        return k != kSentinel && (i == 0 || keys[i - 1] != k);
     }
+    __device__ void emit(uint64_t i, bool kept, uint64_t pos) const {
+        const uint64_t k = keys[i];
+        if (k == kSentinel) return;
+        if (kept) uk[pos] = k;
+        const uint64_t s = k >> b;
+        if (row_first(keys, i, b, s)) first[s] = pos;  // a row's first key is always kept
+        if (row_last(keys, i, n, b, s)) last[s] = pos + (kept ? 1u : 0u);
+    }
+};
+
+// Oriented arcs (keep s -> d iff deg s < deg d, or equal and d < s), written as
+// relabelled ids; end[s] - start[s] = d+(s).
+struct OrientOp {
+    const uint64_t *uk;
+    const uint64_t *d_m2;
+    uint32_t b;
+    uint64_t mask;
+    const uint2 *dd;  // {deg, d+} per vertex
+    const uint32_t *nid;
+    uint32_t *col;
+    uint64_t *start, *end;
+    __device__ uint64_t size() const { return *d_m2; }
+    __device__ bool pred(uint64_t i) const {
+        const uint64_t k = uk[i];
+        const uint32_t s = (uint32_t)(k >> b), d = (uint32_t)(k & mask);
+        const uint32_t ds = dd[s].x, dg = dd[d].x;
+        return ds < dg || (ds == dg && d < s);
+    }
+    __device__ void emit(uint64_t i, bool kept, uint64_t pos) const {
+        const uint64_t k = uk[i], s = k >> b;
+        if (kept) col[pos] = nid[k & mask];
+        const uint64_t n = *d_m2;
+        if (row_first(uk, i, b, s)) start[s] = pos;
+        if (row_last(uk, i, n, b, s)) end[s] = pos + (kept ? 1u : 0u);
+    }
+};
+
+// Target-side in-lists: in row s, the sources d of arcs d -> s (the reverse of
+// an oriented arc) with d+(d) <= d+(s): the arcs the split count handles from
+// the target side.  Rows stay ascending.
+struct InOp {
+    const uint64_t *uk;
+    const uint64_t *d_m2;
+    uint32_t b;
+    uint64_t mask;
+    const uint2 *dd;
+    const uint32_t *nid;
+    uint32_t *col;
+    uint64_t *start, *end;
+    __device__ uint64_t size() const { return *d_m2; }
+    __device__ bool pred(uint64_t i) const {
+        const uint64_t k = uk[i];
+        const uint32_t s = (uint32_t)(k >> b), d = (uint32_t)(k & mask);
+        const uint2 vs = dd[s], vd = dd[d];  // one 8-byte gather per endpoint
+        const bool out_arc = vs.x < vd.x || (vs.x == vd.x && d < s);
+        return !out_arc && vd.y <= vs.y;
+    }
+    __device__ void emit(uint64_t i, bool kept, uint64_t pos) const {
+        const uint64_t k = uk[i], s = k >> b;
+        if (kept) col[pos] = nid[k & mask];
+        const uint64_t n = *d_m2;
+        if (row_first(uk, i, b, s)) start[s] = pos;
+        if (row_last(uk, i, n, b, s)) end[s] = pos + (kept ? 1u : 0u);
+    }
+};
+
+template <class Op>
+static lstc_status compact_pass(const Op &op, uint64_t n_max, uint32_t *tile_cnt, uint64_t *tile_off, uint64_t *d_total,
+                           void *scan_ws, size_t scan_bytes, hipStream_t st) {
+    const uint64_t tiles = (n_max + kCTile - 1) / kCTile;
+    if (tiles == 0) return fill_u64(d_total, 0, 1, st);
+    hipLaunchKernelGGL(k_compact_count<Op>, dim3((uint32_t)tiles), dim3(kCT), 0, st, op, tile_cnt);
+    LSTC_LAUNCHED("k_compact_count");
+    LSTC_TRY(scan_exclusive_u32_to_u64(tile_cnt, tile_off, tiles, d_total, scan_ws, scan_bytes, st));
+    hipLaunchKernelGGL(k_compact_scatter<Op>, dim3((uint32_t)tiles), dim3(kCT), 0, st, op, (const uint64_t *)tile_off);
+    LSTC_LAUNCHED("k_compact_scatter");
+    return LSTC_OK;
 }
 
-__global__ void k_degree(const uint64_t *__restrict__ first, const uint64_t *__restrict__ last, uint32_t n,
-                         uint32_t *__restrict__ deg) {
-    GRID_STRIDE(v, n) deg[v] = (uint32_t)(last[v] - first[v]);
-}
-
-// kept arcs per row from the exclusive scan of keep flags at the row boundaries
-__global__ void k_out_degree(const uint64_t *__restrict__ first, const uint64_t *__restrict__ last,
-                             const uint64_t *__restrict__ pos, uint32_t n, uint32_t *__restrict__ dout) {
-    GRID_STRIDE(v, n) dout[v] = (uint32_t)(pos[last[v]] - pos[first[v]]);
+// Row lengths from the compactions' row bounds; deg also goes into dd[v].x, d+ into dd[v].y.
+__global__ void k_row_lengths(const uint64_t *__restrict__ first, const uint64_t *__restrict__ last, uint32_t n,
+                              uint32_t *__restrict__ len, uint2 *__restrict__ dd, int field) {
+    GRID_STRIDE(v, n) {
+        const uint32_t l = (uint32_t)(last[v] - first[v]);
+        len[v] = l;
+        if (dd) {
+            if (field == 0) dd[v] = make_uint2(l, 0u);
+            else dd[v].y = l;
+        }
+    }
 }
 
 __global__ void k_nonzero(const uint32_t *__restrict__ deg, uint32_t n, int compact,
@@ -112,46 +248,6 @@ __global__ void k_sym_cols(const uint64_t *__restrict__ uk, const uint64_t *__re
     GRID_STRIDE(i, m2) sym_col[i] = nid[uk[i] & mask];
 }
 
-// keep flags for [0, nk]; entries from m2 on (stale from the unique pass) become 0
-__global__ void k_orient_flags(const uint64_t *__restrict__ uk, const uint64_t *__restrict__ d_m2,
-                               uint64_t nk, uint32_t b, uint64_t mask, const uint32_t *__restrict__ deg,
-                               uint32_t *__restrict__ f) {
-    const uint64_t m2 = *d_m2;
-    GRID_STRIDE(i, nk + 1) {
-        if (i >= m2) { f[i] = 0u; continue; }
-        const uint64_t k = uk[i];
-        const uint32_t s = (uint32_t)(k >> b), d = (uint32_t)(k & mask);
-        const uint32_t ds = deg[s], dd = deg[d];
-        f[i] = (ds < dd || (ds == dd && d < s)) ? 1u : 0u;
-    }
-}
-
-// In-arcs d -> s (the reverse keys of row s) with d+(d) <= d+(s): the arcs the
-// count handles from the target side.  Flags for [0, nk]; zero from m2 on.
-__global__ void k_in_flags(const uint64_t *__restrict__ uk, const uint64_t *__restrict__ d_m2, uint64_t nk,
-                           uint32_t b, uint64_t mask, const uint32_t *__restrict__ deg,
-                           const uint32_t *__restrict__ dout, uint32_t *__restrict__ f) {
-    const uint64_t m2 = *d_m2;
-    GRID_STRIDE(i, nk + 1) {
-        if (i >= m2) { f[i] = 0u; continue; }
-        const uint64_t k = uk[i];
-        const uint32_t s = (uint32_t)(k >> b), d = (uint32_t)(k & mask);
-        const uint32_t ds = deg[s], dd = deg[d];
-        const bool out_arc = ds < dd || (ds == dd && d < s);
-        uint32_t fl = 0;
-        if (!out_arc) fl = dout[d] <= dout[s] ? 1u : 0u;  // out-degrees only read for in-arcs
-        f[i] = fl;
-    }
-}
-
-__global__ void k_orient_scatter(const uint64_t *__restrict__ uk, const uint64_t *__restrict__ d_m2,
-                                 uint64_t mask, const uint32_t *__restrict__ f,
-                                 const uint64_t *__restrict__ pos, const uint32_t *__restrict__ nid,
-                                 uint32_t *__restrict__ col) {
-    const uint64_t m2 = *d_m2;
-    GRID_STRIDE(i, m2) if (f[i]) col[pos[i]] = nid[uk[i] & mask];
-}
-
 __global__ void k_orient_offsets(const uint32_t *__restrict__ keepv, const uint64_t *__restrict__ nidscan,
                                  const uint64_t *__restrict__ doscan, uint32_t n,
                                  const uint64_t *__restrict__ d_n_out, const uint64_t *__restrict__ d_m,
@@ -161,11 +257,13 @@ __global__ void k_orient_offsets(const uint32_t *__restrict__ keepv, const uint6
 }
 
 struct PrepLayout {
-    uint64_t *keys, *alt, *pos;
-    uint32_t *flags;
+    uint64_t *keys, *alt;
+    uint32_t *tile_cnt;
+    uint64_t *tile_off;
     uint32_t *deg, *keepv, *nid, *dout, *din;
+    uint2 *dd;
     uint64_t *degscan, *nidscan, *doscan, *first, *last, *discan;
-    uint64_t *scal;  // [0]=m2 [1]=n_out [2]=m [3]=unused
+    uint64_t *scal;  // [0]=m2 [1]=n_out [2]=m [3]=in-list entries
     uint32_t *bad;
     void *sort_ws;
     size_t sort_bytes;
@@ -178,15 +276,17 @@ static PrepLayout prep_layout(void *ws, size_t ws_bytes, uint32_t n, uint64_t m_
     Carver c(ws, ws_bytes);
     PrepLayout L;
     const uint64_t nk = 2 * m_raw;
+    const uint64_t tiles = (nk + kCTile - 1) / kCTile + 1;
     L.keys = c.take<uint64_t>(nk + 1);
     L.alt = c.take<uint64_t>(nk + 1);
-    L.pos = c.take<uint64_t>(nk + 1);
-    L.flags = c.take<uint32_t>(nk + 1);
+    L.tile_cnt = c.take<uint32_t>(tiles);
+    L.tile_off = c.take<uint64_t>(tiles);
     L.deg = c.take<uint32_t>((size_t)n + 1);
     L.keepv = c.take<uint32_t>((size_t)n + 1);
     L.nid = c.take<uint32_t>((size_t)n + 1);
     L.dout = c.take<uint32_t>((size_t)n + 1);
     L.din = c.take<uint32_t>((size_t)n + 1);
+    L.dd = c.take<uint2>((size_t)n + 1);
     L.discan = c.take<uint64_t>((size_t)n + 1);
     L.degscan = c.take<uint64_t>((size_t)n + 1);
     L.nidscan = c.take<uint64_t>((size_t)n + 1);
@@ -197,7 +297,7 @@ static PrepLayout prep_layout(void *ws, size_t ws_bytes, uint32_t n, uint64_t m_
     L.bad = c.take<uint32_t>(4);
     L.sort_bytes = radix_sort_ws_bytes(nk);
     L.sort_ws = c.take<char>(L.sort_bytes);
-    L.scan_bytes = scan_ws_bytes(nk + 1 > n ? nk + 1 : n);
+    L.scan_bytes = scan_ws_bytes(tiles > (uint64_t)n + 1 ? tiles : (uint64_t)n + 1);
     L.scan_ws = c.take<char>(L.scan_bytes);
     L.bytes = c.need();
     return L;
@@ -245,17 +345,13 @@ lstc_status lstc::clean_orient_launch(const uint32_t *d_src, const uint32_t *d_d
         uint64_t *sorted = nullptr;
         LSTC_TRY(radix_sort_u64(L.keys, L.alt, nk, (int)(2 * b > 64 ? 64 : 2 * b), &sorted, L.sort_ws, L.sort_bytes, st));
         uint64_t *uk = (sorted == L.keys) ? L.alt : L.keys;
-        hipLaunchKernelGGL(k_unique_flags, dim3(grid_k), dim3(256), 0, st, sorted, nk, L.flags);
-        LSTC_LAUNCHED("k_unique_flags");
-        LSTC_TRY(scan_exclusive_u32_to_u64(L.flags, L.pos, nk, L.scal + 0, L.scan_ws, L.scan_bytes, st));
-        hipLaunchKernelGGL(k_unique_scatter, dim3(grid_k), dim3(256), 0, st, sorted, nk, L.flags, L.pos, uk);
-        LSTC_LAUNCHED("k_unique_scatter");
+        // unique arcs and the symmetric row bounds (m2 -> scal[0])
         LSTC_TRY(fill_u64(L.first, 0, (uint64_t)n + 1, st));
         LSTC_TRY(fill_u64(L.last, 0, (uint64_t)n + 1, st));
-        hipLaunchKernelGGL(k_row_bounds, dim3(grid_k), dim3(256), 0, st, uk, L.scal + 0, b, L.first, L.last);
-        LSTC_LAUNCHED("k_row_bounds");
-        hipLaunchKernelGGL(k_degree, dim3(grid_n), dim3(256), 0, st, L.first, L.last, n, L.deg);
-        LSTC_LAUNCHED("k_degree");
+        LSTC_TRY(compact_pass(UniqueOp{sorted, nk, b, uk, L.first, L.last}, nk, L.tile_cnt, L.tile_off, L.scal + 0,
+                         L.scan_ws, L.scan_bytes, st));
+        hipLaunchKernelGGL(k_row_lengths, dim3(grid_n), dim3(256), 0, st, L.first, L.last, n, L.deg, L.dd, 0);
+        LSTC_LAUNCHED("k_row_lengths(deg)");
         hipLaunchKernelGGL(k_nonzero, dim3(grid_n), dim3(256), 0, st, L.deg, n, compact, L.keepv);
         LSTC_LAUNCHED("k_nonzero");
         LSTC_TRY(scan_exclusive_u32_to_u64(L.keepv, L.nidscan, n, L.scal + 1, L.scan_ws, L.scan_bytes, st));
@@ -267,31 +363,26 @@ lstc_status lstc::clean_orient_launch(const uint32_t *d_src, const uint32_t *d_d
             hipLaunchKernelGGL(k_sym_cols, dim3(grid_k), dim3(256), 0, st, uk, L.scal + 0, mask, L.nid, d_sym_col);
             LSTC_LAUNCHED("k_sym_cols");
         }
-        // k_orient_flags also zeroes the flags in [m2, nk] left over from the unique pass
-        hipLaunchKernelGGL(k_orient_flags, dim3(grid_k), dim3(256), 0, st, uk, L.scal + 0, nk, b, mask, L.deg,
-                           L.flags);
-        LSTC_LAUNCHED("k_orient_flags");
-        LSTC_TRY(scan_exclusive_u32_to_u64(L.flags, L.pos, nk + 1, L.scal + 2, L.scan_ws, L.scan_bytes, st));
-        hipLaunchKernelGGL(k_out_degree, dim3(grid_n), dim3(256), 0, st, L.first, L.last, L.pos, n, L.dout);
-        LSTC_LAUNCHED("k_out_degree");
-        hipLaunchKernelGGL(k_orient_scatter, dim3(grid_k), dim3(256), 0, st, uk, L.scal + 0, mask, L.flags,
-                           L.pos, L.nid, d_col);
-        LSTC_LAUNCHED("k_orient_scatter");
+        // oriented arcs (m -> scal[2]) and d+ per vertex
+        LSTC_TRY(fill_u64(L.first, 0, (uint64_t)n + 1, st));
+        LSTC_TRY(fill_u64(L.last, 0, (uint64_t)n + 1, st));
+        LSTC_TRY(compact_pass(OrientOp{uk, L.scal + 0, b, mask, L.dd, L.nid, d_col, L.first, L.last}, nk, L.tile_cnt,
+                         L.tile_off, L.scal + 2, L.scan_ws, L.scan_bytes, st));
+        hipLaunchKernelGGL(k_row_lengths, dim3(grid_n), dim3(256), 0, st, L.first, L.last, n, L.dout, L.dd, 1);
+        LSTC_LAUNCHED("k_row_lengths(d+)");
         LSTC_TRY(scan_exclusive_u32_to_u64(L.dout, L.doscan, n, nullptr, L.scan_ws, L.scan_bytes, st));
         hipLaunchKernelGGL(k_orient_offsets, dim3(grid_n), dim3(256), 0, st, L.keepv, L.nidscan, L.doscan, n,
                            L.scal + 1, L.scal + 2, d_off);
         LSTC_LAUNCHED("k_orient_offsets");
         if (d_in_off) {
-            // in-lists of the arcs counted from the target side (same sorted key list, no atomics)
-            hipLaunchKernelGGL(k_in_flags, dim3(grid_k), dim3(256), 0, st, uk, L.scal + 0, nk, b, mask, L.deg,
-                               L.dout, L.flags);
-            LSTC_LAUNCHED("k_in_flags");
-            LSTC_TRY(scan_exclusive_u32_to_u64(L.flags, L.pos, nk + 1, L.scal + 3, L.scan_ws, L.scan_bytes, st));
-            hipLaunchKernelGGL(k_orient_scatter, dim3(grid_k), dim3(256), 0, st, uk, L.scal + 0, mask, L.flags,
-                               L.pos, L.nid, d_in_col);
-            LSTC_LAUNCHED("k_orient_scatter(in)");
-            hipLaunchKernelGGL(k_out_degree, dim3(grid_n), dim3(256), 0, st, L.first, L.last, L.pos, n, L.din);
-            LSTC_LAUNCHED("k_out_degree(in)");
+            // in-lists of the arcs counted from the target side (entries -> scal[3])
+            LSTC_TRY(fill_u64(L.first, 0, (uint64_t)n + 1, st));
+            LSTC_TRY(fill_u64(L.last, 0, (uint64_t)n + 1, st));
+            LSTC_TRY(compact_pass(InOp{uk, L.scal + 0, b, mask, L.dd, L.nid, d_in_col, L.first, L.last}, nk, L.tile_cnt,
+                             L.tile_off, L.scal + 3, L.scan_ws, L.scan_bytes, st));
+            hipLaunchKernelGGL(k_row_lengths, dim3(grid_n), dim3(256), 0, st, L.first, L.last, n, L.din,
+                               (uint2 *)nullptr, 0);
+            LSTC_LAUNCHED("k_row_lengths(in)");
             LSTC_TRY(scan_exclusive_u32_to_u64(L.din, L.discan, n, nullptr, L.scan_ws, L.scan_bytes, st));
             hipLaunchKernelGGL(k_orient_offsets, dim3(grid_n), dim3(256), 0, st, L.keepv, L.nidscan, L.discan, n,
                                L.scal + 1, L.scal + 3, d_in_off);
