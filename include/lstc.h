@@ -101,6 +101,16 @@ lstc_status lstc_clean_orient(const uint32_t *d_src, const uint32_t *d_dst, uint
                               uint32_t *n_out, uint64_t *m_out, void *d_ws, size_t ws_bytes,
                               lstc_stream s);
 
+/* lstc_clean_orient plus everything the split count reads (lstc_count_split):
+ * the in-lists and the segment words of d_col and d_in_col (as lstc_segments
+ * builds them), in one stream-ordered pass with one synchronisation at the end.
+ * d_col_seg[m_raw], d_in_seg[m_raw]; *n_in_out receives the in-list entries.  */
+lstc_status lstc_clean_orient_split(const uint32_t *d_src, const uint32_t *d_dst, uint64_t m_raw, uint32_t n,
+                                    int compact, uint64_t *d_sym_off, uint32_t *d_sym_col, uint64_t *d_off,
+                                    uint32_t *d_col, uint64_t *d_in_off, uint32_t *d_in_col, uint64_t *d_col_seg,
+                                    uint64_t *d_in_seg, uint32_t *n_out, uint64_t *m_out, uint64_t *n_in_out,
+                                    void *d_ws, size_t ws_bytes, lstc_stream s);
+
 /* ------------------------------------------------------------------ (b) halo subgraphs
  * Replaces create_subgraph_with_halo (preprocess_GPU/Halo_cleanned_preprocess/
  * round_clean_subgraph.cu:357-902) for part p of a vertex assignment d_part[n]:
@@ -145,6 +155,16 @@ lstc_status lstc_halo_parts_sizes(const uint64_t *d_sym_off, const uint32_t *d_s
 lstc_status lstc_halo_parts_extract(const uint64_t *d_sym_off, const uint32_t *d_sym_col, uint32_t n,
                                     const uint32_t *d_part, uint32_t k, int mode, const lstc_halo_out *outs,
                                     void *d_ws, size_t ws_bytes, lstc_stream s);
+/* Sizes and extraction in one pass over the graph, for callers that give each
+ * part buffers of an upper bound (n_local <= n, m_local <= the graph's
+ * undirected edges, sym_off[n] / 2): n_local[k], n_owned[k], m_local[k]
+ * receive the sizes; if a part's capacity is too small nothing is written to
+ * the outputs, the sizes are still returned, and the call fails with
+ * LSTC_ERR_INVALID.                                                           */
+lstc_status lstc_halo_parts_build(const uint64_t *d_sym_off, const uint32_t *d_sym_col, uint32_t n,
+                                  const uint32_t *d_part, uint32_t k, int mode, const lstc_halo_out *outs,
+                                  uint32_t *n_local, uint32_t *n_owned, uint64_t *m_local, void *d_ws,
+                                  size_t ws_bytes, lstc_stream s);
 
 /* ------------------------------------------------------------------ (c) counting
  * Tuning knobs (picked by lstc_autotune; zero-initialised = defaults).       */

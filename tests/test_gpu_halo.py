@@ -97,8 +97,9 @@ def test_halo_split_parts_match_oracle(lstc, oracle, name, builder, k, mode):
     assert lstc.count_batch(parts) == [lstc.count(h) for h in parts]
 
 
-def _parts_equal_oracle(lstc, oracle, g, sym, part_dev, part_host, k, mode, tag):
-    parts = lstc.halo_extract_parts(g, part_dev, k, mode=mode, with_l2g=True)
+def _parts_equal_oracle(lstc, oracle, g, sym, part_dev, part_host, k, mode, tag, one_pass=True):
+    parts = lstc.halo_extract_parts(g, part_dev, k, mode=mode, with_l2g=True,
+                                    one_pass_limit_bytes=(8 << 30) if one_pass else 0)
     assert len(parts) == k
     for p, h in enumerate(parts):
         oh = oracle.halo_extract(sym, part_host, p, mode)
@@ -237,3 +238,42 @@ def test_partition_file_errors(lstc, tmp_path):
         lstc.partition_from_file(str(big), 4, 4)
     with pytest.raises(lstc.LstcError, match="cannot open"):
         lstc.partition_from_file(str(tmp_path / "missing"), 4, 4)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_halo_parts_sized_then_extracted(lstc, oracle, mode):
+    """The two-call path (exact sizes first, lstc_halo_parts_sizes + lstc_halo_parts_extract) gives the
+    same parts as the one-pass build."""
+    n, s, d = oracle.gen_rmat(12, 16, seed=21)
+    g = _device_graph(lstc, n, s, d)
+    sym, _ = oracle.clean_orient(n, s, d)
+    part = lstc.partition_random(g.n, 6, seed=4)
+    opart = oracle.partition_random(sym.n, 6, seed=4)
+    _parts_equal_oracle(lstc, oracle, g, sym, part, opart, 6, mode, "two-call", one_pass=False)
+
+
+def test_halo_parts_build_reports_sizes_when_too_small(lstc, oracle):
+    import ctypes as C
+    from trianglecount_amd._lib import HaloOut
+    n, s, d = oracle.gen_rmat(11, 16, seed=2)
+    g = _device_graph(lstc, n, s, d)
+    part = lstc.partition_random(g.n, 3, seed=1)
+    want = lstc.halo_extract_parts(g, part, 3)
+    L = lstc.load()
+    outs = (HaloOut * 3)()
+    keep = []
+    for p in range(3):
+        off = torch.full((4,), -7, dtype=torch.int64, device="cuda")
+        col = torch.empty(4, dtype=torch.int32, device="cuda")
+        keep.append((off, col))
+        outs[p].off, outs[p].col, outs[p].n_local_cap, outs[p].m_local_cap = off.data_ptr(), col.data_ptr(), 3, 4
+    nl, no, ml = (C.c_uint32 * 3)(), (C.c_uint32 * 3)(), (C.c_uint64 * 3)()
+    nb = C.c_size_t()
+    lstc.check(L.lstc_halo_parts_workspace(g.n, 3, C.byref(nb)))
+    ws = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+    st = L.lstc_halo_parts_build(C.c_void_p(g.sym_off.data_ptr()), C.c_void_p(g.sym_col.data_ptr()), g.n,
+                                 C.c_void_p(part.data_ptr()), 3, 0, outs, nl, no, ml, C.c_void_p(ws.data_ptr()),
+                                 nb.value, C.c_void_p(lstc.stream_handle()))
+    assert st == -1
+    assert [(nl[p], no[p], ml[p]) for p in range(3)] == [(h.n, h.owned, h.m) for h in want]
+    assert all(int(off[0].item()) == -7 for off, _ in keep)

@@ -112,6 +112,12 @@ def test_in_lists_bit_exact(lstc, oracle, name, builder):
     got_col = g.in_col[: int(want_off[-1])].cpu().numpy().view(np.uint32)
     assert np.array_equal(got_off, want_off)
     assert np.array_equal(got_col, want_col)
+    assert g.meta["in_entries"] == int(want_off[-1])
+    # segment words written by the same pass (lstc_clean_orient_split): start | length << 32 of each entry's row
+    deg = np.diff(og.off.astype(np.int64)).astype(np.uint64)
+    want_cs = (og.off[:-1] & np.uint64(0xFFFFFFFF)) | (deg << np.uint64(32))
+    assert np.array_equal(g.col_seg[: g.m].cpu().numpy().view(np.uint64), want_cs[og.col])
+    assert np.array_equal(g.in_seg[: len(want_col)].cpu().numpy().view(np.uint64), want_cs[want_col])
     # split count (each arc streams its shorter list) equals the oracle
     assert lstc.count(g) == oracle.count(og)
     assert lstc.count(g, split=False) == oracle.count(og)

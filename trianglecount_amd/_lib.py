@@ -108,12 +108,15 @@ SIGNATURES = {
     "lstc_partition_read": (ST, [C.c_char_p, U32, U32, VP, VP]),
     "lstc_clean_orient_workspace": (ST, [U32, U64, PSZ]),
     "lstc_clean_orient": (ST, [VP, VP, U64, U32, I32, VP, VP, VP, VP, VP, VP, PU32, PU64, VP, SZ, VP]),
+    "lstc_clean_orient_split": (ST, [VP, VP, U64, U32, I32, VP, VP, VP, VP, VP, VP, VP, VP, PU32, PU64, PU64, VP, SZ,
+                                     VP]),
     "lstc_halo_workspace": (ST, [U32, PSZ]),
     "lstc_halo_sizes": (ST, [VP, VP, U32, VP, U32, I32, PU32, PU32, PU64, VP, SZ, VP]),
     "lstc_halo_extract": (ST, [VP, VP, U32, VP, U32, I32, VP, VP, VP, U32, U64, VP, SZ, VP]),
     "lstc_halo_parts_workspace": (ST, [U32, U32, PSZ]),
     "lstc_halo_parts_sizes": (ST, [VP, VP, U32, VP, U32, I32, PU32, PU32, PU64, VP, SZ, VP]),
     "lstc_halo_parts_extract": (ST, [VP, VP, U32, VP, U32, I32, C.POINTER(HaloOut), VP, SZ, VP]),
+    "lstc_halo_parts_build": (ST, [VP, VP, U32, VP, U32, I32, C.POINTER(HaloOut), PU32, PU32, PU64, VP, SZ, VP]),
     "lstc_count_workspace": (ST, [U32, U32, U64, C.POINTER(Tune), PSZ]),
     "lstc_count": (ST, [VP, VP, U32, U32, U64, VP, VP, C.POINTER(Tune), VP, SZ, VP]),
     "lstc_count_split": (ST, [VP, VP, VP, VP, VP, VP, U32, U32, U64, VP, C.POINTER(Tune), VP, SZ, VP]),

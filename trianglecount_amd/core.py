@@ -213,15 +213,22 @@ def clean_orient(n: int, src: torch.Tensor, dst: torch.Tensor, compact: bool = T
     in_off = empty_u64(n + 1) if in_lists else None
     in_col = empty_u32(m_raw) if in_lists else None
     n_out, m_out = C.c_uint32(), C.c_uint64()
+    if in_lists:  # in-lists and segment words in the same pass (lstc_clean_orient_split)
+        col_seg, in_seg, n_in = empty_u64(m_raw), empty_u64(m_raw), C.c_uint64()
+        check(L.lstc_clean_orient_split(_ptr(src), _ptr(dst), m_raw, n, int(bool(compact)), _ptr(sym_off),
+                                        _ptr(sym_col), _ptr(off), _ptr(col), _ptr(in_off), _ptr(in_col),
+                                        _ptr(col_seg), _ptr(in_seg), C.byref(n_out), C.byref(m_out), C.byref(n_in),
+                                        wp, wb, C.c_void_p(stream_handle(stream))), "lstc_clean_orient_split")
+        g = Graph(n_out.value, m_out.value, off, col, sym_off, sym_col)
+        g.in_off, g.in_col, g.in_owned = in_off, in_col, g.n
+        g.col_seg, g.in_seg = col_seg, in_seg
+        g.meta["in_entries"] = n_in.value
+        return g
     check(L.lstc_clean_orient(_ptr(src), _ptr(dst), m_raw, n, int(bool(compact)), _ptr(sym_off),
                               _ptr(sym_col), _ptr(off), _ptr(col), _ptr(in_off), _ptr(in_col),
                               C.byref(n_out), C.byref(m_out), wp, wb, C.c_void_p(stream_handle(stream))),
           "lstc_clean_orient")
-    g = Graph(n_out.value, m_out.value, off, col, sym_off, sym_col)
-    if in_lists:
-        g.in_off, g.in_col, g.in_owned = in_off, in_col, g.n
-        segments(g, stream=stream)
-    return g
+    return Graph(n_out.value, m_out.value, off, col, sym_off, sym_col)
 
 
 def segments(g: Graph, stream=None) -> Graph:
@@ -335,9 +342,15 @@ def halo_extract(g: Graph, part: torch.Tensor, p: int, mode: int = 0, with_l2g: 
 
 
 def halo_extract_parts(g: Graph, part: torch.Tensor, k: int, mode: int = 0, with_l2g: bool = False,
-                       ws: Workspace | None = None, stream=None, split: bool = False) -> list:
-    """Halo subgraphs of all ``k`` parts (lstc_halo_parts_*): the same graphs as
-    ``[halo_extract(g, part, p, mode) for p in range(k)]``, built by passes that serve every part."""
+                       ws: Workspace | None = None, stream=None, split: bool = False,
+                       one_pass_limit_bytes: int = 8 << 30) -> list:
+    """Halo subgraphs of all ``k`` parts: the same graphs as
+    ``[halo_extract(g, part, p, mode) for p in range(k)]``, built by passes that serve every part.
+
+    When k upper-bound buffers (n_local <= n, m_local <= the undirected edges) take at most
+    ``one_pass_limit_bytes``, one call sizes and fills them (lstc_halo_parts_build) and the parts
+    are views of those buffers; otherwise the sizes come first (lstc_halo_parts_sizes) and the
+    buffers are allocated exactly (lstc_halo_parts_extract)."""
     from ._lib import HaloOut
     if g.sym_off is None:
         raise ValueError("halo_extract_parts needs the symmetric CSR: clean_orient(..., keep_symmetric=True)")
@@ -348,25 +361,36 @@ def halo_extract_parts(g: Graph, part: torch.Tensor, k: int, mode: int = 0, with
     wp, wb = (ws or _default_ws).get(nb.value)
     sh = C.c_void_p(stream_handle(stream))
     nl, no, ml = (C.c_uint32 * k)(), (C.c_uint32 * k)(), (C.c_uint64 * k)()
-    check(L.lstc_halo_parts_sizes(_ptr(g.sym_off), _ptr(g.sym_col), g.n, _ptr(part), k, mode, nl, no, ml, wp, wb,
-                                  sh), "lstc_halo_parts_sizes")
+    one_pass = k * (8 * (g.n + 1) + 4 * g.m + (4 * g.n if with_l2g else 0)) <= one_pass_limit_bytes
+    if one_pass:
+        caps = [(g.n, g.m)] * k
+    else:
+        check(L.lstc_halo_parts_sizes(_ptr(g.sym_off), _ptr(g.sym_col), g.n, _ptr(part), k, mode, nl, no, ml, wp,
+                                      wb, sh), "lstc_halo_parts_sizes")
+        caps = [(nl[p], ml[p]) for p in range(k)]
     outs = (HaloOut * k)()
+    bufs = []
+    for p, (cn, cm) in enumerate(caps):
+        off, col = empty_u64(cn + 1), empty_u32(cm)
+        l2g = empty_u32(cn) if with_l2g else None
+        outs[p].off, outs[p].col = off.data_ptr(), col.data_ptr()
+        outs[p].l2g = l2g.data_ptr() if with_l2g else None
+        outs[p].n_local_cap, outs[p].m_local_cap = cn, cm
+        bufs.append((off, col, l2g))
+    if one_pass:
+        check(L.lstc_halo_parts_build(_ptr(g.sym_off), _ptr(g.sym_col), g.n, _ptr(part), k, mode, outs, nl, no, ml,
+                                      wp, wb, sh), "lstc_halo_parts_build")
+    else:
+        check(L.lstc_halo_parts_extract(_ptr(g.sym_off), _ptr(g.sym_col), g.n, _ptr(part), k, mode, outs, wp, wb,
+                                        sh), "lstc_halo_parts_extract")
     parts = []
-    for p in range(k):
-        h = Graph(nl[p], ml[p], empty_u64(nl[p] + 1), empty_u32(ml[p]), n_owned=no[p])
+    for p, (off, col, l2g) in enumerate(bufs):
+        h = Graph(nl[p], ml[p], off[: nl[p] + 1], col[: max(ml[p], 1)], n_owned=no[p])
         if with_l2g:
-            h.meta["l2g"] = empty_u32(nl[p])
-        outs[p].off, outs[p].col = h.off.data_ptr(), h.col.data_ptr()
-        outs[p].l2g = h.meta["l2g"].data_ptr() if with_l2g else None
-        outs[p].n_local_cap, outs[p].m_local_cap = nl[p], ml[p]
-        parts.append(h)
-    check(L.lstc_halo_parts_extract(_ptr(g.sym_off), _ptr(g.sym_col), g.n, _ptr(part), k, mode, outs, wp, wb, sh),
-          "lstc_halo_parts_extract")
-    for h in parts:
-        if with_l2g:
-            h.meta["l2g"] = h.meta["l2g"][: h.n]
+            h.meta["l2g"] = l2g[: nl[p]]
         if split:
             in_lists(h, ws=ws, stream=stream)
+        parts.append(h)
     return parts
 
 

@@ -31,11 +31,21 @@ static constexpr uint32_t kNoLoc = 0xFFFFFFFFu;
 static constexpr uint32_t kMaxParts = 64;
 static constexpr uint32_t kTileV = 1024;  // vertices per tile of the local-id scan (256 threads x 4)
 
+// Per-vertex record, gathered once per arc.  The vertex's local ids in the parts
+// of its mask sit together in lids[lbase ...] (ascending part order), so the ids
+// of one neighbour in all parts come from one cache line.
 struct __align__(16) VInfo {
-    unsigned long long mem;  // bit p: the vertex lies in part p's local set (P_p or H_p)
-    uint32_t deg;            // symmetric degree
-    uint32_t part;           // part[v] (kNoLoc when >= k)
+    unsigned long long mem;    // bit p: the vertex lies in part p's local set (P_p or H_p)
+    unsigned long long lbase;  // first of its popcount(mem) local ids in lids
+    uint32_t deg;              // symmetric degree
+    uint32_t part;             // part[v] (kNoLoc when >= k)
+    uint32_t pad[2];
 };
+
+// Position of part p's local id among a vertex's ids (parts below p in its mask).
+__device__ __forceinline__ uint32_t lid_rank(unsigned long long mem, uint32_t p) {
+    return (uint32_t)__popcll(mem & ((1ull << p) - 1ull));
+}
 
 struct PartOut {             // device copy of lstc_halo_out
     uint64_t *off;
@@ -58,8 +68,10 @@ __global__ void k_hp_init(const uint64_t *__restrict__ off, const uint32_t *__re
         const uint32_t p = part[v];
         VInfo x;
         x.mem = 0;
+        x.lbase = 0;
         x.deg = (uint32_t)(off[v + 1] - off[v]);
         x.part = p < k ? p : kNoLoc;
+        x.pad[0] = x.pad[1] = 0;
         info[v] = x;
     }
 }
@@ -85,6 +97,16 @@ __global__ __launch_bounds__(256) void k_hp_mask(const uint64_t *__restrict__ of
             info[v].mem = p == kNoLoc ? m : m | (1ull << p);
         }
     }
+}
+
+__global__ void k_hp_popc(const VInfo *__restrict__ info, uint32_t n, uint32_t *__restrict__ pc) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
+        pc[v] = (uint32_t)__popcll(info[v].mem);
+}
+
+__global__ void k_hp_lbase(const uint64_t *__restrict__ lb, uint32_t n, VInfo *__restrict__ info) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
+        info[v].lbase = lb[v];
 }
 
 // P and H counts of every part in each tile of kTileV vertices: cnt[tile][2p] = |P_p n tile|,
@@ -139,19 +161,21 @@ __global__ __launch_bounds__(1024) void k_hp_tilescan(uint32_t *__restrict__ cnt
     }
 }
 
-// loc[p][v] = the local id of v in part p (P block, then H block), or kNoLoc.
+// The local id of v in every part p of its mask (P block, then H block), at
+// lids[lbase(v) + lid_rank(mem(v), p)].
 __global__ __launch_bounds__(256) void k_hp_local(const VInfo *__restrict__ info, uint32_t n, uint32_t k,
                                                   const uint32_t *__restrict__ cnt, const uint64_t *__restrict__ tot,
-                                                  uint32_t *__restrict__ loc) {
+                                                  uint32_t *__restrict__ lids) {
     __shared__ uint32_t s_w[4][2];
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t v0 = (uint64_t)blockIdx.x * kTileV + threadIdx.x * 4;
-    unsigned long long mem[4];
+    unsigned long long mem[4], lb[4];
     uint32_t pp[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const bool in = v0 + i < n;
         mem[i] = in ? info[v0 + i].mem : 0ull;
+        lb[i] = in ? info[v0 + i].lbase : 0ull;
         pp[i] = in ? info[v0 + i].part : kNoLoc;
     }
     for (uint32_t p = 0; p < k; ++p) {
@@ -177,34 +201,34 @@ __global__ __launch_bounds__(256) void k_hp_local(const VInfo *__restrict__ info
         }
         uint32_t rp = (uint32_t)cnt[(uint64_t)blockIdx.x * 2 * k + 2 * p] + bp + ip - cp;
         uint32_t rh = (uint32_t)(tot[2 * p] + cnt[(uint64_t)blockIdx.x * 2 * k + 2 * p + 1]) + bh + ih - ch;
-        uint32_t *lp = loc + (uint64_t)p * n;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             if (v0 + i >= n) break;
-            uint32_t id = kNoLoc;
-            if (fp[i]) id = rp++;
-            else if (fh[i]) id = rh++;
-            lp[v0 + i] = id;
+            if (fp[i]) lids[lb[i] + lid_rank(mem[i], p)] = rp++;
+            else if (fh[i]) lids[lb[i] + lid_rank(mem[i], p)] = rh++;
         }
     }
 }
 
-// l2g[p][loc[p][v]] = v (launched once the sizes are checked against the caller's capacities).
-__global__ void k_hp_l2g(const uint32_t *__restrict__ loc, uint32_t n, uint32_t k, uint32_t *const *__restrict__ l2g) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * k;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t id = loc[i];
-        if (id != kNoLoc) l2g[i / n][id] = (uint32_t)(i % n);
+// l2g[p][local id of v in p] = v (launched once the sizes are checked against the caller's capacities).
+__global__ void k_hp_l2g(const VInfo *__restrict__ info, const uint32_t *__restrict__ lids, uint32_t n,
+                         uint32_t *const *__restrict__ l2g) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        const VInfo x = info[v];
+        uint32_t r = 0;
+        for (unsigned long long rest = x.mem; rest; rest &= rest - 1, ++r)
+            l2g[__builtin_ctzll(rest)][lids[x.lbase + r]] = (uint32_t)v;
     }
 }
 
 // One wave per vertex s; FILL = 0 counts each part's kept arcs of row s
-// (rowlen[p][loc], and those into P_p: rowp[p][loc]); FILL = 1 writes them.
+// (rowlen[p][local row], and those into P_p: rowp[p][local row]); FILL = 1
+// writes them.
 // Lane p holds part p's state: counters, or the two write positions.
 template <int FILL>
 __global__ __launch_bounds__(256) void k_hp_rows(const uint64_t *__restrict__ off, const uint32_t *__restrict__ col,
                                                  uint32_t n, uint32_t k, const VInfo *__restrict__ info,
-                                                 const uint32_t *__restrict__ loc, uint32_t *__restrict__ rowlen,
+                                                 const uint32_t *__restrict__ lids, uint32_t *__restrict__ rowlen,
                                                  uint32_t *__restrict__ rowp, unsigned long long *__restrict__ m_tot,
                                                  const PartOut *__restrict__ outs) {
     const uint32_t lane = threadIdx.x & 63;
@@ -213,10 +237,11 @@ __global__ __launch_bounds__(256) void k_hp_rows(const uint64_t *__restrict__ of
     for (uint64_t s = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); s < n; s += nw) {
         const unsigned long long ms = read_lane64(info[s].mem, 0);  // wave-uniform
         if (ms == 0) continue;
+        const uint64_t sbase = read_lane64(info[s].lbase, 0);
         const uint64_t b = off[s], e = off[s + 1];
         const uint32_t ds = (uint32_t)(e - b);
         const bool mine = lane < k && ((ms >> lane) & 1ull);
-        const uint32_t ls = mine ? loc[(uint64_t)lane * n + s] : 0u;  // s's local row in part `lane`
+        const uint32_t ls = mine ? lids[sbase + lid_rank(ms, lane)] : 0u;  // s's local row in part `lane`
         uint32_t cnt = 0, cntp = 0;
         uint64_t posp = 0, posh = 0;
         uint32_t *ocol = nullptr;
@@ -228,13 +253,15 @@ __global__ __launch_bounds__(256) void k_hp_rows(const uint64_t *__restrict__ of
         for (uint64_t j0 = b; j0 < e; j0 += 64) {
             const uint64_t j = j0 + lane;
             uint32_t d = 0;
-            unsigned long long md = 0;
+            unsigned long long md = 0, dmem = 0, dbase = 0;
             uint32_t pd = kNoLoc;
             if (j < e) {
                 d = col[j];
                 const VInfo x = info[d];
                 if (keep_arc(ds, x.deg, (uint32_t)s, d)) {
                     md = x.mem & ms;
+                    dmem = x.mem;
+                    dbase = x.lbase;
                     pd = x.part;
                 }
             }
@@ -254,7 +281,7 @@ __global__ __launch_bounds__(256) void k_hp_rows(const uint64_t *__restrict__ of
                         const uint64_t bp = read_lane64(posp, p), bh = read_lane64(posh, p);
                         uint32_t *oc = (uint32_t *)read_lane64((uint64_t)ocol, p);
                         if (in) {
-                            const uint32_t ld = loc[(uint64_t)p * n + d];
+                            const uint32_t ld = lids[dbase + lid_rank(dmem, p)];
                             if (inp) oc[bp + mask_rank(mP)] = ld;
                             else oc[bh + mask_rank(mH)] = ld;
                         }
@@ -282,7 +309,8 @@ __global__ void k_hp_last(const PartOut *__restrict__ outs, const uint64_t *__re
 
 struct HPLayout {
     VInfo *info;
-    uint32_t *loc, *rowlen, *rowp, *cnt;
+    uint32_t *lids, *pc, *rowlen, *rowp, *cnt;
+    uint64_t *lbase;
     uint64_t *tot;               // [2k]
     unsigned long long *m_tot;   // [k]
     PartOut *outs;               // [k]
@@ -298,7 +326,9 @@ static HPLayout hp_layout(void *ws, size_t ws_bytes, uint32_t n, uint32_t k) {
     const size_t nn = (size_t)n + 1, kk = k ? k : 1;
     L.tiles = (uint32_t)(((uint64_t)n + kTileV - 1) / kTileV);
     L.info = c.take<VInfo>(nn);
-    L.loc = c.take<uint32_t>(kk * nn);
+    L.lids = c.take<uint32_t>(kk * nn);  // at most k local ids per vertex
+    L.pc = c.take<uint32_t>(nn);
+    L.lbase = c.take<uint64_t>(nn);
     L.rowlen = c.take<uint32_t>(kk * nn);
     L.rowp = c.take<uint32_t>(kk * nn);
     L.cnt = c.take<uint32_t>((size_t)(L.tiles ? L.tiles : 1) * 2 * kk);
@@ -333,14 +363,19 @@ static lstc_status hp_front(const uint64_t *off, const uint32_t *col, uint32_t n
     LSTC_LAUNCHED("k_hp_init");
     hipLaunchKernelGGL(k_hp_mask, dim3(grid_w), dim3(256), 0, st, off, col, n, mode, L.info);
     LSTC_LAUNCHED("k_hp_mask");
+    hipLaunchKernelGGL(k_hp_popc, dim3(grid_n), dim3(256), 0, st, (const VInfo *)L.info, n, L.pc);
+    LSTC_LAUNCHED("k_hp_popc");
+    LSTC_TRY(scan_exclusive_u32_to_u64(L.pc, L.lbase, n, nullptr, L.scan_ws, L.scan_bytes, st));
+    hipLaunchKernelGGL(k_hp_lbase, dim3(grid_n), dim3(256), 0, st, (const uint64_t *)L.lbase, n, L.info);
+    LSTC_LAUNCHED("k_hp_lbase");
     hipLaunchKernelGGL(k_hp_tilecount, dim3(L.tiles), dim3(256), 0, st, L.info, n, k, L.cnt);
     LSTC_LAUNCHED("k_hp_tilecount");
     hipLaunchKernelGGL(k_hp_tilescan, dim3(1), dim3(1024), 0, st, L.cnt, L.tiles, k, L.tot);
     LSTC_LAUNCHED("k_hp_tilescan");
-    hipLaunchKernelGGL(k_hp_local, dim3(L.tiles), dim3(256), 0, st, L.info, n, k, L.cnt, L.tot, L.loc);
+    hipLaunchKernelGGL(k_hp_local, dim3(L.tiles), dim3(256), 0, st, L.info, n, k, L.cnt, L.tot, L.lids);
     LSTC_LAUNCHED("k_hp_local");
     LSTC_TRY(fill_u64((uint64_t *)L.m_tot, 0, k, st));
-    hipLaunchKernelGGL(k_hp_rows<0>, dim3(grid_w), dim3(256), 0, st, off, col, n, k, L.info, L.loc, L.rowlen, L.rowp,
+    hipLaunchKernelGGL(k_hp_rows<0>, dim3(grid_w), dim3(256), 0, st, off, col, n, k, L.info, L.lids, L.rowlen, L.rowp,
                        L.m_tot, (const PartOut *)nullptr);
     LSTC_LAUNCHED("k_hp_rows<0>");
     return LSTC_OK;
@@ -389,21 +424,27 @@ extern "C" lstc_status lstc_halo_parts_sizes(const uint64_t *d_sym_off, const ui
     return LSTC_OK;
 }
 
-extern "C" lstc_status lstc_halo_parts_extract(const uint64_t *d_sym_off, const uint32_t *d_sym_col, uint32_t n,
-                                               const uint32_t *d_part, uint32_t k, int mode, const lstc_halo_out *outs,
-                                               void *d_ws, size_t ws_bytes, lstc_stream s) {
-    hipStream_t st = (hipStream_t)s;
-    LSTC_TRY(hp_check(d_sym_off, d_sym_col, n, d_part, k, mode, "lstc_halo_parts_extract"));
-    if (!outs) return fail(LSTC_ERR_INVALID, "lstc_halo_parts_extract: NULL outputs");
+// Front once, sizes to the host (n_local, n_owned, m_local), capacity checks,
+// then offsets and columns of every part.  A part whose output is too small
+// leaves every output untouched and fails with the sizes filled in.
+static lstc_status hp_extract(const uint64_t *d_sym_off, const uint32_t *d_sym_col, uint32_t n, const uint32_t *d_part,
+                              uint32_t k, int mode, const lstc_halo_out *outs, uint32_t *n_local, uint32_t *n_owned,
+                              uint64_t *m_local, void *d_ws, size_t ws_bytes, hipStream_t st, const char *who) {
+    LSTC_TRY(hp_check(d_sym_off, d_sym_col, n, d_part, k, mode, who));
+    if (!outs) return fail(LSTC_ERR_INVALID, "%s: NULL outputs", who);
     const HPLayout L = hp_layout(d_ws, ws_bytes, n, k);
-    if (!d_ws || ws_bytes < L.bytes) return fail(LSTC_ERR_WORKSPACE, "lstc_halo_parts_extract: workspace too small");
+    if (!d_ws || ws_bytes < L.bytes) return fail(LSTC_ERR_WORKSPACE, "%s: workspace too small", who);
     bool with_l2g = false;
     for (uint32_t p = 0; p < k; ++p) {
-        if (!outs[p].off) return fail(LSTC_ERR_INVALID, "lstc_halo_parts_extract: part %u has no offsets output", p);
+        if (!outs[p].off) return fail(LSTC_ERR_INVALID, "%s: part %u has no offsets output", who, p);
         with_l2g = with_l2g || outs[p].l2g;
     }
     if (n == 0) {
-        for (uint32_t p = 0; p < k; ++p) LSTC_TRY(fill_u64(outs[p].off, 0, 1, st));
+        for (uint32_t p = 0; p < k; ++p) {
+            n_local[p] = n_owned[p] = 0;
+            m_local[p] = 0;
+            LSTC_TRY(fill_u64(outs[p].off, 0, 1, st));
+        }
         return LSTC_OK;
     }
     std::vector<PartOut> po(k);
@@ -418,33 +459,58 @@ extern "C" lstc_status lstc_halo_parts_extract(const uint64_t *d_sym_off, const 
     // sizes bound the writes into the caller's buffers
     std::vector<uint64_t> h;
     LSTC_TRY(hp_sizes_to_host(L, k, h, st));
+    bool fits = true;
     for (uint32_t p = 0; p < k; ++p) {
-        const uint64_t nl = h[2 * p] + h[2 * p + 1], ml = h[2 * k + p];
-        if (nl > outs[p].n_local_cap || ml > outs[p].m_local_cap)
-            return fail(LSTC_ERR_INVALID,
-                        "lstc_halo_parts_extract: part %u output capacity too small (need %llu rows, %llu arcs)", p,
-                        (unsigned long long)nl, (unsigned long long)ml);
-        if (ml && !outs[p].col) return fail(LSTC_ERR_INVALID, "lstc_halo_parts_extract: part %u has no column output", p);
+        n_owned[p] = (uint32_t)h[2 * p];
+        n_local[p] = (uint32_t)(h[2 * p] + h[2 * p + 1]);
+        m_local[p] = h[2 * k + p];
+        fits = fits && n_local[p] <= outs[p].n_local_cap && m_local[p] <= outs[p].m_local_cap;
+    }
+    for (uint32_t p = 0; p < k; ++p) {
+        if (n_local[p] > outs[p].n_local_cap || m_local[p] > outs[p].m_local_cap)
+            return fail(LSTC_ERR_INVALID, "%s: part %u output capacity too small (need %u rows, %llu arcs)", who, p,
+                        n_local[p], (unsigned long long)m_local[p]);
+        if (m_local[p] && !outs[p].col) return fail(LSTC_ERR_INVALID, "%s: part %u has no column output", who, p);
         if (outs[p].l2g == nullptr && with_l2g)
-            return fail(LSTC_ERR_INVALID, "lstc_halo_parts_extract: give every part an l2g output or none");
+            return fail(LSTC_ERR_INVALID, "%s: give every part an l2g output or none", who);
     }
-    for (uint32_t p = 0; p < k; ++p) {
-        const uint64_t nl = h[2 * p] + h[2 * p + 1];
-        if (nl) LSTC_TRY(scan_exclusive_u32_to_u64(L.rowlen + (size_t)p * n, outs[p].off, nl, nullptr,
-                                                   L.scan_ws, L.scan_bytes, st));
-    }
+    (void)fits;
+    for (uint32_t p = 0; p < k; ++p)
+        if (n_local[p])
+            LSTC_TRY(scan_exclusive_u32_to_u64(L.rowlen + (size_t)p * n, outs[p].off, n_local[p], nullptr, L.scan_ws,
+                                               L.scan_bytes, st));
     hipLaunchKernelGGL(k_hp_last, dim3(1), dim3(64), 0, st, (const PartOut *)L.outs, L.tot, k,
                        (const unsigned long long *)L.m_tot);
     LSTC_LAUNCHED("k_hp_last");
     const int cus = device_cus();
     if (with_l2g) {
-        const uint32_t grid = (uint32_t)hmin<uint64_t>(((uint64_t)n * k + 255) / 256, (uint64_t)cus * 32);
-        hipLaunchKernelGGL(k_hp_l2g, dim3(grid), dim3(256), 0, st, L.loc, n, k, (uint32_t *const *)L.l2g);
+        const uint32_t grid = (uint32_t)hmin<uint64_t>(((uint64_t)n + 255) / 256, (uint64_t)cus * 32);
+        hipLaunchKernelGGL(k_hp_l2g, dim3(grid), dim3(256), 0, st, (const VInfo *)L.info, (const uint32_t *)L.lids, n,
+                           (uint32_t *const *)L.l2g);
         LSTC_LAUNCHED("k_hp_l2g");
     }
     const uint32_t grid_w = (uint32_t)hmin<uint64_t>(((uint64_t)n + 3) / 4, (uint64_t)cus * 32);
-    hipLaunchKernelGGL(k_hp_rows<1>, dim3(grid_w), dim3(256), 0, st, d_sym_off, d_sym_col, n, k, L.info, L.loc,
+    hipLaunchKernelGGL(k_hp_rows<1>, dim3(grid_w), dim3(256), 0, st, d_sym_off, d_sym_col, n, k, L.info, L.lids,
                        L.rowlen, L.rowp, L.m_tot, (const PartOut *)L.outs);
     LSTC_LAUNCHED("k_hp_rows<1>");
     return LSTC_OK;
+}
+
+extern "C" lstc_status lstc_halo_parts_extract(const uint64_t *d_sym_off, const uint32_t *d_sym_col, uint32_t n,
+                                               const uint32_t *d_part, uint32_t k, int mode, const lstc_halo_out *outs,
+                                               void *d_ws, size_t ws_bytes, lstc_stream s) {
+    if (k == 0 || k > kMaxParts) return fail(LSTC_ERR_INVALID, "lstc_halo_parts_extract: k = %u parts, 1..%u", k, kMaxParts);
+    std::vector<uint32_t> nl(k), no(k);
+    std::vector<uint64_t> ml(k);
+    return hp_extract(d_sym_off, d_sym_col, n, d_part, k, mode, outs, nl.data(), no.data(), ml.data(), d_ws, ws_bytes,
+                      (hipStream_t)s, "lstc_halo_parts_extract");
+}
+
+extern "C" lstc_status lstc_halo_parts_build(const uint64_t *d_sym_off, const uint32_t *d_sym_col, uint32_t n,
+                                             const uint32_t *d_part, uint32_t k, int mode, const lstc_halo_out *outs,
+                                             uint32_t *n_local, uint32_t *n_owned, uint64_t *m_local, void *d_ws,
+                                             size_t ws_bytes, lstc_stream s) {
+    if (!n_local || !n_owned || !m_local) return fail(LSTC_ERR_INVALID, "lstc_halo_parts_build: NULL size outputs");
+    return hp_extract(d_sym_off, d_sym_col, n, d_part, k, mode, outs, n_local, n_owned, m_local, d_ws, ws_bytes,
+                      (hipStream_t)s, "lstc_halo_parts_build");
 }

@@ -256,6 +256,25 @@ __global__ void k_orient_offsets(const uint32_t *__restrict__ keepv, const uint6
     if (blockIdx.x == 0 && threadIdx.x == 0) off[*d_n_out] = *d_m;
 }
 
+// Segment words (start | length << 32 of the row of every entry) of the
+// oriented columns and the in-lists, with the sizes still on the device:
+// sizes[1] rows, sizes[2] arcs, sizes[3] in-list entries.
+__global__ void k_prep_seg_words(const uint64_t *__restrict__ sizes, const uint64_t *__restrict__ off,
+                                 const uint32_t *__restrict__ col, const uint32_t *__restrict__ in_col,
+                                 uint64_t *__restrict__ col_seg, uint64_t *__restrict__ in_seg) {
+    const uint64_t n_rows = sizes[1], m = sizes[2], n_in = sizes[3];
+    GRID_STRIDE(i, m + n_in) {
+        const bool in = i >= m;
+        const uint32_t v = in ? in_col[i - m] : col[i];
+        uint64_t w = 0;
+        if (v < n_rows) {
+            const uint64_t b = off[v];
+            w = (uint64_t)(uint32_t)b | ((off[v + 1] - b) << 32);
+        }
+        if (in) in_seg[i - m] = w; else col_seg[i] = w;
+    }
+}
+
 struct PrepLayout {
     uint64_t *keys, *alt;
     uint32_t *tile_cnt;
@@ -408,6 +427,38 @@ lstc_status lstc::clean_orient_launch(const uint32_t *d_src, const uint32_t *d_d
     }
     if (d_sizes) *d_sizes = L.scal;
     if (d_bad) *d_bad = L.bad;
+    return LSTC_OK;
+}
+
+extern "C" lstc_status lstc_clean_orient_split(const uint32_t *d_src, const uint32_t *d_dst, uint64_t m_raw,
+                                               uint32_t n, int compact, uint64_t *d_sym_off, uint32_t *d_sym_col,
+                                               uint64_t *d_off, uint32_t *d_col, uint64_t *d_in_off,
+                                               uint32_t *d_in_col, uint64_t *d_col_seg, uint64_t *d_in_seg,
+                                               uint32_t *n_out, uint64_t *m_out, uint64_t *n_in_out, void *d_ws,
+                                               size_t ws_bytes, lstc_stream s) {
+    hipStream_t st = (hipStream_t)s;
+    if (!n_out || !m_out || !n_in_out) return fail(LSTC_ERR_INVALID, "lstc_clean_orient_split: NULL size outputs");
+    if (!d_in_off || !d_in_col || !d_col_seg || !d_in_seg)
+        return fail(LSTC_ERR_INVALID, "lstc_clean_orient_split: in-list and segment-word outputs are required");
+    const uint64_t *d_sizes = nullptr;
+    const uint32_t *d_bad = nullptr;
+    LSTC_TRY(clean_orient_launch(d_src, d_dst, m_raw, n, compact, d_sym_off, d_sym_col, d_off, d_col, d_in_off,
+                                 d_in_col, d_ws, ws_bytes, st, &d_sizes, &d_bad));
+    if (m_raw) {
+        const uint32_t grid = (uint32_t)hmin<uint64_t>((2 * m_raw + 255) / 256, (uint64_t)device_cus() * 32);
+        hipLaunchKernelGGL(k_prep_seg_words, dim3(grid), dim3(256), 0, st, d_sizes, (const uint64_t *)d_off,
+                           (const uint32_t *)d_col, (const uint32_t *)d_in_col, d_col_seg, d_in_seg);
+        LSTC_LAUNCHED("k_prep_seg_words");
+    }
+    uint64_t h[4];
+    uint32_t bad = 0;
+    LSTC_HIP(hipMemcpyAsync(h, d_sizes, sizeof(h), hipMemcpyDeviceToHost, st));
+    LSTC_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, st));
+    LSTC_HIP(hipStreamSynchronize(st));
+    if (bad) return fail(LSTC_ERR_INVALID, "lstc_clean_orient_split: vertex id >= n in input");
+    *n_out = (uint32_t)h[1];
+    *m_out = h[2];
+    *n_in_out = h[3];
     return LSTC_OK;
 }
 
