@@ -79,7 +79,8 @@ __global__ void k_hp_init(const uint64_t *__restrict__ off, const uint32_t *__re
 // mem[v] = own part | parts of the neighbours d that put v into their halo:
 // every neighbour (mode 0) or the neighbours with an arc d -> v (mode 1).
 __global__ __launch_bounds__(256) void k_hp_mask(const uint64_t *__restrict__ off, const uint32_t *__restrict__ col,
-                                                 uint32_t n, int mode, VInfo *__restrict__ info) {
+                                                 const uint32_t *__restrict__ part, uint32_t n, uint32_t k, int mode,
+                                                 VInfo *__restrict__ info) {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x >> 6);
     for (uint64_t v = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); v < n; v += nw) {
@@ -88,8 +89,13 @@ __global__ __launch_bounds__(256) void k_hp_mask(const uint64_t *__restrict__ of
         unsigned long long m = 0;
         for (uint64_t j = b + lane; j < e; j += 64) {
             const uint32_t d = col[j];
-            const VInfo x = info[d];
-            if (x.part != kNoLoc && (mode == 0 || keep_arc(x.deg, dv, d, (uint32_t)v))) m |= 1ull << x.part;
+            if (mode == 0) {  // the 4-byte part id is all a neighbour needs
+                const uint32_t pd = part[d];
+                if (pd < k) m |= 1ull << pd;
+            } else {
+                const VInfo x = info[d];
+                if (x.part != kNoLoc && keep_arc(x.deg, dv, d, (uint32_t)v)) m |= 1ull << x.part;
+            }
         }
         m = wave_or64(m);
         if (lane == 0) {
@@ -361,7 +367,7 @@ static lstc_status hp_front(const uint64_t *off, const uint32_t *col, uint32_t n
     const uint32_t grid_w = (uint32_t)hmin<uint64_t>(((uint64_t)n + 3) / 4, (uint64_t)cus * 32);
     hipLaunchKernelGGL(k_hp_init, dim3(grid_n), dim3(256), 0, st, off, part, n, k, L.info);
     LSTC_LAUNCHED("k_hp_init");
-    hipLaunchKernelGGL(k_hp_mask, dim3(grid_w), dim3(256), 0, st, off, col, n, mode, L.info);
+    hipLaunchKernelGGL(k_hp_mask, dim3(grid_w), dim3(256), 0, st, off, col, part, n, k, mode, L.info);
     LSTC_LAUNCHED("k_hp_mask");
     hipLaunchKernelGGL(k_hp_popc, dim3(grid_n), dim3(256), 0, st, (const VInfo *)L.info, n, L.pc);
     LSTC_LAUNCHED("k_hp_popc");
