@@ -69,6 +69,8 @@ def test_tc_matches_oracle(lstc, oracle, rmat_files):
         tri = [int(x) for x in re.findall(r"Triangle : (\d+)", out)]
         assert tri == [oracle.count(g)] * 3
         assert "Total avg of 3 RUNS" in out
+        # the split count's one-time input is reported apart from the runs
+        assert ("In-lists build :" in out) == (extra == [])
 
 
 @pytest.mark.parametrize("prune", [False, True])
