@@ -15,37 +15,69 @@
 // word start(N+(v)) | d+(v) << 32, so the counting walk reads one coalesced
 // word per segment instead of off[v] and off[v+1] at a random place.
 #include "lstc_internal.h"
+#include "wave.h"
 
 namespace lstc {
 
-__global__ void k_inl_count(const uint64_t *__restrict__ off, const uint32_t *__restrict__ col, uint32_t n_owned,
-                            uint32_t *__restrict__ cnt) {
-    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u <= n_owned;
-         u += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t c = 0;
-        if (u < n_owned) {
-            const uint64_t b = off[u], e = off[u + 1];
-            const uint64_t du = e - b;
-            for (uint64_t i = b; i < e; ++i) {
-                const uint32_t w = col[i];
-                c += (du <= off[w + 1] - off[w]) ? 1u : 0u;
+// The kept arcs (u -> w with u < n_owned and d+(u) <= d+(w)) of a block's
+// kInlRows consecutive rows, walked flat in row order, 256 arcs at a time: each
+// arc finds its row by binary search of the rows' offsets in LDS, so loads of
+// col are coalesced however long the rows are.  Pass 0 counts the kept arcs
+// per row; pass 1 writes their keys u << 32 | w in arc order from pos[first
+// row of the block] on (the kept arcs of consecutive rows are consecutive).
+static constexpr uint32_t kInlRows = 256;
+
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_inl_flat(const uint64_t *__restrict__ off, const uint32_t *__restrict__ col,
+                                                  uint32_t n_owned, uint32_t *__restrict__ cnt,
+                                                  const uint64_t *__restrict__ pos, uint64_t *__restrict__ keys) {
+    __shared__ uint64_t s_off[kInlRows + 1];
+    __shared__ uint32_t s_cnt[kInlRows];
+    __shared__ uint32_t s_w[4];
+    const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    if (!EMIT && blockIdx.x == 0 && t == 0) cnt[n_owned] = 0;
+    for (uint64_t r0 = (uint64_t)blockIdx.x * kInlRows; r0 < n_owned; r0 += (uint64_t)gridDim.x * kInlRows) {
+        const uint32_t R = n_owned - r0 < kInlRows ? (uint32_t)(n_owned - r0) : kInlRows;
+        if (t <= R) s_off[t] = off[r0 + t];
+        if (t == 0 && R == kInlRows) s_off[kInlRows] = off[r0 + kInlRows];
+        if (!EMIT) s_cnt[t] = 0;
+        __syncthreads();
+        const uint64_t a0 = s_off[0], a1 = s_off[R];
+        uint64_t outp = EMIT ? pos[r0] : 0;
+        for (uint64_t c0 = a0; c0 < a1; c0 += 256) {
+            const uint64_t i = c0 + t;
+            bool keep = false;
+            uint32_t row = 0, w = 0;
+            if (i < a1) {
+                uint32_t lo = 0, hi = R;  // s_off[lo] <= i < s_off[hi]
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_off[mid] <= i) lo = mid; else hi = mid;
+                }
+                row = lo;
+                w = col[i];
+                const uint64_t du = s_off[row + 1] - s_off[row];
+                keep = du <= off[w + 1] - off[w];
+            }
+            if (!EMIT) {
+                if (keep) atomicAdd(&s_cnt[row], 1u);
+            } else {
+                const uint64_t mk = ballot(keep);
+                if (lane == 0) s_w[wv] = (uint32_t)__popcll(mk);
+                __syncthreads();
+                uint32_t before = 0;
+                for (uint32_t x = 0; x < wv; ++x) before += s_w[x];
+                const uint32_t total = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+                if (keep) keys[outp + before + mask_rank(mk)] = ((r0 + row) << 32) | w;
+                outp += total;
+                __syncthreads();
             }
         }
-        cnt[u] = c;
-    }
-}
-
-__global__ void k_inl_keys(const uint64_t *__restrict__ off, const uint32_t *__restrict__ col, uint32_t n_owned,
-                           const uint64_t *__restrict__ pos, uint64_t *__restrict__ keys) {
-    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n_owned;
-         u += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t b = off[u], e = off[u + 1];
-        const uint64_t du = e - b;
-        uint64_t p = pos[u];
-        for (uint64_t i = b; i < e; ++i) {
-            const uint32_t w = col[i];
-            if (du <= off[w + 1] - off[w]) keys[p++] = (u << 32) | w;
+        if (!EMIT) {
+            __syncthreads();
+            if (t < R) cnt[r0 + t] = s_cnt[t];
         }
+        __syncthreads();
     }
 }
 
@@ -70,45 +102,29 @@ __global__ void k_inl_offsets(const uint64_t *__restrict__ keys, uint64_t k, uin
 
 __global__ void k_seg_words(const uint32_t *__restrict__ ids, uint64_t n, const uint64_t *__restrict__ off,
                             uint32_t n_rows, uint64_t *__restrict__ words) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t v = ids[i];
-        uint64_t w = 0;  // ids >= n_rows (no list): length 0
-        if (v < n_rows) {
-            const uint64_t b = off[v];
-            w = (uint64_t)(uint32_t)b | ((off[v + 1] - b) << 32);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    // 8 entries per thread and step: their ids, then their row offsets, are loaded together
+    for (uint64_t i0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i0 < n; i0 += 8 * stride) {
+        uint32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint64_t i = i0 + j * stride;
+            v[j] = i < n ? ids[i] : 0xFFFFFFFFu;
         }
-        words[i] = w;
-    }
-}
-
-// The same words for entry counts known only on the device (*m_at arcs,
-// *n_in_at in-list entries; the server's stream-ordered preparation).
-__global__ void k_seg_words_dev(const uint32_t *__restrict__ col, const uint32_t *__restrict__ in_col,
-                                const uint64_t *__restrict__ m_at, const uint64_t *__restrict__ n_in_at, uint64_t cap,
-                                const uint64_t *__restrict__ off, uint32_t n_rows, uint64_t *__restrict__ col_seg,
-                                uint64_t *__restrict__ in_seg) {
-    const uint64_t m = *m_at < cap ? *m_at : cap, n_in = *n_in_at < cap ? *n_in_at : cap;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m + n_in;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const bool in = i >= m;
-        const uint32_t v = in ? in_col[i - m] : col[i];
-        uint64_t w = 0;
-        if (v < n_rows) {
-            const uint64_t b = off[v];
-            w = (uint64_t)(uint32_t)b | ((off[v + 1] - b) << 32);
+        uint64_t b[8], e[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const bool ok = v[j] < n_rows;  // ids >= n_rows (no list): length 0
+            b[j] = ok ? off[v[j]] : 0;
+            e[j] = ok ? off[v[j] + 1] : 0;
         }
-        if (in) in_seg[i - m] = w; else col_seg[i] = w;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint64_t i = i0 + j * stride;
+            if (i >= n) break;
+            words[i] = (uint64_t)(uint32_t)b[j] | ((e[j] - b[j]) << 32);
+        }
     }
-}
-
-lstc_status segments_launch_dev(const uint64_t *d_off, const uint32_t *d_col, const uint64_t *d_in_off,
-                                const uint32_t *d_in_col, uint32_t n_rows, uint64_t cap, uint64_t *d_col_seg,
-                                uint64_t *d_in_seg, hipStream_t st) {
-    const uint32_t grid = (uint32_t)hmin<uint64_t>((2 * cap + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_seg_words_dev, dim3(grid), dim3(256), 0, st, d_col, d_in_col, d_off + n_rows,
-                       d_in_off + n_rows, cap, d_off, n_rows, d_col_seg, d_in_seg);
-    LSTC_LAUNCHED("k_seg_words_dev");
-    return LSTC_OK;
 }
 
 static int bits_for(uint32_t n) {
@@ -158,17 +174,20 @@ extern "C" lstc_status lstc_in_lists(const uint64_t *d_off, const uint32_t *d_co
     uint64_t *alt = c.take<uint64_t>(m ? m : 1);
     char *tail = (char *)d_ws + head;
     const size_t tail_bytes = ws_bytes - head;
-    const uint32_t grid_rows = (uint32_t)hmin<uint64_t>(((uint64_t)n_owned + 256) / 256, 65535);
-    hipLaunchKernelGGL(k_inl_count, dim3(grid_rows), dim3(256), 0, st, d_off, d_col, n_owned, cnt);
-    LSTC_LAUNCHED("k_inl_count");
+    const uint32_t grid_rows =
+        (uint32_t)hmin<uint64_t>(((uint64_t)n_owned + kInlRows) / kInlRows, (uint64_t)device_cus() * 16);
+    hipLaunchKernelGGL(k_inl_flat<false>, dim3(grid_rows), dim3(256), 0, st, d_off, d_col, n_owned, cnt,
+                       (const uint64_t *)nullptr, (uint64_t *)nullptr);
+    LSTC_LAUNCHED("k_inl_flat<count>");
     LSTC_TRY(scan_exclusive_u32_to_u64(cnt, pos, (uint64_t)n_owned + 1, nullptr, tail, tail_bytes, st));
     uint64_t k = 0;
     LSTC_HIP(hipMemcpyAsync(&k, pos + n_owned, sizeof(k), hipMemcpyDeviceToHost, st));
     LSTC_HIP(hipStreamSynchronize(st));
     if (k > m) return fail(LSTC_ERR_INVALID, "lstc_in_lists: %llu entries exceed m", (unsigned long long)k);
     if (n_owned)
-        hipLaunchKernelGGL(k_inl_keys, dim3(grid_rows), dim3(256), 0, st, d_off, d_col, n_owned, pos, keys);
-    LSTC_LAUNCHED("k_inl_keys");
+        hipLaunchKernelGGL(k_inl_flat<true>, dim3(grid_rows), dim3(256), 0, st, d_off, d_col, n_owned,
+                           (uint32_t *)nullptr, (const uint64_t *)pos, keys);
+    LSTC_LAUNCHED("k_inl_flat<keys>");
     uint64_t *sorted = keys;
     LSTC_TRY(radix_sort_u64(keys, alt, k, bits_for(n_rows), &sorted, tail, tail_bytes, st));
     const uint32_t grid_k = (uint32_t)hmin<uint64_t>((k + 255) / 256 + 1, 65535);
@@ -194,12 +213,12 @@ extern "C" lstc_status lstc_segments(const uint64_t *d_off, const uint32_t *d_co
     if (!d_off || (m && (!d_col || !d_col_seg)) || (n_in && (!d_in_col || !d_in_seg)))
         return fail(LSTC_ERR_INVALID, "lstc_segments: NULL array");
     if (m) {
-        const uint32_t grid = (uint32_t)hmin<uint64_t>((m + 255) / 256, 65535);
+        const uint32_t grid = (uint32_t)hmin<uint64_t>((m + 2047) / 2048, (uint64_t)device_cus() * 32);
         hipLaunchKernelGGL(k_seg_words, dim3(grid), dim3(256), 0, st, d_col, m, d_off, n_rows, d_col_seg);
         LSTC_LAUNCHED("k_seg_words");
     }
     if (n_in) {
-        const uint32_t grid = (uint32_t)hmin<uint64_t>((n_in + 255) / 256, 65535);
+        const uint32_t grid = (uint32_t)hmin<uint64_t>((n_in + 2047) / 2048, (uint64_t)device_cus() * 32);
         hipLaunchKernelGGL(k_seg_words, dim3(grid), dim3(256), 0, st, d_in_col, n_in, d_off, n_rows, d_in_seg);
         LSTC_LAUNCHED("k_seg_words");
     }

@@ -90,17 +90,14 @@ lstc_status radix_sort_u64(uint64_t *keys, uint64_t *alt, uint64_t n, int key_bi
                            uint64_t **result, void *ws, size_t ws_bytes, hipStream_t s);
 
 // (a) clean + orient without synchronising (prep.hip); sizes stay on the device:
-// (*d_sizes)[1] = vertices after cleaning, (*d_sizes)[2] = undirected edges.
+// (*d_sizes)[1] = vertices after cleaning, (*d_sizes)[2] = undirected edges,
+// (*d_sizes)[3] = in-list entries.  In-lists and segment words are optional
+// (NULL); segment words of the in-lists need the in-lists.
 lstc_status clean_orient_launch(const uint32_t *d_src, const uint32_t *d_dst, uint64_t m_raw, uint32_t n,
                                 int compact, uint64_t *d_sym_off, uint32_t *d_sym_col, uint64_t *d_off,
-                                uint32_t *d_col, uint64_t *d_in_off, uint32_t *d_in_col, void *d_ws,
-                                size_t ws_bytes, hipStream_t st,
+                                uint32_t *d_col, uint64_t *d_in_off, uint32_t *d_in_col, uint64_t *d_col_seg,
+                                uint64_t *d_in_seg, void *d_ws, size_t ws_bytes, hipStream_t st,
                                 const uint64_t **d_sizes, const uint32_t **d_bad);
-// Segment words (lstc_segments) of a graph whose arc and in-list entry counts
-// are only on the device (d_off[n_rows], d_in_off[n_rows], each at most cap).
-lstc_status segments_launch_dev(const uint64_t *d_off, const uint32_t *d_col, const uint64_t *d_in_off,
-                                const uint32_t *d_in_col, uint32_t n_rows, uint64_t cap, uint64_t *d_col_seg,
-                                uint64_t *d_in_seg, hipStream_t st);
 
 // Fill kernels (used instead of hipMemsetAsync so callers may capture them in graphs).
 lstc_status fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s);

@@ -8,16 +8,20 @@
 //   keys  : each raw pair (s, d), s != d, emits (s<<b | d) and (d<<b | s);
 //           loops and out-of-range ids emit a sentinel that sorts last
 //   sort  : LSD radix sort over 2b bits (prims.hip)
-//   then three compactions of the sorted list, each a count pass (per tile of
-//   4096 keys) + a scan of the tile counts + a scatter pass that recomputes the
-//   predicate and writes the kept entries at their final positions (no flag or
-//   position arrays); the scatter also records each row's first / end position:
-//   unique: first of each run          -> symmetric arc list, row bounds, deg[v]
+//   then compactions of the sorted list, each a count (per tile of 4096 keys),
+//   a scan of the tile counts and a scatter that recomputes the predicate and
+//   writes the kept entries at their final positions (no flag or position
+//   arrays):
+//   unique : first of each run -> symmetric arc list, row bounds -> deg[v]
 //   relabel (compact): id' = rank of v among vertices with deg > 0 (monotone)
 //   symmetric CSR: offsets = scan(deg) compacted, columns relabelled
-//   orient: keep s->d iff deg s < deg d or (equal and d < s)
-//           -> oriented columns (rows ascending), d+(v), offsets = scan(d+)
-//   in    : reverse keys of kept arcs with d+(d) <= d+(s) -> target-side in-lists
+//   orient : keep s->d iff deg s < deg d or (equal and d < s); the count pass
+//            also adds d+(v) per run of equal sources, so offsets = scan(d+)
+//            are known before the scatter, which writes the columns and their
+//            segment words (start | d+ << 32 of the target's row) from one
+//            16-byte record per endpoint and counts the in-list arcs
+//   in     : reverse keys of kept arcs with d+(d) <= d+(s) -> target-side
+//            in-lists and their segment words
 #include "lstc_internal.h"
 #include "wave.h"
 
@@ -53,144 +57,284 @@ __global__ void k_make_keys(const uint32_t *__restrict__ src, const uint32_t *__
 
 // ------------------------------------------------------------------ compactions
 // Tile = 256 threads x 16 rounds = 4096 keys; key (round r, thread t) is tile
-// position r * 256 + t, so loads are coalesced and the order inside the tile is
-// (round, wave, lane).  The scatter pass ranks the kept keys of a tile in that
-// order from one ballot per round and a 64-entry prefix over (round, wave).
+// position r * 256 + t, so loads are coalesced and a wave's 64 lanes hold 64
+// consecutive keys of a round.  A count pass gives each tile's number of kept
+// keys; after a scan of those, a scatter pass writes the kept keys in order,
+// round by round (one ballot per round and a running count across the
+// workgroup), so no flag or position array is needed.  Loads are issued in
+// batches: a thread first loads all its keys, then the gathers of kCG rounds at
+// a time, so each thread has many loads in flight instead of one dependent
+// chain per key.  The tile's keys also sit in LDS, with the key before and
+// after the tile, for the tests that look at neighbours.
+//
+// An Op provides: size(), keys(); fetch(k) -> Aux, the gathers its predicate
+// and its output need; pred(i, k, prev, aux); emit(i, k, prev, next, kept, pos,
+// aux).  prev / next are the keys at i - 1 / i + 1 when kNeedsPrev (else 0;
+// unspecified at i = 0 / i = n - 1, where the Op must not use them).  kRuns: the count pass also adds
+// each source's kept keys into runs()[source].  kSecond: the scatter pass also
+// counts a second predicate pred2 per tile (into tile_cnt2) and per source
+// (into runs2()), for the next compaction over the same keys.
 static constexpr uint32_t kCT = 256;
 static constexpr uint32_t kCR = 16;
+static constexpr uint32_t kCG = 8;  // rounds per batch of gathers
 static constexpr uint64_t kCTile = (uint64_t)kCT * kCR;
+
+// Adds, for each run of equal sources among the wave's lanes, the number of its
+// kept keys into cnt[source] (one atomic per run; sources ascend across lanes).
+__device__ __forceinline__ void add_runs(uint32_t *cnt, uint32_t src, bool kept, bool valid, uint32_t lane) {
+    const uint32_t prev = (uint32_t)__shfl_up((int)src, 1);
+    const uint64_t heads = ballot(valid && (lane == 0 || prev != src));
+    const uint64_t km = ballot(valid && kept);
+    if (valid && lane_in(heads)) {
+        const uint64_t above = heads & ~((2ull << lane) - 1ull);  // heads after this lane (none at lane 63)
+        const uint64_t upto = above ? ((above & (0ull - above)) - 1ull) : ~0ull;
+        const uint32_t c = (uint32_t)__popcll(km & upto & ~((1ull << lane) - 1ull));
+        if (c) atomicAdd(cnt + src, c);
+    }
+}
 
 template <class Op>
 __global__ __launch_bounds__(kCT) void k_compact_count(Op op, uint32_t *__restrict__ tile_cnt) {
+    __shared__ uint64_t s_k[Op::kNeedsPrev ? kCTile + 1 : 1];  // s_k[p + 1] = key at tile position p
     __shared__ uint32_t s_w[kCT / 64];
     const uint64_t n = op.size();
     const uint64_t tile0 = (uint64_t)blockIdx.x * kCTile;
+    const uint32_t lane = threadIdx.x & 63;
     uint32_t c = 0;
     if (tile0 < n) {
-#pragma unroll 4
+        const uint64_t *keys = op.keys();
+        uint64_t k[kCR];
+#pragma unroll
         for (uint32_t r = 0; r < kCR; ++r) {
             const uint64_t i = tile0 + (uint64_t)r * kCT + threadIdx.x;
-            if (i < n && op.pred(i)) ++c;
+            k[r] = i < n ? keys[i] : 0;
+        }
+        if (Op::kNeedsPrev) {
+#pragma unroll
+            for (uint32_t r = 0; r < kCR; ++r) s_k[1 + r * kCT + threadIdx.x] = k[r];
+            if (threadIdx.x == 0) s_k[0] = tile0 ? keys[tile0 - 1] : 0;
+            __syncthreads();
+        }
+#pragma unroll
+        for (uint32_t g = 0; g < kCR; g += kCG) {
+            typename Op::Aux a[kCG];
+#pragma unroll
+            for (uint32_t r = 0; r < kCG; ++r) a[r] = op.fetch(k[g + r]);
+#pragma unroll
+            for (uint32_t r = 0; r < kCG; ++r) {
+                const uint32_t p = (g + r) * kCT + threadIdx.x;
+                const uint64_t i = tile0 + p;
+                const bool kp = i < n && op.pred(i, k[g + r], Op::kNeedsPrev ? s_k[p] : 0, a[r]);
+                c += kp ? 1u : 0u;
+                if (Op::kRuns) add_runs(op.runs(), op.src(k[g + r]), kp, i < n, lane);
+            }
         }
     }
     c = wave_sum32(c);
-    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
+    if (lane == 0) s_w[threadIdx.x >> 6] = c;
     __syncthreads();
     if (threadIdx.x == 0) tile_cnt[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
 }
 
 template <class Op>
-__global__ __launch_bounds__(kCT) void k_compact_scatter(Op op, const uint64_t *__restrict__ tile_off) {
-    __shared__ uint32_t s_pre[kCR * (kCT / 64)];  // kept keys per (round, wave), then their exclusive prefix
+__global__ __launch_bounds__(kCT) void k_compact_scatter(Op op, const uint64_t *__restrict__ tile_off,
+                                                         uint32_t *__restrict__ tile_cnt2) {
+    __shared__ uint64_t s_k[Op::kNeedsPrev ? kCTile + 2 : 1];  // s_k[p + 1] = key at position p; s_k[0], s_k[kCTile + 1]
+    __shared__ uint32_t s_w[2][kCT / 64];    // kept keys per wave of a round (two rounds in flight)
+    __shared__ uint32_t s_w2[kCT / 64];
     const uint64_t n = op.size();
     const uint64_t tile0 = (uint64_t)blockIdx.x * kCTile;
-    if (tile0 >= n) return;  // the whole block leaves together
+    if (tile0 >= n) {  // the whole block leaves together
+        if (Op::kSecond && threadIdx.x == 0) tile_cnt2[blockIdx.x] = 0;
+        return;
+    }
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    uint64_t kept[kCR];
+    const uint64_t *keys = op.keys();
+    uint64_t k[kCR];
 #pragma unroll
     for (uint32_t r = 0; r < kCR; ++r) {
         const uint64_t i = tile0 + (uint64_t)r * kCT + threadIdx.x;
-        kept[r] = ballot(i < n && op.pred(i));
-        if (lane == 0) s_pre[r * (kCT / 64) + wv] = (uint32_t)__popcll(kept[r]);
+        k[r] = i < n ? keys[i] : 0;
     }
-    __syncthreads();
-    if (wv == 0) {
-        const uint32_t v = s_pre[lane];
-        const uint32_t incl = wave_incl_scan(v, lane);
-        s_pre[lane] = incl - v;
-    }
-    __syncthreads();
-    const uint64_t base = tile_off[blockIdx.x];
+    if (Op::kNeedsPrev) {
 #pragma unroll
-    for (uint32_t r = 0; r < kCR; ++r) {
-        const uint64_t i = tile0 + (uint64_t)r * kCT + threadIdx.x;
-        if (i < n) op.emit(i, lane_in(kept[r]), base + s_pre[r * (kCT / 64) + wv] + mask_rank(kept[r]));
+        for (uint32_t r = 0; r < kCR; ++r) s_k[1 + r * kCT + threadIdx.x] = k[r];
+        if (threadIdx.x == 0) s_k[0] = tile0 ? keys[tile0 - 1] : 0;
+        if (threadIdx.x == 1) s_k[kCTile + 1] = tile0 + kCTile < n ? keys[tile0 + kCTile] : 0;
+        __syncthreads();
     }
-}
-
-// The row of key i starts (ends) there when the previous (next) key has another row.
-__device__ __forceinline__ bool row_first(const uint64_t *k, uint64_t i, uint32_t b, uint64_t s) {
-    return i == 0 || (k[i - 1] >> b) != s;
-}
-__device__ __forceinline__ bool row_last(const uint64_t *k, uint64_t i, uint64_t n, uint32_t b, uint64_t s) {
-    return i + 1 == n || (k[i + 1] >> b) != s;
+    uint64_t pos = tile_off[blockIdx.x];  // output position of the round's first kept key
+    uint32_t c2 = 0;
+#pragma unroll
+    for (uint32_t g = 0; g < kCR; g += kCG) {
+        typename Op::Aux a[kCG];
+#pragma unroll
+        for (uint32_t r = 0; r < kCG; ++r) a[r] = op.fetch(k[g + r]);
+#pragma unroll
+        for (uint32_t r = 0; r < kCG; ++r) {
+            const uint32_t p = (g + r) * kCT + threadIdx.x;
+            const uint64_t i = tile0 + p;
+            const bool valid = i < n;
+            const uint64_t prev = Op::kNeedsPrev ? s_k[p] : 0, next = Op::kNeedsPrev ? s_k[p + 2] : 0;
+            const bool kp = valid && op.pred(i, k[g + r], prev, a[r]);
+            const uint64_t km = ballot(kp);
+            if (lane == 0) s_w[r & 1][wv] = (uint32_t)__popcll(km);
+            if (Op::kSecond) {
+                const bool k2 = valid && op.pred2(k[g + r], a[r]);
+                c2 += k2 ? 1u : 0u;
+                add_runs(op.runs2(), op.src(k[g + r]), k2, valid, lane);
+            }
+            __syncthreads();
+            uint32_t before = 0, total = 0;
+#pragma unroll
+            for (uint32_t x = 0; x < kCT / 64; ++x) {
+                const uint32_t v = s_w[r & 1][x];
+                before += x < wv ? v : 0u;
+                total += v;
+            }
+            if (valid) op.emit(i, k[g + r], prev, next, kp, pos + before + mask_rank(km), a[r]);
+            pos += total;
+        }
+    }
+    if (Op::kSecond) {
+        c2 = wave_sum32(c2);
+        if (lane == 0) s_w2[wv] = c2;
+        __syncthreads();
+        if (threadIdx.x == 0) tile_cnt2[blockIdx.x] = s_w2[0] + s_w2[1] + s_w2[2] + s_w2[3];
+    }
 }
 
 // Unique keys of the sorted list (sentinels dropped), with each row's bounds
 // [first, last) in the unique list.
 struct UniqueOp {
-    const uint64_t *keys;
+    const uint64_t *sorted;
     uint64_t n;
     uint32_t b;
     uint64_t *uk, *first, *last;
+    struct Aux {};
+    static constexpr bool kNeedsPrev = true, kRuns = false, kSecond = false;
     __device__ uint64_t size() const { return n; }
-    __device__ bool pred(uint64_t i) const {
-        const uint64_t k = keys[i];
-        return k != kSentinel && (i == 0 || keys[i - 1] != k);
+    __device__ const uint64_t *keys() const { return sorted; }
+    __device__ Aux fetch(uint64_t) const { return Aux{}; }
+    __device__ bool pred(uint64_t i, uint64_t k, uint64_t prev, Aux) const {
+        return k != kSentinel && (i == 0 || prev != k);
     }
-    __device__ void emit(uint64_t i, bool kept, uint64_t pos) const {
-        const uint64_t k = keys[i];
+    __device__ bool pred2(uint64_t, Aux) const { return false; }
+    __device__ uint32_t src(uint64_t k) const { return (uint32_t)(k >> b); }
+    __device__ uint32_t *runs() const { return nullptr; }
+    __device__ uint32_t *runs2() const { return nullptr; }
+    __device__ void emit(uint64_t i, uint64_t k, uint64_t prev, uint64_t next, bool kept, uint64_t pos, Aux) const {
         if (k == kSentinel) return;
         if (kept) uk[pos] = k;
         const uint64_t s = k >> b;
-        if (row_first(keys, i, b, s)) first[s] = pos;  // a row's first key is always kept
-        if (row_last(keys, i, n, b, s)) last[s] = pos + (kept ? 1u : 0u);
+        if (i == 0 || (prev >> b) != s) first[s] = pos;  // a row's first key is always kept
+        if (i + 1 == n || (next >> b) != s) last[s] = pos + (kept ? 1u : 0u);
     }
 };
 
-// Oriented arcs (keep s -> d iff deg s < deg d, or equal and d < s), written as
-// relabelled ids; end[s] - start[s] = d+(s).
-struct OrientOp {
+// The orientation rule: keep s -> d iff deg s < deg d, or equal and d < s.
+__device__ __forceinline__ bool out_arc(uint32_t ds, uint32_t dd, uint32_t s, uint32_t d) {
+    return ds < dd || (ds == dd && d < s);
+}
+
+// Count pass of the oriented arcs over the unique symmetric keys (s << b | d,
+// old ids): kept arcs per tile, and d+ per old source id.
+struct OrientCountOp {
     const uint64_t *uk;
     const uint64_t *d_m2;
     uint32_t b;
     uint64_t mask;
-    const uint2 *dd;  // {deg, d+} per vertex
-    const uint32_t *nid;
-    uint32_t *col;
-    uint64_t *start, *end;
+    const uint32_t *deg;  // symmetric degree per old id
+    uint32_t *dout;
+    struct Aux { uint32_t ds, dd; };
+    static constexpr bool kNeedsPrev = false, kRuns = true, kSecond = false;
     __device__ uint64_t size() const { return *d_m2; }
-    __device__ bool pred(uint64_t i) const {
-        const uint64_t k = uk[i];
-        const uint32_t s = (uint32_t)(k >> b), d = (uint32_t)(k & mask);
-        const uint32_t ds = dd[s].x, dg = dd[d].x;
-        return ds < dg || (ds == dg && d < s);
+    __device__ const uint64_t *keys() const { return uk; }
+    __device__ Aux fetch(uint64_t k) const { return Aux{deg[(uint32_t)(k >> b)], deg[(uint32_t)(k & mask)]}; }
+    __device__ bool pred(uint64_t, uint64_t k, uint64_t, Aux a) const {
+        return out_arc(a.ds, a.dd, (uint32_t)(k >> b), (uint32_t)(k & mask));
     }
-    __device__ void emit(uint64_t i, bool kept, uint64_t pos) const {
-        const uint64_t k = uk[i], s = k >> b;
-        if (kept) col[pos] = nid[k & mask];
-        const uint64_t n = *d_m2;
-        if (row_first(uk, i, b, s)) start[s] = pos;
-        if (row_last(uk, i, n, b, s)) end[s] = pos + (kept ? 1u : 0u);
-    }
+    __device__ bool pred2(uint64_t, Aux) const { return false; }
+    __device__ uint32_t src(uint64_t k) const { return (uint32_t)(k >> b); }
+    __device__ uint32_t *runs() const { return dout; }
+    __device__ uint32_t *runs2() const { return nullptr; }
+    __device__ void emit(uint64_t, uint64_t, uint64_t, uint64_t, bool, uint64_t, Aux) const {}
 };
 
-// Target-side in-lists: in row s, the sources d of arcs d -> s (the reverse of
-// an oriented arc) with d+(d) <= d+(s): the arcs the split count handles from
-// the target side.  Rows stay ascending.
-struct InOp {
+// Per old vertex id: its new id, d+, the start of its oriented row (low 32
+// bits) and its symmetric degree -- everything a kept arc's endpoints add to
+// the predicates and to the outputs, in one 16-byte gather.
+struct __align__(16) VRec {
+    uint32_t nid, dplus, off_lo, deg;
+};
+
+__device__ __forceinline__ uint64_t seg_word(const VRec &v) { return (uint64_t)v.off_lo | ((uint64_t)v.dplus << 32); }
+
+// An arc the split count handles from the target side: in row s, the source d
+// of an oriented arc d -> s with d+(d) <= d+(s).
+__device__ __forceinline__ bool in_arc(const VRec &vs, const VRec &vd, uint32_t s, uint32_t d) {
+    return !out_arc(vs.deg, vd.deg, s, d) && vd.dplus <= vs.dplus;
+}
+
+// Scatter pass of the oriented arcs: relabelled target ids and, if asked, their
+// segment words (start | d+ << 32 of the target's row); counts the in-list
+// arcs of the same keys per tile and per source for the in-list pass.
+struct OrientEmitOp {
     const uint64_t *uk;
     const uint64_t *d_m2;
     uint32_t b;
     uint64_t mask;
-    const uint2 *dd;
-    const uint32_t *nid;
+    const VRec *rec;
     uint32_t *col;
-    uint64_t *start, *end;
+    uint64_t *col_seg;
+    uint32_t *din;
+    struct Aux { VRec vs, vd; };
+    static constexpr bool kNeedsPrev = false, kRuns = false, kSecond = true;
     __device__ uint64_t size() const { return *d_m2; }
-    __device__ bool pred(uint64_t i) const {
-        const uint64_t k = uk[i];
-        const uint32_t s = (uint32_t)(k >> b), d = (uint32_t)(k & mask);
-        const uint2 vs = dd[s], vd = dd[d];  // one 8-byte gather per endpoint
-        const bool out_arc = vs.x < vd.x || (vs.x == vd.x && d < s);
-        return !out_arc && vd.y <= vs.y;
+    __device__ const uint64_t *keys() const { return uk; }
+    __device__ Aux fetch(uint64_t k) const { return Aux{rec[(uint32_t)(k >> b)], rec[(uint32_t)(k & mask)]}; }
+    __device__ bool pred(uint64_t, uint64_t k, uint64_t, const Aux &a) const {
+        return out_arc(a.vs.deg, a.vd.deg, (uint32_t)(k >> b), (uint32_t)(k & mask));
     }
-    __device__ void emit(uint64_t i, bool kept, uint64_t pos) const {
-        const uint64_t k = uk[i], s = k >> b;
-        if (kept) col[pos] = nid[k & mask];
-        const uint64_t n = *d_m2;
-        if (row_first(uk, i, b, s)) start[s] = pos;
-        if (row_last(uk, i, n, b, s)) end[s] = pos + (kept ? 1u : 0u);
+    __device__ bool pred2(uint64_t k, const Aux &a) const {
+        return in_arc(a.vs, a.vd, (uint32_t)(k >> b), (uint32_t)(k & mask));
+    }
+    __device__ uint32_t src(uint64_t k) const { return (uint32_t)(k >> b); }
+    __device__ uint32_t *runs() const { return nullptr; }
+    __device__ uint32_t *runs2() const { return din; }
+    __device__ void emit(uint64_t, uint64_t, uint64_t, uint64_t, bool kept, uint64_t pos, const Aux &a) const {
+        if (!kept) return;
+        col[pos] = a.vd.nid;
+        if (col_seg) col_seg[pos] = seg_word(a.vd);
+    }
+};
+
+// Scatter pass of the target-side in-lists: in row s, the sources d of in-list
+// arcs, ascending, and their segment words.
+struct InEmitOp {
+    const uint64_t *uk;
+    const uint64_t *d_m2;
+    uint32_t b;
+    uint64_t mask;
+    const VRec *rec;
+    uint32_t *col;
+    uint64_t *col_seg;
+    struct Aux { VRec vs, vd; };
+    static constexpr bool kNeedsPrev = false, kRuns = false, kSecond = false;
+    __device__ uint64_t size() const { return *d_m2; }
+    __device__ const uint64_t *keys() const { return uk; }
+    __device__ Aux fetch(uint64_t k) const { return Aux{rec[(uint32_t)(k >> b)], rec[(uint32_t)(k & mask)]}; }
+    __device__ bool pred(uint64_t, uint64_t k, uint64_t, const Aux &a) const {
+        return in_arc(a.vs, a.vd, (uint32_t)(k >> b), (uint32_t)(k & mask));
+    }
+    __device__ bool pred2(uint64_t, const Aux &) const { return false; }
+    __device__ uint32_t src(uint64_t k) const { return (uint32_t)(k >> b); }
+    __device__ uint32_t *runs() const { return nullptr; }
+    __device__ uint32_t *runs2() const { return nullptr; }
+    __device__ void emit(uint64_t, uint64_t, uint64_t, uint64_t, bool kept, uint64_t pos, const Aux &a) const {
+        if (!kept) return;
+        col[pos] = a.vd.nid;
+        if (col_seg) col_seg[pos] = seg_word(a.vd);
     }
 };
 
@@ -202,22 +346,16 @@ static lstc_status compact_pass(const Op &op, uint64_t n_max, uint32_t *tile_cnt
     hipLaunchKernelGGL(k_compact_count<Op>, dim3((uint32_t)tiles), dim3(kCT), 0, st, op, tile_cnt);
     LSTC_LAUNCHED("k_compact_count");
     LSTC_TRY(scan_exclusive_u32_to_u64(tile_cnt, tile_off, tiles, d_total, scan_ws, scan_bytes, st));
-    hipLaunchKernelGGL(k_compact_scatter<Op>, dim3((uint32_t)tiles), dim3(kCT), 0, st, op, (const uint64_t *)tile_off);
+    hipLaunchKernelGGL(k_compact_scatter<Op>, dim3((uint32_t)tiles), dim3(kCT), 0, st, op, (const uint64_t *)tile_off,
+                       (uint32_t *)nullptr);
     LSTC_LAUNCHED("k_compact_scatter");
     return LSTC_OK;
 }
 
-// Row lengths from the compactions' row bounds; deg also goes into dd[v].x, d+ into dd[v].y.
+// Row lengths from the unique pass's row bounds (the symmetric degree).
 __global__ void k_row_lengths(const uint64_t *__restrict__ first, const uint64_t *__restrict__ last, uint32_t n,
-                              uint32_t *__restrict__ len, uint2 *__restrict__ dd, int field) {
-    GRID_STRIDE(v, n) {
-        const uint32_t l = (uint32_t)(last[v] - first[v]);
-        len[v] = l;
-        if (dd) {
-            if (field == 0) dd[v] = make_uint2(l, 0u);
-            else dd[v].y = l;
-        }
-    }
+                              uint32_t *__restrict__ len) {
+    GRID_STRIDE(v, n) len[v] = (uint32_t)(last[v] - first[v]);
 }
 
 __global__ void k_nonzero(const uint32_t *__restrict__ deg, uint32_t n, int compact,
@@ -256,23 +394,13 @@ __global__ void k_orient_offsets(const uint32_t *__restrict__ keepv, const uint6
     if (blockIdx.x == 0 && threadIdx.x == 0) off[*d_n_out] = *d_m;
 }
 
-// Segment words (start | length << 32 of the row of every entry) of the
-// oriented columns and the in-lists, with the sizes still on the device:
-// sizes[1] rows, sizes[2] arcs, sizes[3] in-list entries.
-__global__ void k_prep_seg_words(const uint64_t *__restrict__ sizes, const uint64_t *__restrict__ off,
-                                 const uint32_t *__restrict__ col, const uint32_t *__restrict__ in_col,
-                                 uint64_t *__restrict__ col_seg, uint64_t *__restrict__ in_seg) {
-    const uint64_t n_rows = sizes[1], m = sizes[2], n_in = sizes[3];
-    GRID_STRIDE(i, m + n_in) {
-        const bool in = i >= m;
-        const uint32_t v = in ? in_col[i - m] : col[i];
-        uint64_t w = 0;
-        if (v < n_rows) {
-            const uint64_t b = off[v];
-            w = (uint64_t)(uint32_t)b | ((off[v + 1] - b) << 32);
-        }
-        if (in) in_seg[i - m] = w; else col_seg[i] = w;
-    }
+// Per-vertex records for the oriented and in-list scatters (old ids; doscan is
+// the exclusive scan of d+ over old ids, which is the oriented row start of the
+// vertex's new id, since the relabelling keeps the order).
+__global__ void k_vrec(const uint32_t *__restrict__ nid, const uint32_t *__restrict__ dout,
+                       const uint64_t *__restrict__ doscan, const uint32_t *__restrict__ deg, uint32_t n,
+                       VRec *__restrict__ rec) {
+    GRID_STRIDE(v, n) rec[v] = VRec{nid[v], dout[v], (uint32_t)doscan[v], deg[v]};
 }
 
 struct PrepLayout {
@@ -280,7 +408,7 @@ struct PrepLayout {
     uint32_t *tile_cnt;
     uint64_t *tile_off;
     uint32_t *deg, *keepv, *nid, *dout, *din;
-    uint2 *dd;
+    VRec *rec;
     uint64_t *degscan, *nidscan, *doscan, *first, *last, *discan;
     uint64_t *scal;  // [0]=m2 [1]=n_out [2]=m [3]=in-list entries
     uint32_t *bad;
@@ -305,7 +433,7 @@ static PrepLayout prep_layout(void *ws, size_t ws_bytes, uint32_t n, uint64_t m_
     L.nid = c.take<uint32_t>((size_t)n + 1);
     L.dout = c.take<uint32_t>((size_t)n + 1);
     L.din = c.take<uint32_t>((size_t)n + 1);
-    L.dd = c.take<uint2>((size_t)n + 1);
+    L.rec = c.take<VRec>((size_t)n + 1);
     L.discan = c.take<uint64_t>((size_t)n + 1);
     L.degscan = c.take<uint64_t>((size_t)n + 1);
     L.nidscan = c.take<uint64_t>((size_t)n + 1);
@@ -333,24 +461,26 @@ extern "C" lstc_status lstc_clean_orient_workspace(uint32_t n, uint64_t m_raw, s
 }
 
 // Enqueues the whole pipeline without synchronising.  On return *d_sizes points
-// at device words {m2, n_out, m, -} and *d_bad at the out-of-range flag; both are
-// valid once the stream reaches this point.
+// at device words {m2, n_out, m, in-list entries} and *d_bad at the out-of-range
+// flag; both are valid once the stream reaches this point.
 lstc_status lstc::clean_orient_launch(const uint32_t *d_src, const uint32_t *d_dst, uint64_t m_raw, uint32_t n,
                                       int compact, uint64_t *d_sym_off, uint32_t *d_sym_col, uint64_t *d_off,
-                                      uint32_t *d_col, uint64_t *d_in_off, uint32_t *d_in_col, void *d_ws,
-                                      size_t ws_bytes, hipStream_t st,
+                                      uint32_t *d_col, uint64_t *d_in_off, uint32_t *d_in_col, uint64_t *d_col_seg,
+                                      uint64_t *d_in_seg, void *d_ws, size_t ws_bytes, hipStream_t st,
                                       const uint64_t **d_sizes, const uint32_t **d_bad) {
     if (n == 0xFFFFFFFFu) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: n must be < 2^32-1");
     if (!d_off) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: NULL output");
     if ((d_sym_off == nullptr) != (d_sym_col == nullptr))
         return fail(LSTC_ERR_INVALID, "lstc_clean_orient: give both symmetric outputs or neither");
     if (m_raw && (!d_src || !d_dst || !d_col)) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: NULL edge arrays");
+    if (d_in_seg && !d_in_col) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: in-list segment words need in-lists");
     PrepLayout L = prep_layout(d_ws, ws_bytes, n, m_raw);
     if (!d_ws || ws_bytes < L.bytes)
         return fail(LSTC_ERR_WORKSPACE, "lstc_clean_orient: workspace %zu < %zu bytes", ws_bytes, L.bytes);
     const uint32_t b = (uint32_t)bits_for(n ? n : 1);
     const uint64_t mask = (b >= 64) ? ~0ull : ((1ull << b) - 1);
     const uint64_t nk = 2 * m_raw;
+    const uint64_t tiles = (nk + kCTile - 1) / kCTile;
     const uint32_t grid_k = (uint32_t)hmin<uint64_t>((nk + 255) / 256 + 1, (uint64_t)device_cus() * 32);
     const uint32_t grid_n = (uint32_t)hmin<uint64_t>(((uint64_t)n + 255) / 256 + 1, (uint64_t)device_cus() * 32);
 
@@ -358,6 +488,7 @@ lstc_status lstc::clean_orient_launch(const uint32_t *d_src, const uint32_t *d_d
     LSTC_TRY(fill_u32(L.bad, 0, 4, st));
     LSTC_TRY(fill_u32(L.deg, 0, (uint64_t)n + 1, st));
     LSTC_TRY(fill_u32(L.dout, 0, (uint64_t)n + 1, st));
+    LSTC_TRY(fill_u32(L.din, 0, (uint64_t)n + 1, st));
     if (m_raw) {
         hipLaunchKernelGGL(k_make_keys, dim3(grid_k), dim3(256), 0, st, d_src, d_dst, m_raw, n, b, L.keys, L.bad);
         LSTC_LAUNCHED("k_make_keys");
@@ -368,8 +499,8 @@ lstc_status lstc::clean_orient_launch(const uint32_t *d_src, const uint32_t *d_d
         LSTC_TRY(fill_u64(L.first, 0, (uint64_t)n + 1, st));
         LSTC_TRY(fill_u64(L.last, 0, (uint64_t)n + 1, st));
         LSTC_TRY(compact_pass(UniqueOp{sorted, nk, b, uk, L.first, L.last}, nk, L.tile_cnt, L.tile_off, L.scal + 0,
-                         L.scan_ws, L.scan_bytes, st));
-        hipLaunchKernelGGL(k_row_lengths, dim3(grid_n), dim3(256), 0, st, L.first, L.last, n, L.deg, L.dd, 0);
+                              L.scan_ws, L.scan_bytes, st));
+        hipLaunchKernelGGL(k_row_lengths, dim3(grid_n), dim3(256), 0, st, L.first, L.last, n, L.deg);
         LSTC_LAUNCHED("k_row_lengths(deg)");
         hipLaunchKernelGGL(k_nonzero, dim3(grid_n), dim3(256), 0, st, L.deg, n, compact, L.keepv);
         LSTC_LAUNCHED("k_nonzero");
@@ -382,30 +513,33 @@ lstc_status lstc::clean_orient_launch(const uint32_t *d_src, const uint32_t *d_d
             hipLaunchKernelGGL(k_sym_cols, dim3(grid_k), dim3(256), 0, st, uk, L.scal + 0, mask, L.nid, d_sym_col);
             LSTC_LAUNCHED("k_sym_cols");
         }
-        // oriented arcs (m -> scal[2]) and d+ per vertex
-        LSTC_TRY(fill_u64(L.first, 0, (uint64_t)n + 1, st));
-        LSTC_TRY(fill_u64(L.last, 0, (uint64_t)n + 1, st));
-        LSTC_TRY(compact_pass(OrientOp{uk, L.scal + 0, b, mask, L.dd, L.nid, d_col, L.first, L.last}, nk, L.tile_cnt,
-                         L.tile_off, L.scal + 2, L.scan_ws, L.scan_bytes, st));
-        hipLaunchKernelGGL(k_row_lengths, dim3(grid_n), dim3(256), 0, st, L.first, L.last, n, L.dout, L.dd, 1);
-        LSTC_LAUNCHED("k_row_lengths(d+)");
+        // oriented arcs: count pass (tile counts, d+ per vertex; m -> scal[2]), offsets, per-vertex records
+        hipLaunchKernelGGL(k_compact_count<OrientCountOp>, dim3((uint32_t)tiles), dim3(kCT), 0, st,
+                           OrientCountOp{uk, L.scal + 0, b, mask, L.deg, L.dout}, L.tile_cnt);
+        LSTC_LAUNCHED("k_compact_count<orient>");
+        LSTC_TRY(scan_exclusive_u32_to_u64(L.tile_cnt, L.tile_off, tiles, L.scal + 2, L.scan_ws, L.scan_bytes, st));
         LSTC_TRY(scan_exclusive_u32_to_u64(L.dout, L.doscan, n, nullptr, L.scan_ws, L.scan_bytes, st));
         hipLaunchKernelGGL(k_orient_offsets, dim3(grid_n), dim3(256), 0, st, L.keepv, L.nidscan, L.doscan, n,
                            L.scal + 1, L.scal + 2, d_off);
         LSTC_LAUNCHED("k_orient_offsets");
+        hipLaunchKernelGGL(k_vrec, dim3(grid_n), dim3(256), 0, st, L.nid, L.dout, L.doscan, L.deg, n, L.rec);
+        LSTC_LAUNCHED("k_vrec");
+        // scatter pass: columns (and segment words); counts the in-list arcs per tile and per vertex
+        hipLaunchKernelGGL(k_compact_scatter<OrientEmitOp>, dim3((uint32_t)tiles), dim3(kCT), 0, st,
+                           OrientEmitOp{uk, L.scal + 0, b, mask, L.rec, d_col, d_col_seg, L.din},
+                           (const uint64_t *)L.tile_off, L.tile_cnt);
+        LSTC_LAUNCHED("k_compact_scatter<orient>");
         if (d_in_off) {
             // in-lists of the arcs counted from the target side (entries -> scal[3])
-            LSTC_TRY(fill_u64(L.first, 0, (uint64_t)n + 1, st));
-            LSTC_TRY(fill_u64(L.last, 0, (uint64_t)n + 1, st));
-            LSTC_TRY(compact_pass(InOp{uk, L.scal + 0, b, mask, L.dd, L.nid, d_in_col, L.first, L.last}, nk, L.tile_cnt,
-                             L.tile_off, L.scal + 3, L.scan_ws, L.scan_bytes, st));
-            hipLaunchKernelGGL(k_row_lengths, dim3(grid_n), dim3(256), 0, st, L.first, L.last, n, L.din,
-                               (uint2 *)nullptr, 0);
-            LSTC_LAUNCHED("k_row_lengths(in)");
+            LSTC_TRY(scan_exclusive_u32_to_u64(L.tile_cnt, L.tile_off, tiles, L.scal + 3, L.scan_ws, L.scan_bytes, st));
             LSTC_TRY(scan_exclusive_u32_to_u64(L.din, L.discan, n, nullptr, L.scan_ws, L.scan_bytes, st));
             hipLaunchKernelGGL(k_orient_offsets, dim3(grid_n), dim3(256), 0, st, L.keepv, L.nidscan, L.discan, n,
                                L.scal + 1, L.scal + 3, d_in_off);
             LSTC_LAUNCHED("k_orient_offsets(in)");
+            hipLaunchKernelGGL(k_compact_scatter<InEmitOp>, dim3((uint32_t)tiles), dim3(kCT), 0, st,
+                               InEmitOp{uk, L.scal + 0, b, mask, L.rec, d_in_col, d_in_seg},
+                               (const uint64_t *)L.tile_off, (uint32_t *)nullptr);
+            LSTC_LAUNCHED("k_compact_scatter<in>");
         }
     } else {
         // no edges: every vertex is isolated
@@ -443,13 +577,7 @@ extern "C" lstc_status lstc_clean_orient_split(const uint32_t *d_src, const uint
     const uint64_t *d_sizes = nullptr;
     const uint32_t *d_bad = nullptr;
     LSTC_TRY(clean_orient_launch(d_src, d_dst, m_raw, n, compact, d_sym_off, d_sym_col, d_off, d_col, d_in_off,
-                                 d_in_col, d_ws, ws_bytes, st, &d_sizes, &d_bad));
-    if (m_raw) {
-        const uint32_t grid = (uint32_t)hmin<uint64_t>((2 * m_raw + 255) / 256, (uint64_t)device_cus() * 32);
-        hipLaunchKernelGGL(k_prep_seg_words, dim3(grid), dim3(256), 0, st, d_sizes, (const uint64_t *)d_off,
-                           (const uint32_t *)d_col, (const uint32_t *)d_in_col, d_col_seg, d_in_seg);
-        LSTC_LAUNCHED("k_prep_seg_words");
-    }
+                                 d_in_col, d_col_seg, d_in_seg, d_ws, ws_bytes, st, &d_sizes, &d_bad));
     uint64_t h[4];
     uint32_t bad = 0;
     LSTC_HIP(hipMemcpyAsync(h, d_sizes, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -474,7 +602,7 @@ extern "C" lstc_status lstc_clean_orient(const uint32_t *d_src, const uint32_t *
     if ((d_in_off == nullptr) != (d_in_col == nullptr))
         return fail(LSTC_ERR_INVALID, "lstc_clean_orient: give both in-list outputs or neither");
     LSTC_TRY(clean_orient_launch(d_src, d_dst, m_raw, n, compact, d_sym_off, d_sym_col, d_off, d_col, d_in_off,
-                                 d_in_col, d_ws, ws_bytes, st, &d_sizes, &d_bad));
+                                 d_in_col, nullptr, nullptr, d_ws, ws_bytes, st, &d_sizes, &d_bad));
     uint64_t h[4];
     uint32_t bad = 0;
     LSTC_HIP(hipMemcpyAsync(h, d_sizes, sizeof(h), hipMemcpyDeviceToHost, st));

@@ -157,9 +157,12 @@ lstc_status scan_exclusive_u32_to_u64(const uint32_t *in, uint64_t *out, uint64_
 }
 
 // ------------------------------------------------------------------ radix sort
-// Tile = 4 waves x 16 rounds x 64 lanes = 4096 keys.  Key i of a tile sits at
-// (wave, round, lane) = (i / 1024, (i / 64) % 16, i % 64), so ranking in that
-// lexicographic order is stable.
+// LSD over 8-bit digits; per digit a histogram pass (per tile), one scan of the
+// digit-major tile counts, and a scatter pass.  Tile = 4 waves x 16 rounds x 64
+// lanes = 4096 keys.  Key i of a tile sits at (wave, round, lane) = (i / 1024,
+// (i / 64) % 16, i % 64), so ranking in that lexicographic order is stable.
+// Both passes load all of a thread's keys before they use any, so every wave
+// has its 16 loads in flight at once.
 static constexpr int kSortWaves = 4;
 static constexpr int kSortRounds = 16;
 static constexpr uint64_t kSortTile = kSortWaves * kSortRounds * 64;
@@ -170,14 +173,18 @@ __global__ __launch_bounds__(kSortWaves * 64) void k_radix_hist(const uint64_t *
                                                                 uint32_t *__restrict__ hist) {
     __shared__ uint32_t cnt[256];
     cnt[threadIdx.x] = 0;
-    __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-    // kSortTile keys per block, kSortWaves*64 threads: kSortRounds keys per thread
-#pragma unroll 4
+    uint64_t k[kSortRounds];
+#pragma unroll
     for (int r = 0; r < kSortRounds; ++r) {
-        uint64_t i = base + (uint64_t)r * (kSortWaves * 64) + threadIdx.x;
-        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+        const uint64_t i = base + (uint64_t)r * (kSortWaves * 64) + threadIdx.x;
+        k[r] = i < n ? keys[i] : 0;
     }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r)
+        if (base + (uint64_t)r * (kSortWaves * 64) + threadIdx.x < n)
+            atomicAdd(&cnt[(uint32_t)(k[r] >> shift) & 255u], 1u);
     __syncthreads();
     hist[(uint64_t)threadIdx.x * num_tiles + blockIdx.x] = cnt[threadIdx.x];
 }
@@ -196,7 +203,6 @@ __global__ __launch_bounds__(kSortWaves * 64) void k_radix_scatter(
     __shared__ uint64_t stage[kSortTile];
     const uint32_t tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     for (int w = 0; w < kSortWaves; ++w) wcnt[w][tid] = 0;
-    __syncthreads();
     const uint64_t tile0 = (uint64_t)blockIdx.x * kSortTile;
     const uint64_t base = tile0 + (uint64_t)wv * (kSortRounds * 64);
     uint64_t k[kSortRounds];
@@ -204,8 +210,12 @@ __global__ __launch_bounds__(kSortWaves * 64) void k_radix_scatter(
 #pragma unroll
     for (int r = 0; r < kSortRounds; ++r) {
         const uint64_t i = base + (uint64_t)r * 64 + lane;
-        const bool valid = i < n;
-        k[r] = valid ? keys[i] : 0;
+        k[r] = i < n ? keys[i] : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r) {
+        const bool valid = base + (uint64_t)r * 64 + lane < n;
         const uint32_t d = (uint32_t)(k[r] >> shift) & 255u;
         uint64_t peers = ballot(valid);
 #pragma unroll

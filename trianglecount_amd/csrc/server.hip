@@ -263,9 +263,7 @@ static lstc_status admit(lstc_server *S) {
         if (rc != LSTC_OK) return rc;
         // compact = 0 keeps n rows, so the count needs no host-side sizes
         LSTC_TRY(clean_orient_launch(r->src, r->dst, r->spec.m_raw, n, 0, nullptr, nullptr, r->off, r->col, r->in_off,
-                                     r->in_col, r->ws, r->ws_bytes, st, nullptr, &r->d_bad));
-        LSTC_TRY(segments_launch_dev(r->off, r->col, r->in_off, r->in_col, n, r->spec.m_raw, r->col_seg, r->in_seg,
-                                     st));
+                                     r->in_col, r->col_seg, r->in_seg, r->ws, r->ws_bytes, st, nullptr, &r->d_bad));
         // the out-of-range flag of the cleaning comes back with the preparation
         LSTC_HIP(hipMemcpyAsync(S->h_flags + r->flag_slot, r->d_bad, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         LSTC_HIP(hipEventCreateWithFlags(&r->prep_done, hipEventDisableTiming));
