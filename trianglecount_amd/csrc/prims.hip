@@ -157,23 +157,28 @@ lstc_status scan_exclusive_u32_to_u64(const uint32_t *in, uint64_t *out, uint64_
 }
 
 // ------------------------------------------------------------------ radix sort
-// LSD over 8-bit digits; per digit a histogram pass (per tile), one scan of the
-// digit-major tile counts, and a scatter pass.  Tile = 4 waves x 16 rounds x 64
-// lanes = 4096 keys.  Key i of a tile sits at (wave, round, lane) = (i / 1024,
-// (i / 64) % 16, i % 64), so ranking in that lexicographic order is stable.
-// Both passes load all of a thread's keys before they use any, so every wave
-// has its 16 loads in flight at once.
+// LSD over 8-bit digits; per digit a histogram pass, one scan of the
+// digit-major block counts, and a scatter pass.  A block owns kSortSub
+// consecutive sub-tiles of 4 waves x 16 rounds x 64 lanes = 4096 keys; key i of
+// a sub-tile sits at (wave, round, lane) = (i / 1024, (i / 64) % 16, i % 64), so
+// ranking in that lexicographic order, sub-tile after sub-tile, is stable.
+// Owning several sub-tiles keeps the count matrix (one count per block and
+// digit, written and read at a stride) small next to the keys.  Both passes
+// load a sub-tile's keys before they use any, and the next sub-tile's while
+// they work on the current one.
 static constexpr int kSortWaves = 4;
 static constexpr int kSortRounds = 16;
-static constexpr uint64_t kSortTile = kSortWaves * kSortRounds * 64;
+static constexpr uint32_t kSortSub = 4;
+static constexpr uint64_t kSortTile = kSortWaves * kSortRounds * 64;  // keys per sub-tile
+static constexpr uint64_t kSortBlock = kSortTile * kSortSub;          // keys per block
 
 __global__ __launch_bounds__(kSortWaves * 64) void k_radix_hist(const uint64_t *__restrict__ keys,
                                                                 uint64_t n, uint32_t shift,
-                                                                uint32_t num_tiles,
+                                                                uint32_t num_blocks,
                                                                 uint32_t *__restrict__ hist) {
     __shared__ uint32_t cnt[256];
     cnt[threadIdx.x] = 0;
-    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+    const uint64_t base = (uint64_t)blockIdx.x * kSortBlock;
     uint64_t k[kSortRounds];
 #pragma unroll
     for (int r = 0; r < kSortRounds; ++r) {
@@ -181,91 +186,128 @@ __global__ __launch_bounds__(kSortWaves * 64) void k_radix_hist(const uint64_t *
         k[r] = i < n ? keys[i] : 0;
     }
     __syncthreads();
+    for (uint32_t sub = 0; sub < kSortSub; ++sub) {
+        const uint64_t b0 = base + sub * kSortTile;
+        uint64_t kn[kSortRounds];
+        if (sub + 1 < kSortSub) {
 #pragma unroll
-    for (int r = 0; r < kSortRounds; ++r)
-        if (base + (uint64_t)r * (kSortWaves * 64) + threadIdx.x < n)
-            atomicAdd(&cnt[(uint32_t)(k[r] >> shift) & 255u], 1u);
+            for (int r = 0; r < kSortRounds; ++r) {
+                const uint64_t i = b0 + kSortTile + (uint64_t)r * (kSortWaves * 64) + threadIdx.x;
+                kn[r] = i < n ? keys[i] : 0;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kSortRounds; ++r)
+            if (b0 + (uint64_t)r * (kSortWaves * 64) + threadIdx.x < n)
+                atomicAdd(&cnt[(uint32_t)(k[r] >> shift) & 255u], 1u);
+        if (sub + 1 < kSortSub) {
+#pragma unroll
+            for (int r = 0; r < kSortRounds; ++r) k[r] = kn[r];
+        }
+    }
     __syncthreads();
-    hist[(uint64_t)threadIdx.x * num_tiles + blockIdx.x] = cnt[threadIdx.x];
+    hist[(uint64_t)threadIdx.x * num_blocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Scatter of one tile: keys are ranked stably per digit (ballot match inside a
-// wave, running counts across rounds and waves), staged in LDS in digit order,
-// then written out by consecutive threads, so each digit's run of the tile
-// leaves as contiguous stores instead of one scattered store per key.
+// Scatter of a block's sub-tiles, in order: keys are ranked stably per digit
+// (ballot match inside a wave, running counts across rounds and waves), staged
+// in LDS in digit order, then written out by consecutive threads, so each
+// digit's run of the sub-tile leaves as contiguous stores; the block's running
+// offset of each digit advances by the sub-tile's count.
 __global__ __launch_bounds__(kSortWaves * 64) void k_radix_scatter(
     const uint64_t *__restrict__ keys, uint64_t *__restrict__ out, uint64_t n, uint32_t shift,
-    uint32_t num_tiles, const uint64_t *__restrict__ goff) {
+    uint32_t num_blocks, const uint64_t *__restrict__ goff) {
     __shared__ uint32_t wcnt[kSortWaves][256];
     __shared__ uint32_t dstart[256];
     __shared__ uint32_t wsum[kSortWaves];
     __shared__ uint64_t gbase[256];
     __shared__ uint64_t stage[kSortTile];
     const uint32_t tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-    for (int w = 0; w < kSortWaves; ++w) wcnt[w][tid] = 0;
-    const uint64_t tile0 = (uint64_t)blockIdx.x * kSortTile;
-    const uint64_t base = tile0 + (uint64_t)wv * (kSortRounds * 64);
+    uint64_t run = goff[(uint64_t)tid * num_blocks + blockIdx.x];  // digit tid's next output position
     uint64_t k[kSortRounds];
-    uint32_t rank[kSortRounds];
+    {
+        const uint64_t b = (uint64_t)blockIdx.x * kSortBlock + (uint64_t)wv * (kSortRounds * 64);
 #pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) {
-        const uint64_t i = base + (uint64_t)r * 64 + lane;
-        k[r] = i < n ? keys[i] : 0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) {
-        const bool valid = base + (uint64_t)r * 64 + lane < n;
-        const uint32_t d = (uint32_t)(k[r] >> shift) & 255u;
-        uint64_t peers = ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bm = ballot(bit);
-            peers &= bit ? bm : ~bm;
+        for (int r = 0; r < kSortRounds; ++r) {
+            const uint64_t i = b + (uint64_t)r * 64 + lane;
+            k[r] = i < n ? keys[i] : 0;
         }
-        uint32_t before = 0;
-        if (valid) before = wcnt[wv][d];
-        rank[r] = before + mask_rank(peers);
-        if (valid && mask_rank(peers) == 0) wcnt[wv][d] = before + (uint32_t)__popcll(peers);
     }
-    __syncthreads();
-    // thread tid owns digit tid: offsets of the waves inside the digit, the
-    // digit's start inside the tile, and its global base for this tile
-    uint32_t tot = 0;
-    for (int w = 0; w < kSortWaves; ++w) {
-        const uint32_t t = wcnt[w][tid];
-        wcnt[w][tid] = tot;
-        tot += t;
-    }
-    const uint32_t incl = wave_incl_scan(tot, lane);
-    if (lane == 63) wsum[wv] = incl;
-    __syncthreads();
-    uint32_t before = 0;
-    for (uint32_t w = 0; w < wv; ++w) before += wsum[w];
-    const uint32_t start = before + incl - tot;
-    dstart[tid] = start;
-    gbase[tid] = goff[(uint64_t)tid * num_tiles + blockIdx.x] - start;
-    __syncthreads();
+    for (uint32_t sub = 0; sub < kSortSub; ++sub) {
+        const uint64_t tile0 = (uint64_t)blockIdx.x * kSortBlock + sub * kSortTile;
+        if (tile0 >= n) break;  // block-uniform
+        const uint64_t base = tile0 + (uint64_t)wv * (kSortRounds * 64);
+        uint64_t kn[kSortRounds];
+        if (sub + 1 < kSortSub) {
 #pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) {
-        const uint64_t i = base + (uint64_t)r * 64 + lane;
-        if (i < n) {
+            for (int r = 0; r < kSortRounds; ++r) {
+                const uint64_t i = base + kSortTile + (uint64_t)r * 64 + lane;
+                kn[r] = i < n ? keys[i] : 0;
+            }
+        }
+        for (int w = 0; w < kSortWaves; ++w) wcnt[w][tid] = 0;
+        __syncthreads();
+        uint32_t rank[kSortRounds];
+#pragma unroll
+        for (int r = 0; r < kSortRounds; ++r) {
+            const bool valid = base + (uint64_t)r * 64 + lane < n;
             const uint32_t d = (uint32_t)(k[r] >> shift) & 255u;
-            stage[dstart[d] + wcnt[wv][d] + rank[r]] = k[r];
+            uint64_t peers = ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const bool bit = (d >> b) & 1u;
+                const uint64_t bm = ballot(bit);
+                peers &= bit ? bm : ~bm;
+            }
+            uint32_t before = 0;
+            if (valid) before = wcnt[wv][d];
+            rank[r] = before + mask_rank(peers);
+            if (valid && mask_rank(peers) == 0) wcnt[wv][d] = before + (uint32_t)__popcll(peers);
         }
-    }
-    __syncthreads();
-    const uint32_t tile_n = (uint32_t)(n - tile0 < kSortTile ? n - tile0 : kSortTile);
-    for (uint32_t i = tid; i < tile_n; i += kSortWaves * 64) {
-        const uint64_t key = stage[i];
-        out[gbase[(uint32_t)(key >> shift) & 255u] + i] = key;
+        __syncthreads();
+        // thread tid owns digit tid: offsets of the waves inside the digit, the
+        // digit's start inside the sub-tile, and its output base
+        uint32_t tot = 0;
+        for (int w = 0; w < kSortWaves; ++w) {
+            const uint32_t t = wcnt[w][tid];
+            wcnt[w][tid] = tot;
+            tot += t;
+        }
+        const uint32_t incl = wave_incl_scan(tot, lane);
+        if (lane == 63) wsum[wv] = incl;
+        __syncthreads();
+        uint32_t before = 0;
+        for (uint32_t w = 0; w < wv; ++w) before += wsum[w];
+        const uint32_t start = before + incl - tot;
+        dstart[tid] = start;
+        gbase[tid] = run - start;
+        run += tot;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kSortRounds; ++r) {
+            const uint64_t i = base + (uint64_t)r * 64 + lane;
+            if (i < n) {
+                const uint32_t d = (uint32_t)(k[r] >> shift) & 255u;
+                stage[dstart[d] + wcnt[wv][d] + rank[r]] = k[r];
+            }
+        }
+        __syncthreads();
+        const uint32_t tile_n = (uint32_t)(n - tile0 < kSortTile ? n - tile0 : kSortTile);
+        for (uint32_t i = tid; i < tile_n; i += kSortWaves * 64) {
+            const uint64_t key = stage[i];
+            out[gbase[(uint32_t)(key >> shift) & 255u] + i] = key;
+        }
+        if (sub + 1 < kSortSub) {
+#pragma unroll
+            for (int r = 0; r < kSortRounds; ++r) k[r] = kn[r];
+        }
+        __syncthreads();  // stage, wcnt and gbase are reused by the next sub-tile
     }
 }
 
 static size_t radix_layout(uint64_t n, uint32_t **hist, uint64_t **goff, void **scan_ws,
                            size_t *scan_bytes, void *ws, size_t ws_bytes) {
-    const uint64_t tiles = (n + kSortTile - 1) / kSortTile;
+    const uint64_t tiles = (n + kSortBlock - 1) / kSortBlock;
     Carver c(ws, ws_bytes);
     *hist = c.take<uint32_t>(tiles * 256);
     *goff = c.take<uint64_t>(tiles * 256);
@@ -287,7 +329,7 @@ lstc_status radix_sort_u64(uint64_t *keys, uint64_t *alt, uint64_t n, int key_bi
     uint32_t *hist; uint64_t *goff; void *sws; size_t sb;
     size_t need = radix_layout(n, &hist, &goff, &sws, &sb, ws, ws_bytes);
     if (!ws || ws_bytes < need) return fail(LSTC_ERR_WORKSPACE, "radix sort: workspace too small");
-    const uint64_t tiles = (n + kSortTile - 1) / kSortTile;
+    const uint64_t tiles = (n + kSortBlock - 1) / kSortBlock;
     if (tiles > 0x7FFFFFFFull) return fail(LSTC_ERR_INVALID, "radix sort: too many keys");
     uint64_t *in = keys, *out = alt;
     for (int shift = 0; shift < key_bits; shift += 8) {
