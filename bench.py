@@ -231,6 +231,32 @@ def partitioned(T, g, k, seed, steps, ws):
             "part_owned": [p.owned for p in parts], "part_arcs": [p.m for p in parts]}
 
 
+def support_leg(T, g, tri, st, steps=3):
+    """Per-edge support and per-vertex triangle counts of the headline graph (lstc_triangle_support):
+    time per call, and the sums every triangle fixes (it adds one to three arcs and three vertices).
+    Algorithmic bytes: every list N+(w), w in N+(u), streamed once per arc u -> w (4 B an entry:
+    sum over vertices of d-(v) d+(v)), N+(u) read once, two row offsets per arc, the support array
+    written once and the per-vertex counts once, and one 8-byte read-modify-write per triangle
+    for the support of its w -> x arc."""
+    import torch
+    T.triangle_support(g)  # first call outside the timing (output allocations)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        sup, pv = T.triangle_support(g)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    s_sup = int(sup.to(torch.int64).sum().item())
+    s_pv = int(pv.sum().item())
+    alg = 4 * st["sum_in_out"] + 4 * g.m + 16 * g.m + 4 * g.m + 8 * g.n + 8 * tri
+    return {"ms": round(ms, 3), "gteps": round(g.m / (ms / 1e3) / 1e9, 4), "steps": steps,
+            "algorithmic_bytes": alg, "achieved_gbs": round(alg / (ms / 1e3) / 1e9, 1),
+            "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "support_sum": s_sup, "per_vertex_sum": s_pv, "sums_equal_3x_triangles": s_sup == 3 * tri == s_pv}
+
+
 def _per_graph_calls(T, graphs, ws, tune=None):
     """Pre-bound C calls of the per-graph path: graph i counts into res[i] (lstc_count_split
     when it carries in-lists, else lstc_count), sharing one workspace sized for the largest."""
@@ -437,7 +463,7 @@ def main():
     ap.add_argument("--serve-verify", choices=["all", "none"], default="all")
     ap.add_argument("--load-requests", type=int, default=2000, help="requests per point of the serving load sweep")
     ap.add_argument("--load-rates", default="2000,4000,8000", help="arrival rates of the load sweep (+ a burst)")
-    ap.add_argument("--legs", default="autotune,partitioned,mixed,serving,load",
+    ap.add_argument("--legs", default="autotune,partitioned,support,mixed,serving,load",
                     help="the extra workloads to run (comma list; for profiling one of them)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run an int32 elementwise copy of this many MiB (reads N, writes N bytes), "
@@ -540,6 +566,12 @@ def main():
             raise SystemExit("partitioned sum differs from the unpartitioned count")
     g.sym_off = g.sym_col = None
     torch.cuda.empty_cache()
+    if "support" in legs:
+        extra["support"] = support_leg(T, g, tri, st)
+        log(f"triangle support: {extra['support']}")
+        if not extra["support"]["sums_equal_3x_triangles"]:
+            raise SystemExit("support / per-vertex sums differ from 3x the count")
+        torch.cuda.empty_cache()
     if "mixed" in legs:
         mg, _names = W.mixed_graphs(args.seed, ws)
         torch.cuda.synchronize()
@@ -595,7 +627,7 @@ def main():
         "cpu_baseline": cpu,
     }
     names = {"partitioned": "partitioned_s23_k8", "mixed": "batched_vs_per_graph", "serving": "serving",
-             "serving_load": "serving_load", "autotune": "autotune"}
+             "serving_load": "serving_load", "autotune": "autotune", "support": "triangle_support_s23"}
     for k, v in extra.items():
         res[names[k]] = v
     print(json.dumps(res))

@@ -8,44 +8,70 @@
 // as an entry x of N+(w) that N+(u) also holds, and adds one to the support of
 // its three arcs and to the count of its three vertices:
 //
-//   one wave per owned row u (and per tile of 1024 keys of N+(u), for longer
-//   rows): the tile is staged in LDS, sorted as the CSR keeps it, with one
-//   counter per key (the arcs u -> x);
+//   one wave per owned row u (and per tile of kSupTile keys of N+(u), for longer
+//   rows): the tile goes into an LDS hash table of 4-slot buckets (key -> its
+//   index in the tile), with one counter per key (the arcs u -> x);
 //   the lists N+(w), w in N+(u), are walked flat, 64 segments per window and 64
-//   entries per round; each lane finds its segment by a binary search of the
-//   window's prefix sums and its entry x by a binary search of the tile;
+//   entries per round, kSupUnroll rounds with their loads in flight together;
+//   a lane's segment is the number of segment starts at or before its position
+//   (a ballot of start marks in LDS), and its entry x is looked up in the table;
 //   a hit adds to the tile counter of x (LDS), to the window counter of w (LDS),
-//   and, with global atomics, to the support of arc w -> x and the count of x;
-//   window and tile counters go out with one atomic each.
+//   and, with a global atomic, to the support of arc w -> x; window and tile
+//   counters go out with one atomic each, to the supports of arcs u -> w and
+//   u -> x and to the counts of w and x (so a hub vertex receives one atomic per
+//   arc of its, not one per triangle of its).
 #include "lstc_internal.h"
 #include "wave.h"
 
 namespace lstc {
 
-static constexpr uint32_t kSupTile = 1024;  // keys of N+(u) per tile
+static constexpr uint32_t kSupTile = 512;            // keys of N+(u) per tile
+static constexpr uint32_t kSupSlots = 2 * kSupTile;  // table slots: load <= 1/2
+static constexpr uint32_t kSupBuckets = kSupSlots / 4;
 static constexpr int kSupWaves = 4;
+static constexpr int kSupUnroll = 8;             // rounds of 64 entries whose loads are in flight together
+static constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;  // ids are below 2^32 - 1
 
-// Last index i in [0, n) with a[i] <= t (a ascending, a[0] <= t).
-__device__ __forceinline__ uint32_t last_le(const uint32_t *a, uint32_t n, uint32_t t) {
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (a[mid] <= t) lo = mid; else hi = mid;
+__device__ __forceinline__ uint32_t sup_home(uint32_t x) { return (x * 0x9E3779B1u) >> 24; }  // 256 buckets
+
+struct SupWave {
+    uint4 tab[kSupBuckets];       // keys, 4 slots per bucket, filled in slot order
+    uint16_t tidx[kSupSlots];     // tile index of the key in each slot
+    uint32_t cnt[kSupTile];       // triangles found per tile key (arc u -> x)
+    uint32_t start[64];           // window: flat start of each non-empty segment
+    uint64_t base[64];            //         column index of its first entry
+    uint32_t nb[64];              //         its neighbour index in N+(u) (arc u -> w)
+    uint32_t hits[64];            //         triangles found per segment
+    uint8_t mark[64];             //         segment starts inside the current round
+};
+
+// Tile index of x in the table, or -1.  A key lies in the first bucket from its
+// home that had a free slot when it was inserted, and slots never empty, so the
+// walk ends at the key or at a bucket that is not full; at most half of the
+// slots are taken, so that happens within one lap (the bound).
+__device__ __forceinline__ int sup_find(const SupWave &S, uint32_t x) {
+    uint32_t b = sup_home(x);
+    for (uint32_t step = 0; step < kSupBuckets; ++step) {
+        const uint4 v = S.tab[b];
+        if (v.x == x) return S.tidx[4 * b];
+        if (v.y == x) return S.tidx[4 * b + 1];
+        if (v.z == x) return S.tidx[4 * b + 2];
+        if (v.w == x) return S.tidx[4 * b + 3];
+        if (v.w == kEmptySlot) return -1;
+        b = (b + 1) & (kSupBuckets - 1);
     }
-    return lo;
+    return -1;
 }
 
 __global__ __launch_bounds__(kSupWaves * 64) void k_support(const uint64_t *__restrict__ off,
                                                             const uint32_t *__restrict__ col, uint32_t n_rows,
                                                             uint32_t n_owned, uint32_t *__restrict__ support,
                                                             unsigned long long *__restrict__ per_vertex) {
-    __shared__ uint32_t s_key[kSupWaves][kSupTile];
-    __shared__ uint32_t s_cnt[kSupWaves][kSupTile];
-    __shared__ uint32_t s_start[kSupWaves][64];  // window: exclusive prefix of segment lengths
-    __shared__ uint64_t s_base[kSupWaves][64];   //          column index of each segment's first entry
-    __shared__ uint32_t s_hits[kSupWaves][64];   //          triangles found per segment
+    __shared__ SupWave s_w[kSupWaves];
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    uint32_t *key = s_key[wv], *cnt = s_cnt[wv];
+    SupWave &S = s_w[wv];
+    uint32_t *slot = (uint32_t *)S.tab;
+    const uint64_t le_mask = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
     const uint64_t nw = (uint64_t)gridDim.x * kSupWaves;
     for (uint64_t u = (uint64_t)blockIdx.x * kSupWaves + wv; u < n_owned; u += nw) {
         const uint64_t rb = off[u];
@@ -54,9 +80,24 @@ __global__ __launch_bounds__(kSupWaves * 64) void k_support(const uint64_t *__re
         unsigned long long row_tri = 0;
         for (uint32_t t0 = 0; t0 < K; t0 += kSupTile) {
             const uint32_t T = min(K - t0, kSupTile);
+            for (uint32_t i = lane; i < kSupSlots; i += 64) slot[i] = kEmptySlot;
+            for (uint32_t i = lane; i < T; i += 64) S.cnt[i] = 0;
             for (uint32_t i = lane; i < T; i += 64) {
-                key[i] = col[rb + t0 + i];
-                cnt[i] = 0;
+                const uint32_t key = col[rb + t0 + i];
+                uint32_t b = sup_home(key);
+                for (uint32_t step = 0; step < kSupBuckets; ++step) {  // a slot is tried once the ones before it are taken
+                    uint32_t *q = slot + 4 * b;
+                    int sl = -1;
+                    if (atomicCAS(q, kEmptySlot, key) == kEmptySlot) sl = 0;
+                    else if (atomicCAS(q + 1, kEmptySlot, key) == kEmptySlot) sl = 1;
+                    else if (atomicCAS(q + 2, kEmptySlot, key) == kEmptySlot) sl = 2;
+                    else if (atomicCAS(q + 3, kEmptySlot, key) == kEmptySlot) sl = 3;
+                    if (sl >= 0) {
+                        S.tidx[4 * b + sl] = (uint16_t)i;
+                        break;
+                    }
+                    b = (b + 1) & (kSupBuckets - 1);
+                }
             }
             const uint32_t lo = col[rb + t0], hi = col[rb + t0 + T - 1];
             const bool tiled = K > kSupTile;
@@ -73,43 +114,65 @@ __global__ __launch_bounds__(kSupWaves * 64) void k_support(const uint64_t *__re
                         if (tiled && len && (col[wb + len - 1] < lo || col[wb] > hi)) len = 0;
                     }
                 }
+                const uint64_t ne = ballot(len > 0);
+                if (ne == 0) continue;
                 const uint32_t incl = wave_incl_scan(len, lane);
                 const uint32_t total = read_lane(incl, 63);
-                s_start[wv][lane] = incl - len;
-                s_base[wv][lane] = wb;
-                s_hits[wv][lane] = 0;
-                const uint32_t nseg = min(64u, K - j0);
-                for (uint32_t B = 0; B < total; B += 64) {
-                    const uint32_t t = B + lane;
-                    if (t < total) {
-                        // the segment holding flat position t: the last one starting at or before t
-                        // (empty segments share their start with the next one, so they are passed over)
-                        const uint32_t sg = last_le(s_start[wv], nseg, t);
-                        const uint64_t pos = s_base[wv][sg] + (t - s_start[wv][sg]);
-                        const uint32_t x = col[pos];
-                        if (x >= lo && x <= hi) {
-                            const uint32_t i = last_le(key, T, x);
-                            if (key[i] == x) {
-                                atomicAdd(&cnt[i], 1u);            // arc u -> x
-                                atomicAdd(&s_hits[wv][sg], 1u);   // arc u -> w
-                                if (support) atomicAdd(&support[pos], 1u);  // arc w -> x
-                                if (per_vertex) atomicAdd(&per_vertex[x], 1ull);
-                            }
+                const uint32_t st = incl - len;  // this lane's segment start (when non-empty)
+                if (len > 0) {                   // non-empty segments, compacted in order
+                    const uint32_t c = mask_rank(ne);
+                    S.start[c] = st;
+                    S.base[c] = wb;
+                    S.nb[c] = j;
+                    S.hits[c] = 0;
+                }
+                S.mark[lane] = 0;
+                for (uint32_t B = 0; B < total; B += 64 * kSupUnroll) {
+                    uint64_t pos[kSupUnroll];
+                    uint32_t sg[kSupUnroll], x[kSupUnroll];
+                    bool ok[kSupUnroll];
+#pragma unroll
+                    for (int r = 0; r < kSupUnroll; ++r) {
+                        const uint32_t Br = B + 64 * r;
+                        // segment of position Br + lane: the last one starting at or before it
+                        const uint32_t j0r = (uint32_t)__popcll(ballot(len > 0 && st <= Br)) - 1u;
+                        const bool in = len > 0 && st > Br && st < Br + 64;
+                        if (in) S.mark[st - Br] = 1;
+                        const uint64_t marks = ballot(S.mark[lane] != 0);
+                        if (in) S.mark[st - Br] = 0;
+                        sg[r] = j0r + (uint32_t)__popcll(marks & le_mask);
+                        ok[r] = Br + lane < total;
+                        pos[r] = ok[r] ? S.base[sg[r]] + (Br + lane - S.start[sg[r]]) : 0u;
+                    }
+#pragma unroll
+                    for (int r = 0; r < kSupUnroll; ++r) x[r] = ok[r] ? col[pos[r]] : kEmptySlot;
+#pragma unroll
+                    for (int r = 0; r < kSupUnroll; ++r) {
+                        if (x[r] < lo || x[r] > hi) continue;  // also every position past the end
+                        const int i = sup_find(S, x[r]);
+                        if (i >= 0) {
+                            atomicAdd(&S.cnt[i], 1u);                       // arc u -> x
+                            atomicAdd(&S.hits[sg[r]], 1u);                  // arc u -> w
+                            if (support) atomicAdd(&support[pos[r]], 1u);  // arc w -> x
                         }
                     }
                 }
-                if (lane < nseg) {
-                    const uint32_t h = s_hits[wv][lane];
+                if (lane < (uint32_t)__popcll(ne)) {
+                    const uint32_t h = S.hits[lane];
                     if (h) {
-                        if (support) atomicAdd(&support[rb + j0 + lane], h);
-                        if (per_vertex) atomicAdd(&per_vertex[col[rb + j0 + lane]], (unsigned long long)h);
+                        const uint64_t a = rb + S.nb[lane];
+                        if (support) atomicAdd(&support[a], h);
+                        if (per_vertex) atomicAdd(&per_vertex[col[a]], (unsigned long long)h);
                         row_tri += h;
                     }
                 }
             }
-            if (support)
-                for (uint32_t i = lane; i < T; i += 64)
-                    if (cnt[i]) atomicAdd(&support[rb + t0 + i], cnt[i]);
+            for (uint32_t i = lane; i < T; i += 64) {
+                const uint32_t c = S.cnt[i];
+                if (!c) continue;
+                if (support) atomicAdd(&support[rb + t0 + i], c);                                // arc u -> x
+                if (per_vertex) atomicAdd(&per_vertex[col[rb + t0 + i]], (unsigned long long)c);  // vertex x
+            }
         }
         const unsigned long long tot = wave_sum64(row_tri);
         if (per_vertex && lane == 0 && tot) atomicAdd(&per_vertex[u], tot);
