@@ -36,6 +36,7 @@ typedef int32_t lstc_status;
 #define LSTC_ERR_WORKSPACE (-5)   /* workspace missing or too small */
 #define LSTC_ERR_NOT_READY (-6)   /* result not yet available (server poll) */
 #define LSTC_ERR_CANCELLED (-7)   /* request cancelled or timed out */
+#define LSTC_ERR_DEVICE (-8)      /* a device-side wait exceeded its bound; the result is not valid */
 
 typedef void *lstc_stream; /* hipStream_t */
 

@@ -21,6 +21,7 @@ LSTC_ERR_HIP = -4
 LSTC_ERR_WORKSPACE = -5
 LSTC_ERR_NOT_READY = -6
 LSTC_ERR_CANCELLED = -7
+LSTC_ERR_DEVICE = -8
 
 
 class LstcError(RuntimeError):

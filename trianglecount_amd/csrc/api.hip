@@ -47,6 +47,7 @@ extern "C" const char *lstc_status_string(lstc_status st) {
         case LSTC_ERR_WORKSPACE: return "workspace missing or too small";
         case LSTC_ERR_NOT_READY: return "result not ready";
         case LSTC_ERR_CANCELLED: return "request cancelled";
+        case LSTC_ERR_DEVICE: return "device-side wait exceeded its bound";
         default: return "unknown status";
     }
 }

@@ -70,10 +70,8 @@ __global__ void k_make_keys(const uint32_t *__restrict__ src, const uint32_t *__
 // An Op provides: size(), keys(); fetch(k) -> Aux, the gathers its predicate
 // and its output need; pred(i, k, prev, aux); emit(i, k, prev, next, kept, pos,
 // aux).  prev / next are the keys at i - 1 / i + 1 when kNeedsPrev (else 0;
-// unspecified at i = 0 / i = n - 1, where the Op must not use them).  kRuns: the count pass also adds
-// each source's kept keys into runs()[source].  kSecond: the scatter pass also
-// counts a second predicate pred2 per tile (into tile_cnt2) and per source
-// (into runs2()), for the next compaction over the same keys.
+// unspecified at i = 0 / i = n - 1, where the Op must not use them).  kRuns:
+// the count pass also adds each source's kept keys into runs()[source].
 static constexpr uint32_t kCT = 256;
 static constexpr uint32_t kCR = 16;
 static constexpr uint32_t kCG = 8;  // rounds per batch of gathers
@@ -137,17 +135,12 @@ __global__ __launch_bounds__(kCT) void k_compact_count(Op op, uint32_t *__restri
 }
 
 template <class Op>
-__global__ __launch_bounds__(kCT) void k_compact_scatter(Op op, const uint64_t *__restrict__ tile_off,
-                                                         uint32_t *__restrict__ tile_cnt2) {
+__global__ __launch_bounds__(kCT) void k_compact_scatter(Op op, const uint64_t *__restrict__ tile_off) {
     __shared__ uint64_t s_k[Op::kNeedsPrev ? kCTile + 2 : 1];  // s_k[p + 1] = key at position p; s_k[0], s_k[kCTile + 1]
     __shared__ uint32_t s_w[2][kCT / 64];    // kept keys per wave of a round (two rounds in flight)
-    __shared__ uint32_t s_w2[kCT / 64];
     const uint64_t n = op.size();
     const uint64_t tile0 = (uint64_t)blockIdx.x * kCTile;
-    if (tile0 >= n) {  // the whole block leaves together
-        if (Op::kSecond && threadIdx.x == 0) tile_cnt2[blockIdx.x] = 0;
-        return;
-    }
+    if (tile0 >= n) return;  // the whole block leaves together
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t *keys = op.keys();
     uint64_t k[kCR];
@@ -164,7 +157,6 @@ __global__ __launch_bounds__(kCT) void k_compact_scatter(Op op, const uint64_t *
         __syncthreads();
     }
     uint64_t pos = tile_off[blockIdx.x];  // output position of the round's first kept key
-    uint32_t c2 = 0;
 #pragma unroll
     for (uint32_t g = 0; g < kCR; g += kCG) {
         typename Op::Aux a[kCG];
@@ -179,11 +171,6 @@ __global__ __launch_bounds__(kCT) void k_compact_scatter(Op op, const uint64_t *
             const bool kp = valid && op.pred(i, k[g + r], prev, a[r]);
             const uint64_t km = ballot(kp);
             if (lane == 0) s_w[r & 1][wv] = (uint32_t)__popcll(km);
-            if (Op::kSecond) {
-                const bool k2 = valid && op.pred2(k[g + r], a[r]);
-                c2 += k2 ? 1u : 0u;
-                add_runs(op.runs2(), op.src(k[g + r]), k2, valid, lane);
-            }
             __syncthreads();
             uint32_t before = 0, total = 0;
 #pragma unroll
@@ -196,12 +183,6 @@ __global__ __launch_bounds__(kCT) void k_compact_scatter(Op op, const uint64_t *
             pos += total;
         }
     }
-    if (Op::kSecond) {
-        c2 = wave_sum32(c2);
-        if (lane == 0) s_w2[wv] = c2;
-        __syncthreads();
-        if (threadIdx.x == 0) tile_cnt2[blockIdx.x] = s_w2[0] + s_w2[1] + s_w2[2] + s_w2[3];
-    }
 }
 
 // Unique keys of the sorted list (sentinels dropped), with each row's bounds
@@ -212,17 +193,15 @@ struct UniqueOp {
     uint32_t b;
     uint64_t *uk, *first, *last;
     struct Aux {};
-    static constexpr bool kNeedsPrev = true, kRuns = false, kSecond = false;
+    static constexpr bool kNeedsPrev = true, kRuns = false;
     __device__ uint64_t size() const { return n; }
     __device__ const uint64_t *keys() const { return sorted; }
     __device__ Aux fetch(uint64_t) const { return Aux{}; }
     __device__ bool pred(uint64_t i, uint64_t k, uint64_t prev, Aux) const {
         return k != kSentinel && (i == 0 || prev != k);
     }
-    __device__ bool pred2(uint64_t, Aux) const { return false; }
     __device__ uint32_t src(uint64_t k) const { return (uint32_t)(k >> b); }
     __device__ uint32_t *runs() const { return nullptr; }
-    __device__ uint32_t *runs2() const { return nullptr; }
     __device__ void emit(uint64_t i, uint64_t k, uint64_t prev, uint64_t next, bool kept, uint64_t pos, Aux) const {
         if (k == kSentinel) return;
         if (kept) uk[pos] = k;
@@ -247,17 +226,15 @@ struct OrientCountOp {
     const uint32_t *deg;  // symmetric degree per old id
     uint32_t *dout;
     struct Aux { uint32_t ds, dd; };
-    static constexpr bool kNeedsPrev = false, kRuns = true, kSecond = false;
+    static constexpr bool kNeedsPrev = false, kRuns = true;
     __device__ uint64_t size() const { return *d_m2; }
     __device__ const uint64_t *keys() const { return uk; }
     __device__ Aux fetch(uint64_t k) const { return Aux{deg[(uint32_t)(k >> b)], deg[(uint32_t)(k & mask)]}; }
     __device__ bool pred(uint64_t, uint64_t k, uint64_t, Aux a) const {
         return out_arc(a.ds, a.dd, (uint32_t)(k >> b), (uint32_t)(k & mask));
     }
-    __device__ bool pred2(uint64_t, Aux) const { return false; }
     __device__ uint32_t src(uint64_t k) const { return (uint32_t)(k >> b); }
     __device__ uint32_t *runs() const { return dout; }
-    __device__ uint32_t *runs2() const { return nullptr; }
     __device__ void emit(uint64_t, uint64_t, uint64_t, uint64_t, bool, uint64_t, Aux) const {}
 };
 
@@ -276,10 +253,21 @@ __device__ __forceinline__ bool in_arc(const VRec &vs, const VRec &vd, uint32_t 
     return !out_arc(vs.deg, vd.deg, s, d) && vd.dplus <= vs.dplus;
 }
 
-// Scatter pass of the oriented arcs: relabelled target ids and, if asked, their
-// segment words (start | d+ << 32 of the target's row); counts the in-list
-// arcs of the same keys per tile and per source for the in-list pass.
-struct OrientEmitOp {
+// Scatter pass of the oriented arcs and of the in-list arcs together, over the
+// unique symmetric keys (old ids), one tile of 4096 keys per workgroup:
+//   oriented arcs: the relabelled target and, if asked, its segment word
+//     (start | d+ << 32 of the target's row), at positions from the count pass's
+//     tile offsets;
+//   in-list arcs (in row s, the source d of an oriented arc d -> s with
+//     d+(d) <= d+(s)): the source and its segment word, at positions from a
+//     chained scan across tiles (decoupled look-back), and their number per
+//     source for the in-list offsets.
+// Tiles are numbered by a ticket taken when a workgroup starts, so every earlier
+// tile belongs to a workgroup that is running or done, and that workgroup
+// publishes its count before it waits on anything: the look-back ends.  Its
+// wait is also bounded; past the bound bit 1 of *flag is set and the in-lists
+// are not valid (the call then fails).
+struct OrientInPass {
     const uint64_t *uk;
     const uint64_t *d_m2;
     uint32_t b;
@@ -287,56 +275,136 @@ struct OrientEmitOp {
     const VRec *rec;
     uint32_t *col;
     uint64_t *col_seg;
+    uint32_t *in_col;   // NULL: no in-lists
+    uint64_t *in_seg;
     uint32_t *din;
-    struct Aux { VRec vs, vd; };
-    static constexpr bool kNeedsPrev = false, kRuns = false, kSecond = true;
-    __device__ uint64_t size() const { return *d_m2; }
-    __device__ const uint64_t *keys() const { return uk; }
-    __device__ Aux fetch(uint64_t k) const { return Aux{rec[(uint32_t)(k >> b)], rec[(uint32_t)(k & mask)]}; }
-    __device__ bool pred(uint64_t, uint64_t k, uint64_t, const Aux &a) const {
-        return out_arc(a.vs.deg, a.vd.deg, (uint32_t)(k >> b), (uint32_t)(k & mask));
-    }
-    __device__ bool pred2(uint64_t k, const Aux &a) const {
-        return in_arc(a.vs, a.vd, (uint32_t)(k >> b), (uint32_t)(k & mask));
-    }
-    __device__ uint32_t src(uint64_t k) const { return (uint32_t)(k >> b); }
-    __device__ uint32_t *runs() const { return nullptr; }
-    __device__ uint32_t *runs2() const { return din; }
-    __device__ void emit(uint64_t, uint64_t, uint64_t, uint64_t, bool kept, uint64_t pos, const Aux &a) const {
-        if (!kept) return;
-        col[pos] = a.vd.nid;
-        if (col_seg) col_seg[pos] = seg_word(a.vd);
-    }
+    uint64_t *status;   // per tile: flag << 62 | count, zero = not published
+    uint32_t *ticket;
+    uint64_t *in_total;
+    uint32_t *flag;
 };
+static constexpr uint64_t kTsAgg = 1ull << 62, kTsIncl = 2ull << 62, kTsCount = (1ull << 62) - 1;
+static constexpr uint32_t kLookBackPolls = 1u << 20;
 
-// Scatter pass of the target-side in-lists: in row s, the sources d of in-list
-// arcs, ascending, and their segment words.
-struct InEmitOp {
-    const uint64_t *uk;
-    const uint64_t *d_m2;
-    uint32_t b;
-    uint64_t mask;
-    const VRec *rec;
-    uint32_t *col;
-    uint64_t *col_seg;
-    struct Aux { VRec vs, vd; };
-    static constexpr bool kNeedsPrev = false, kRuns = false, kSecond = false;
-    __device__ uint64_t size() const { return *d_m2; }
-    __device__ const uint64_t *keys() const { return uk; }
-    __device__ Aux fetch(uint64_t k) const { return Aux{rec[(uint32_t)(k >> b)], rec[(uint32_t)(k & mask)]}; }
-    __device__ bool pred(uint64_t, uint64_t k, uint64_t, const Aux &a) const {
-        return in_arc(a.vs, a.vd, (uint32_t)(k >> b), (uint32_t)(k & mask));
+__device__ __forceinline__ uint64_t status_load(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(kCT) void k_orient_in_scatter(OrientInPass op, const uint64_t *__restrict__ tile_off) {
+    __shared__ uint32_t s_w[2][kCT / 64];        // oriented arcs per wave of a round (two rounds in flight)
+    __shared__ uint32_t s_in[kCR * (kCT / 64)];  // in-list arcs per (round, wave), then their exclusive prefix
+    __shared__ uint32_t s_tile;
+    __shared__ uint64_t s_excl;
+    if (threadIdx.x == 0) s_tile = atomicAdd(op.ticket, 1u);
+    __syncthreads();
+    const uint64_t tile = s_tile;
+    const uint64_t n = *op.d_m2;
+    const uint64_t tile0 = tile * kCTile;
+    if (tile0 >= n) return;  // no keys; every later tile has none either
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const bool with_in = op.in_col != nullptr;
+    uint64_t k[kCR];
+#pragma unroll
+    for (uint32_t r = 0; r < kCR; ++r) {
+        const uint64_t i = tile0 + (uint64_t)r * kCT + threadIdx.x;
+        k[r] = i < n ? op.uk[i] : 0;
     }
-    __device__ bool pred2(uint64_t, const Aux &) const { return false; }
-    __device__ uint32_t src(uint64_t k) const { return (uint32_t)(k >> b); }
-    __device__ uint32_t *runs() const { return nullptr; }
-    __device__ uint32_t *runs2() const { return nullptr; }
-    __device__ void emit(uint64_t, uint64_t, uint64_t, uint64_t, bool kept, uint64_t pos, const Aux &a) const {
-        if (!kept) return;
-        col[pos] = a.vd.nid;
-        if (col_seg) col_seg[pos] = seg_word(a.vd);
+    uint64_t pos = tile_off[tile];         // output position of the round's first oriented arc
+    uint64_t inm[kCR];                     // in-list arcs of each round (wave mask)
+    uint32_t iv[kCR], ilo[kCR], idp[kCR];  // their source's new id, row start (low bits) and d+
+#pragma unroll
+    for (uint32_t g = 0; g < kCR; g += kCG) {
+        VRec vs[kCG], vd[kCG];
+#pragma unroll
+        for (uint32_t r = 0; r < kCG; ++r) {
+            vs[r] = op.rec[(uint32_t)(k[g + r] >> op.b)];
+            vd[r] = op.rec[(uint32_t)(k[g + r] & op.mask)];
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kCG; ++r) {
+            const uint64_t i = tile0 + (uint64_t)(g + r) * kCT + threadIdx.x;
+            const bool valid = i < n;
+            const uint32_t s = (uint32_t)(k[g + r] >> op.b), d = (uint32_t)(k[g + r] & op.mask);
+            const bool ko = valid && out_arc(vs[r].deg, vd[r].deg, s, d);
+            const bool ki = valid && with_in && in_arc(vs[r], vd[r], s, d);
+            const uint64_t km = ballot(ko);
+            if (lane == 0) s_w[r & 1][wv] = (uint32_t)__popcll(km);
+            inm[g + r] = ballot(ki);
+            if (lane == 0) s_in[(g + r) * (kCT / 64) + wv] = (uint32_t)__popcll(inm[g + r]);
+            iv[g + r] = vd[r].nid;
+            ilo[g + r] = vd[r].off_lo;
+            idp[g + r] = vd[r].dplus;
+            if (with_in) add_runs(op.din, s, ki, valid, lane);
+            __syncthreads();
+            uint32_t before = 0, total = 0;
+#pragma unroll
+            for (uint32_t x = 0; x < kCT / 64; ++x) {
+                const uint32_t v = s_w[r & 1][x];
+                before += x < wv ? v : 0u;
+                total += v;
+            }
+            if (ko) {
+                const uint64_t q = pos + before + mask_rank(km);
+                op.col[q] = vd[r].nid;
+                if (op.col_seg) op.col_seg[q] = seg_word(vd[r]);
+            }
+            pos += total;
+        }
     }
-};
+    if (!with_in) return;
+    __syncthreads();
+    if (wv == 0) {
+        // the tile's in-list arcs: exclusive prefix over (round, wave), then the
+        // tile's place among all tiles
+        const uint32_t v = s_in[lane];
+        const uint32_t incl = wave_incl_scan(v, lane);
+        s_in[lane] = incl - v;
+        const uint64_t c = read_lane(incl, 63);
+        uint64_t excl = 0;
+        if (tile == 0) {
+            if (lane == 0) status_store(op.status, kTsIncl | c);
+        } else {
+            if (lane == 0) status_store(op.status + tile, kTsAgg | c);
+            int64_t t = (int64_t)tile - 1;  // lane j examines tile t - j
+            uint32_t polls = 0;
+            while (true) {
+                const int64_t tj = t - (int64_t)lane;
+                const uint64_t w = tj >= 0 ? status_load(op.status + tj) : kTsIncl;  // before tile 0: prefix 0
+                const uint64_t mi = ballot((w >> 62) == 2), mz = ballot((w >> 62) == 0);
+                const uint64_t first = mi & (0ull - mi);           // nearest tile with its inclusive prefix
+                const uint64_t below = first ? first - 1 : ~0ull;  // the tiles between it and this one
+                if (mz & below) {                                  // one of them has not published yet
+                    if (++polls > kLookBackPolls) {
+                        if (lane == 0) atomicOr(op.flag, 2u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += wave_sum64(lane_in(below | first) ? (w & kTsCount) : 0ull);
+                if (first) break;
+                t -= 64;
+            }
+            if (lane == 0) status_store(op.status + tile, kTsIncl | (excl + c));
+        }
+        if (lane == 0) {
+            s_excl = excl;
+            if (tile0 + kCTile >= n) *op.in_total = excl + c;  // the last tile holds the total
+        }
+    }
+    __syncthreads();
+    const uint64_t base = s_excl;
+#pragma unroll
+    for (uint32_t r = 0; r < kCR; ++r) {
+        if (!lane_in(inm[r])) continue;
+        const uint64_t q = base + s_in[r * (kCT / 64) + wv] + mask_rank(inm[r]);
+        op.in_col[q] = iv[r];
+        if (op.in_seg) op.in_seg[q] = (uint64_t)ilo[r] | ((uint64_t)idp[r] << 32);
+    }
+}
 
 template <class Op>
 static lstc_status compact_pass(const Op &op, uint64_t n_max, uint32_t *tile_cnt, uint64_t *tile_off, uint64_t *d_total,
@@ -346,8 +414,7 @@ static lstc_status compact_pass(const Op &op, uint64_t n_max, uint32_t *tile_cnt
     hipLaunchKernelGGL(k_compact_count<Op>, dim3((uint32_t)tiles), dim3(kCT), 0, st, op, tile_cnt);
     LSTC_LAUNCHED("k_compact_count");
     LSTC_TRY(scan_exclusive_u32_to_u64(tile_cnt, tile_off, tiles, d_total, scan_ws, scan_bytes, st));
-    hipLaunchKernelGGL(k_compact_scatter<Op>, dim3((uint32_t)tiles), dim3(kCT), 0, st, op, (const uint64_t *)tile_off,
-                       (uint32_t *)nullptr);
+    hipLaunchKernelGGL(k_compact_scatter<Op>, dim3((uint32_t)tiles), dim3(kCT), 0, st, op, (const uint64_t *)tile_off);
     LSTC_LAUNCHED("k_compact_scatter");
     return LSTC_OK;
 }
@@ -407,6 +474,8 @@ struct PrepLayout {
     uint64_t *keys, *alt;
     uint32_t *tile_cnt;
     uint64_t *tile_off;
+    uint64_t *tstat;  // look-back words of the in-list scan, one per tile
+    uint32_t *ticket;
     uint32_t *deg, *keepv, *nid, *dout, *din;
     VRec *rec;
     uint64_t *degscan, *nidscan, *doscan, *first, *last, *discan;
@@ -428,6 +497,8 @@ static PrepLayout prep_layout(void *ws, size_t ws_bytes, uint32_t n, uint64_t m_
     L.alt = c.take<uint64_t>(nk + 1);
     L.tile_cnt = c.take<uint32_t>(tiles);
     L.tile_off = c.take<uint64_t>(tiles);
+    L.tstat = c.take<uint64_t>(tiles);
+    L.ticket = c.take<uint32_t>(4);
     L.deg = c.take<uint32_t>((size_t)n + 1);
     L.keepv = c.take<uint32_t>((size_t)n + 1);
     L.nid = c.take<uint32_t>((size_t)n + 1);
@@ -524,22 +595,19 @@ lstc_status lstc::clean_orient_launch(const uint32_t *d_src, const uint32_t *d_d
         LSTC_LAUNCHED("k_orient_offsets");
         hipLaunchKernelGGL(k_vrec, dim3(grid_n), dim3(256), 0, st, L.nid, L.dout, L.doscan, L.deg, n, L.rec);
         LSTC_LAUNCHED("k_vrec");
-        // scatter pass: columns (and segment words); counts the in-list arcs per tile and per vertex
-        hipLaunchKernelGGL(k_compact_scatter<OrientEmitOp>, dim3((uint32_t)tiles), dim3(kCT), 0, st,
-                           OrientEmitOp{uk, L.scal + 0, b, mask, L.rec, d_col, d_col_seg, L.din},
-                           (const uint64_t *)L.tile_off, L.tile_cnt);
-        LSTC_LAUNCHED("k_compact_scatter<orient>");
+        // scatter pass: columns (and segment words); in-lists (entries -> scal[3]) in the same pass
+        LSTC_TRY(fill_u64(L.tstat, 0, tiles, st));
+        LSTC_TRY(fill_u32(L.ticket, 0, 1, st));
+        hipLaunchKernelGGL(k_orient_in_scatter, dim3((uint32_t)tiles), dim3(kCT), 0, st,
+                           OrientInPass{uk, L.scal + 0, b, mask, L.rec, d_col, d_col_seg, d_in_col, d_in_seg, L.din,
+                                        L.tstat, L.ticket, L.scal + 3, L.bad},
+                           (const uint64_t *)L.tile_off);
+        LSTC_LAUNCHED("k_orient_in_scatter");
         if (d_in_off) {
-            // in-lists of the arcs counted from the target side (entries -> scal[3])
-            LSTC_TRY(scan_exclusive_u32_to_u64(L.tile_cnt, L.tile_off, tiles, L.scal + 3, L.scan_ws, L.scan_bytes, st));
             LSTC_TRY(scan_exclusive_u32_to_u64(L.din, L.discan, n, nullptr, L.scan_ws, L.scan_bytes, st));
             hipLaunchKernelGGL(k_orient_offsets, dim3(grid_n), dim3(256), 0, st, L.keepv, L.nidscan, L.discan, n,
                                L.scal + 1, L.scal + 3, d_in_off);
             LSTC_LAUNCHED("k_orient_offsets(in)");
-            hipLaunchKernelGGL(k_compact_scatter<InEmitOp>, dim3((uint32_t)tiles), dim3(kCT), 0, st,
-                               InEmitOp{uk, L.scal + 0, b, mask, L.rec, d_in_col, d_in_seg},
-                               (const uint64_t *)L.tile_off, (uint32_t *)nullptr);
-            LSTC_LAUNCHED("k_compact_scatter<in>");
         }
     } else {
         // no edges: every vertex is isolated
@@ -583,6 +651,7 @@ extern "C" lstc_status lstc_clean_orient_split(const uint32_t *d_src, const uint
     LSTC_HIP(hipMemcpyAsync(h, d_sizes, sizeof(h), hipMemcpyDeviceToHost, st));
     LSTC_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, st));
     LSTC_HIP(hipStreamSynchronize(st));
+    if (bad & 2u) return fail(LSTC_ERR_DEVICE, "lstc_clean_orient_split: in-list look-back exceeded its bound");
     if (bad) return fail(LSTC_ERR_INVALID, "lstc_clean_orient_split: vertex id >= n in input");
     *n_out = (uint32_t)h[1];
     *m_out = h[2];
@@ -608,6 +677,7 @@ extern "C" lstc_status lstc_clean_orient(const uint32_t *d_src, const uint32_t *
     LSTC_HIP(hipMemcpyAsync(h, d_sizes, sizeof(h), hipMemcpyDeviceToHost, st));
     LSTC_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, st));
     LSTC_HIP(hipStreamSynchronize(st));
+    if (bad & 2u) return fail(LSTC_ERR_DEVICE, "lstc_clean_orient: in-list look-back exceeded its bound");
     if (bad) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: vertex id >= n in input");
     *n_out = (uint32_t)h[1];
     *m_out = h[2];

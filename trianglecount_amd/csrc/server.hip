@@ -292,14 +292,14 @@ static lstc_status collect_prepared(lstc_server *S) {
         r->prep_done = nullptr;
         S->pool.release(r->transient);
         r->transient = nullptr;
-        const bool bad = S->h_flags[r->flag_slot] != 0;
+        const uint32_t bad = S->h_flags[r->flag_slot];
         S->free_flags.push_back(r->flag_slot);
         r->flag_slot = -1;
         S->preparing[i] = S->preparing.back();
         S->preparing.pop_back();
-        if (bad) {  // an id out of range: the graph is not counted
+        if (bad) {  // an id out of range (or a failed in-list scan): the graph is not counted
             S->pool.release(r->resident);
-            publish_failed(S, r, LSTC_ERR_INVALID);
+            publish_failed(S, r, (bad & 2u) ? LSTC_ERR_DEVICE : LSTC_ERR_INVALID);
             delete r;
             failed_any = true;
             continue;
