@@ -400,6 +400,11 @@ lstc_status lstc_server_submit(lstc_server *srv, const lstc_request *req, uint64
 /* LSTC_ERR_NOT_READY until the result exists; a taken result is removed. */
 lstc_status lstc_server_poll(lstc_server *srv, uint64_t id, lstc_result *out);
 lstc_status lstc_server_wait(lstc_server *srv, uint64_t id, double timeout_s, lstc_result *out);
+/* Cancels a request that has not been admitted yet: it is dropped before it
+ * takes memory and its result is LSTC_ERR_CANCELLED.  LSTC_ERR_NOT_READY for a
+ * request already admitted or finished (it keeps its normal result),
+ * LSTC_ERR_INVALID for an id this server never returned. */
+lstc_status lstc_server_cancel(lstc_server *srv, uint64_t id);
 lstc_status lstc_server_get_stats(lstc_server *srv, lstc_server_stats *out);
 /* Finishes every submitted request, then frees the server. */
 lstc_status lstc_server_destroy(lstc_server *srv);

@@ -116,3 +116,31 @@ def test_poll_not_ready_then_ready(lstc, oracle):
         assert srv.poll(rid) is None  # a taken result is gone
     finally:
         srv.close()
+
+
+def test_cancel_queued_request(lstc):
+    """A request not admitted yet can be cancelled (its result is LSTC_ERR_CANCELLED, it takes no
+    memory); admitted or finished ones keep their exact result; unknown ids are refused."""
+    import json
+    import os
+    from trianglecount_amd._lib import LSTC_ERR_CANCELLED
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "rmat_counts.json")))["18"]
+    # ~290 MB per RMAT s18 working set: the pool (7/8 of 480 MB) holds one at a time,
+    # so the second and third requests wait in the queue while the first is counted
+    srv = lstc.Server(480 << 20)
+    try:
+        ids = [srv.submit_rmat(18, 16 << 18, gold["seed"]) for _ in range(3)]
+        assert srv.cancel(ids[2]) is True
+        for rid in ids[:2]:
+            r = srv.wait(rid, 120.0)
+            assert r["count"] == gold["triangles"] and r["edges"] == gold["m"]
+        with pytest.raises(lstc.LstcError) as e:
+            srv.wait(ids[2], 120.0)
+        assert e.value.status == LSTC_ERR_CANCELLED
+        assert srv.cancel(ids[0]) is False  # finished: nothing to cancel
+        with pytest.raises(lstc.LstcError):
+            srv.cancel(ids[2] + 1000)  # never returned by this server
+        st = srv.stats()
+        assert st["completed"] == 3
+    finally:
+        srv.close()

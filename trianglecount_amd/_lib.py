@@ -146,6 +146,7 @@ SIGNATURES = {
     "lstc_server_submit": (ST, [VP, C.POINTER(Request), PU64]),
     "lstc_server_poll": (ST, [VP, U64, C.POINTER(Result)]),
     "lstc_server_wait": (ST, [VP, U64, DBL, C.POINTER(Result)]),
+    "lstc_server_cancel": (ST, [VP, U64]),
     "lstc_server_get_stats": (ST, [VP, C.POINTER(ServerStats)]),
     "lstc_server_destroy": (ST, [VP]),
 }

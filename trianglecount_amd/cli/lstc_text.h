@@ -47,17 +47,37 @@ struct Scanner {
         buf.resize(got + 1);
         ok = true;
     }
-    // next unsigned integer; skips anything that is not a digit
+    // next unsigned integer; skips anything that is not a digit; false at the end
+    // of the file or for a number that does not fit 64 bits
     bool next(uint64_t &v) {
         const size_t n = buf.size() - 1;
         while (pos < n && (buf[pos] < '0' || buf[pos] > '9')) ++pos;
         if (pos >= n) return false;
         uint64_t x = 0;
-        while (pos < n && buf[pos] >= '0' && buf[pos] <= '9') x = x * 10 + (uint64_t)(buf[pos++] - '0');
+        while (pos < n && buf[pos] >= '0' && buf[pos] <= '9') {
+            const uint64_t d = (uint64_t)(buf[pos++] - '0');
+            if (x > (UINT64_MAX - d) / 10) return false;
+            x = x * 10 + d;
+        }
         v = x;
         return true;
     }
+    // could the rest of the file hold `count` more integers (a digit and a separator each)?
+    bool room(uint64_t count) const { return count <= (buf.size() - pos) / 2 + 1; }
 };
+
+// A CSR read from a file is used to index arrays: offsets must start at 0, never
+// decrease and end at |col|, and every column id must name a row.
+inline bool check_csr(const std::vector<uint64_t> &off, const std::vector<uint32_t> &col, uint64_t rows,
+                      std::string &err) {
+    if (off.empty() || off[0] != 0) { err = "row_ptr does not start at 0"; return false; }
+    for (size_t i = 1; i < off.size(); ++i)
+        if (off[i] < off[i - 1]) { err = "row_ptr decreases"; return false; }
+    if (off.back() != col.size()) { err = "row_ptr end != |col|"; return false; }
+    for (uint32_t c : col)
+        if (c >= rows) { err = "column id beyond the rows"; return false; }
+    return true;
+}
 
 struct Csr {
     uint64_t n = 0, header_edges = 0;  // header_edges: the second header number as written
@@ -72,6 +92,7 @@ inline bool read_oriented(const char *path, Csr &g, std::string &err) {
     uint64_t V, E, rps, cis;
     if (!s.next(V) || !s.next(E) || !s.next(rps) || !s.next(cis)) { err = "bad header"; return false; }
     if (rps == 0) { err = "empty row_ptr"; return false; }
+    if (!s.room(rps) || !s.room(cis) || rps - 1 > 0xFFFFFFFEull) { err = "sizes in the header exceed the file"; return false; }
     g.n = rps - 1;
     g.header_edges = E;
     g.off.resize(rps);
@@ -80,11 +101,11 @@ inline bool read_oriented(const char *path, Csr &g, std::string &err) {
     for (uint64_t i = 0; i < cis; ++i) {
         uint64_t v;
         if (!s.next(v)) { err = "short col"; return false; }
+        if (v > 0xFFFFFFFFull) { err = "column id beyond 32 bits"; return false; }
         g.col[i] = (uint32_t)v;
     }
-    if (g.off[g.n] != cis) { err = "row_ptr end != |col|"; return false; }
     (void)V;
-    return true;
+    return check_csr(g.off, g.col, g.n, err);
 }
 
 inline bool write_oriented(const char *path, uint64_t V, uint64_t E, const uint64_t *off, uint64_t n,
@@ -111,6 +132,10 @@ inline bool read_part_input(const char *path, Csr &g, std::vector<uint64_t> &deg
         return false;
     }
     if (rps == 0) { err = "empty row_ptr"; return false; }
+    if (!s.room(rps) || !s.room(cis) || !s.room(V) || rps - 1 > 0xFFFFFFFEull) {
+        err = "sizes in the header exceed the file";
+        return false;
+    }
     g.n = rps - 1;
     g.header_edges = E;
     g.off.resize(rps);
@@ -119,12 +144,13 @@ inline bool read_part_input(const char *path, Csr &g, std::vector<uint64_t> &deg
     for (uint64_t i = 0; i < cis; ++i) {
         uint64_t v;
         if (!s.next(v)) { err = "short col"; return false; }
+        if (v > 0xFFFFFFFFull) { err = "column id beyond 32 bits"; return false; }
         g.col[i] = (uint32_t)v;
     }
     deg.resize(V);
     for (uint64_t i = 0; i < V; ++i) if (!s.next(deg[i])) { err = "short in_deg"; return false; }
     (void)nm;
-    return true;
+    return check_csr(g.off, g.col, g.n, err);
 }
 
 inline bool write_part_input(const char *path, const uint64_t *off, uint64_t n, const uint32_t *col, uint64_t m2) {
@@ -145,6 +171,7 @@ inline bool write_part_input(const char *path, const uint64_t *off, uint64_t n, 
 inline bool read_node_parts(const char *path, uint64_t n, std::vector<uint32_t> &part, std::string &err) {
     Scanner s(path);
     if (!s.ok) { err = std::string("cannot open ") + path; return false; }
+    if (!s.room(n)) { err = "short node_parts"; return false; }
     part.resize(n);
     for (uint64_t i = 0; i < n; ++i) {
         uint64_t v;
@@ -175,21 +202,30 @@ inline bool read_partitioned(const char *path, uint64_t k, uint64_t &V, uint64_t
     Scanner s(path);
     if (!s.ok) { err = std::string("cannot open ") + path; return false; }
     if (!s.next(V) || !s.next(E)) { err = "bad header"; return false; }
+    if (!s.room(k)) { err = "fewer parts in the file than asked for"; return false; }
     parts.resize(k);
     for (uint64_t p = 0; p < k; ++p) {
         uint64_t vp, rps, cis, tver;
         if (!s.next(vp) || !s.next(rps) || !s.next(cis) || !s.next(tver)) { err = "bad part header"; return false; }
+        if (rps == 0) { err = "empty part row_ptr"; return false; }
+        if (!s.room(rps) || !s.room(cis) || rps - 1 > 0xFFFFFFFEull) {
+            err = "part sizes exceed the file";
+            return false;
+        }
         PartCsr &q = parts[p];
-        q.n_local = rps ? rps - 1 : 0;
+        q.n_local = rps - 1;
         q.n_owned = tver;
+        if (tver > q.n_local) { err = "more owned rows than rows in a part"; return false; }
         q.off.resize(rps);
         q.col.resize(cis);
         for (uint64_t i = 0; i < rps; ++i) if (!s.next(q.off[i])) { err = "short part row_ptr"; return false; }
         for (uint64_t i = 0; i < cis; ++i) {
             uint64_t v;
             if (!s.next(v)) { err = "short part col"; return false; }
+            if (v > 0xFFFFFFFFull) { err = "column id beyond 32 bits"; return false; }
             q.col[i] = (uint32_t)v;
         }
+        if (!check_csr(q.off, q.col, q.n_local, err)) return false;
         (void)vp;
     }
     return true;

@@ -523,6 +523,16 @@ class Server:
         self._keep.pop(rid, None)
         return {"count": int(res.count), "edges": int(res.edges), "latency_s": float(res.latency_s)}
 
+    def cancel(self, rid: int) -> bool:
+        """Cancels a request not admitted yet (its wait then raises LstcError with status
+        LSTC_ERR_CANCELLED); False when it is already admitted or finished (it keeps its result)."""
+        from ._lib import LSTC_ERR_NOT_READY
+        st = self.L.lstc_server_cancel(self.h, rid)
+        if st == LSTC_ERR_NOT_READY:
+            return False
+        check(st, "lstc_server_cancel")
+        return True
+
     def stats(self) -> dict:
         from ._lib import ServerStats
         st = ServerStats()

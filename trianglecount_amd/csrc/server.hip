@@ -32,6 +32,7 @@
 #include <mutex>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "count_engine.h"
@@ -176,6 +177,7 @@ struct lstc_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Req *> submitted;          // guarded by mu
+    std::unordered_set<uint64_t> queued, cancelled;  // guarded by mu: not yet admitted / asked to cancel
     std::unordered_map<uint64_t, Result> results;  // guarded by mu
     std::vector<double> latencies;        // guarded by mu
     uint64_t next_id = 1, n_submitted = 0, n_rejected = 0, n_completed = 0, n_batches = 0, edges_done = 0;
@@ -213,10 +215,28 @@ static uint32_t request_rows(const lstc_request &r) {
 }
 
 // Admit the head of `pending` while memory allows; returns false on a HIP error.
+// A request asked to be cancelled is dropped here, before it takes memory.
 static lstc_status admit(lstc_server *S) {
     while (!S->pending.empty()) {
         Req *r = S->pending.front();
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            if (S->cancelled.erase(r->id)) {
+                S->queued.erase(r->id);
+                S->results[r->id] = Result{LSTC_ERR_CANCELLED, 0, 0,
+                                           std::chrono::duration<double>(Clock::now() - r->t_submit).count()};
+                S->n_completed++;
+                S->pending.pop_front();
+                delete r;
+                S->cv_done.notify_all();
+                continue;
+            }
+        }
         if (r->spec.kind == LSTC_REQ_CSR) {  // caller-resident CSR: nothing to prepare
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                S->queued.erase(r->id);
+            }
             r->off = (uint64_t *)r->spec.off;
             r->col = (uint32_t *)r->spec.col;
             r->n_rows = r->spec.n_rows;
@@ -233,6 +253,11 @@ static lstc_status admit(lstc_server *S) {
         if (!t) break;
         char *res = S->pool.alloc(rb);
         if (!res) { S->pool.release(t); break; }
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->queued.erase(r->id);  // from here on the request runs to its result
+            S->cancelled.erase(r->id);
+        }
         S->pending.pop_front();
         r->transient = t;
         r->resident = res;
@@ -418,6 +443,7 @@ static void fail_all(lstc_server *S, lstc_status st) {
     }
     for (Req *r : S->submitted) drop(r);
     S->pending.clear(); S->preparing.clear(); S->ready.clear(); S->submitted.clear();
+    S->queued.clear(); S->cancelled.clear();
     S->cv_done.notify_all();
 }
 
@@ -537,6 +563,7 @@ extern "C" lstc_status lstc_server_submit(lstc_server *S, const lstc_request *re
         q->id = S->next_id++;
         S->n_submitted++;
         S->submitted.push_back(q);
+        S->queued.insert(q->id);
         *id = q->id;
     }
     S->cv_work.notify_one();
@@ -572,6 +599,19 @@ extern "C" lstc_status lstc_server_wait(lstc_server *S, uint64_t id, double time
         if (S->cv_done.wait_until(lk, deadline) == std::cv_status::timeout && Clock::now() >= deadline)
             return fail(LSTC_ERR_NOT_READY, "lstc_server_wait: timed out");
     }
+}
+
+extern "C" lstc_status lstc_server_cancel(lstc_server *S, uint64_t id) {
+    if (!S) return fail(LSTC_ERR_INVALID, "lstc_server_cancel: NULL server");
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (id == 0 || id >= S->next_id) return fail(LSTC_ERR_INVALID, "lstc_server_cancel: unknown request %llu",
+                                                 (unsigned long long)id);
+    if (!S->queued.count(id))
+        return fail(LSTC_ERR_NOT_READY, "lstc_server_cancel: request %llu is already admitted or finished",
+                    (unsigned long long)id);
+    S->cancelled.insert(id);
+    S->cv_work.notify_one();
+    return LSTC_OK;
 }
 
 extern "C" lstc_status lstc_server_get_stats(lstc_server *S, lstc_server_stats *out) {
