@@ -1,4 +1,6 @@
-// api.hip -- status strings, per-thread error detail, device properties.
+// api.hip -- status strings, per-thread error detail, device properties,
+// profiler ranges.
+#include <dlfcn.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <mutex>
@@ -33,6 +35,37 @@ int device_cus() {
         cache[dev] = cus;
     }
     return cache[dev];
+}
+
+// Named host ranges around the library's entry points for ROCm profilers
+// (rocprofv3 --marker-trace shows them beside the kernels).  The ROCm tracing
+// library is loaded at run time, so the library does not link against it;
+// without it, or with LSTC_NO_RANGES set, a range costs one branch.
+namespace {
+struct RangeApi {
+    int (*push)(const char *) = nullptr;
+    int (*pop)() = nullptr;
+    RangeApi() {
+        if (getenv("LSTC_NO_RANGES")) return;
+        void *h = dlopen("libroctx64.so.4", RTLD_NOW | RTLD_LOCAL);
+        if (!h) h = dlopen("libroctx64.so", RTLD_NOW | RTLD_LOCAL);
+        if (!h) return;
+        push = (int (*)(const char *))dlsym(h, "roctxRangePushA");
+        pop = (int (*)())dlsym(h, "roctxRangePop");
+        if (!push || !pop) push = nullptr, pop = nullptr;
+    }
+};
+const RangeApi &range_api() {
+    static const RangeApi api;
+    return api;
+}
+}  // namespace
+
+TraceRange::TraceRange(const char *name) : on(range_api().push != nullptr) {
+    if (on) range_api().push(name);
+}
+TraceRange::~TraceRange() {
+    if (on) range_api().pop();
 }
 
 }  // namespace lstc

@@ -89,6 +89,7 @@ extern "C" lstc_status lstc_count_batch_workspace(const lstc_graph_desc *graphs,
 
 extern "C" lstc_status lstc_count_batch(const lstc_graph_desc *graphs, uint32_t G, uint64_t *d_counts,
                                         const lstc_tune *tune, void *d_ws, size_t ws_bytes, lstc_stream s) {
+    TraceRange range_("lstc_count_batch");
     hipStream_t st = (hipStream_t)s;
     std::vector<HostGraph> hosts;
     std::vector<GraphDev> devs;
@@ -153,11 +154,13 @@ static lstc_status plan_enqueue(lstc_batch_plan *p, uint64_t *d_counts, hipStrea
 }
 
 extern "C" lstc_status lstc_batch_plan_run(lstc_batch_plan *p, uint64_t *d_counts, lstc_stream s) {
+    TraceRange range_("lstc_batch_plan_run");
     if (!p || !d_counts) return fail(LSTC_ERR_INVALID, "lstc_batch_plan_run: NULL argument");
     return plan_enqueue(p, d_counts, (hipStream_t)s);
 }
 
 extern "C" lstc_status lstc_batch_plan_launch_graph(lstc_batch_plan *p, uint64_t *d_counts, lstc_stream s) {
+    TraceRange range_("lstc_batch_plan_launch_graph");
     if (!p || !d_counts) return fail(LSTC_ERR_INVALID, "lstc_batch_plan_launch_graph: NULL argument");
     if (!p->exec || p->captured_counts != d_counts) {
         // a replay of the old graph may still be queued: let it finish first

@@ -20,6 +20,7 @@ extern "C" lstc_status lstc_count_workspace(uint32_t n_rows, uint32_t n_owned, u
 extern "C" lstc_status lstc_count(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows,
                                   uint32_t n_owned, uint64_t m, uint64_t *d_count, uint64_t *d_per_row,
                                   const lstc_tune *tune, void *d_ws, size_t ws_bytes, lstc_stream s) {
+    TraceRange range_("lstc_count");
     if (!d_count) return fail(LSTC_ERR_INVALID, "lstc_count: NULL output");
     HostGraph h{d_off, d_col, d_per_row, n_rows, n_owned, m};
     LSTC_TRY(validate_graph(h, "lstc_count"));
@@ -32,6 +33,7 @@ extern "C" lstc_status lstc_count_split(const uint64_t *d_off, const uint32_t *d
                                         const uint64_t *d_in_seg, uint32_t n_rows, uint32_t n_owned, uint64_t m,
                                         uint64_t *d_count, const lstc_tune *tune, void *d_ws, size_t ws_bytes,
                                         lstc_stream s) {
+    TraceRange range_("lstc_count_split");
     if (!d_count) return fail(LSTC_ERR_INVALID, "lstc_count_split: NULL output");
     if (!d_in_off || !d_in_col) return fail(LSTC_ERR_INVALID, "lstc_count_split: NULL in-lists");
     HostGraph h{d_off, d_col, nullptr, n_rows, n_owned, m, d_in_off, d_in_col, d_col_seg, d_in_seg};

@@ -52,6 +52,7 @@ extern "C" lstc_status lstc_count_split_static(const uint64_t *d_off, const uint
                                                const uint32_t *d_in_col, const uint64_t *d_col_seg,
                                                const uint64_t *d_in_seg, uint32_t n_rows, uint32_t n_owned,
                                                uint64_t m, uint64_t *d_count, const lstc_tune *tune, lstc_stream s) {
+    TraceRange range_("lstc_count_split_static");
     if (!d_count) return fail(LSTC_ERR_INVALID, "lstc_count_split_static: NULL output");
     if (!d_in_off || !d_in_col) return fail(LSTC_ERR_INVALID, "lstc_count_split_static: NULL in-lists");
     HostGraph h{d_off, d_col, nullptr, n_rows, n_owned, m, d_in_off, d_in_col, d_col_seg, d_in_seg};

@@ -139,6 +139,7 @@ using namespace lstc;
 extern "C" lstc_status lstc_gen_rmat(uint32_t scale, uint64_t m_raw, double a, double b, double c,
                                      uint64_t seed, uint32_t *d_src, uint32_t *d_dst,
                                      lstc_stream s) {
+    TraceRange range_("lstc_gen_rmat");
     if (scale == 0 || scale > 31) return fail(LSTC_ERR_INVALID, "lstc_gen_rmat: scale %u not in 1..31", scale);
     if (!(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0))
         return fail(LSTC_ERR_INVALID, "lstc_gen_rmat: bad a/b/c");
@@ -153,6 +154,7 @@ extern "C" lstc_status lstc_gen_rmat(uint32_t scale, uint64_t m_raw, double a, d
 
 extern "C" lstc_status lstc_gen_er(uint32_t n, uint64_t m_raw, uint64_t seed, uint32_t *d_src,
                                    uint32_t *d_dst, lstc_stream s) {
+    TraceRange range_("lstc_gen_er");
     if (n == 0 || n == 0xFFFFFFFFu) return fail(LSTC_ERR_INVALID, "lstc_gen_er: bad n");
     if (m_raw == 0) return LSTC_OK;
     if (!d_src || !d_dst) return fail(LSTC_ERR_INVALID, "lstc_gen_er: NULL output");
@@ -172,6 +174,7 @@ extern "C" lstc_status lstc_gen_chung_lu(uint32_t n, double gamma, double mean_d
                                          uint64_t m_raw, uint64_t seed, uint32_t *d_src,
                                          uint32_t *d_dst, void *d_ws, size_t ws_bytes,
                                          lstc_stream s) {
+    TraceRange range_("lstc_gen_chung_lu");
     std::vector<uint64_t> cw;
     if (n == 0xFFFFFFFFu || !chung_lu_table(n, gamma, mean_deg, w_max, cw))
         return fail(LSTC_ERR_INVALID, "lstc_gen_chung_lu: bad parameters");
@@ -192,6 +195,7 @@ extern "C" lstc_status lstc_gen_chung_lu(uint32_t n, double gamma, double mean_d
 
 extern "C" lstc_status lstc_gen_grid(uint32_t rows, uint32_t cols, double p_diag, uint64_t seed,
                                      uint32_t *d_src, uint32_t *d_dst, lstc_stream s) {
+    TraceRange range_("lstc_gen_grid");
     const uint64_t nv = (uint64_t)rows * cols;
     if (nv >= 0xFFFFFFFFull) return fail(LSTC_ERR_INVALID, "lstc_gen_grid: too many vertices");
     if (nv == 0) return LSTC_OK;

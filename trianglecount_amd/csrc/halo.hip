@@ -231,6 +231,7 @@ extern "C" lstc_status lstc_halo_sizes(const uint64_t *d_sym_off, const uint32_t
                                        const uint32_t *d_part, uint32_t p, int mode, uint32_t *n_local,
                                        uint32_t *n_owned, uint64_t *m_local, void *d_ws, size_t ws_bytes,
                                        lstc_stream s) {
+    TraceRange range_("lstc_halo_sizes");
     hipStream_t st = (hipStream_t)s;
     LSTC_TRY(halo_check_args(d_sym_off, d_sym_col, n, d_part));
     if (!n_local || !n_owned || !m_local) return fail(LSTC_ERR_INVALID, "lstc_halo_sizes: NULL output");
@@ -252,6 +253,7 @@ extern "C" lstc_status lstc_halo_extract(const uint64_t *d_sym_off, const uint32
                                          const uint32_t *d_part, uint32_t p, int mode, uint64_t *d_off_out,
                                          uint32_t *d_col_out, uint32_t *d_l2g_out, uint32_t n_local_cap,
                                          uint64_t m_local_cap, void *d_ws, size_t ws_bytes, lstc_stream s) {
+    TraceRange range_("lstc_halo_extract");
     hipStream_t st = (hipStream_t)s;
     LSTC_TRY(halo_check_args(d_sym_off, d_sym_col, n, d_part));
     if (mode != 0 && mode != 1) return fail(LSTC_ERR_INVALID, "lstc_halo_extract: mode must be 0 or 1");

@@ -505,6 +505,7 @@ static lstc_status hp_extract(const uint64_t *d_sym_off, const uint32_t *d_sym_c
 extern "C" lstc_status lstc_halo_parts_extract(const uint64_t *d_sym_off, const uint32_t *d_sym_col, uint32_t n,
                                                const uint32_t *d_part, uint32_t k, int mode, const lstc_halo_out *outs,
                                                void *d_ws, size_t ws_bytes, lstc_stream s) {
+    TraceRange range_("lstc_halo_parts_extract");
     if (k == 0 || k > kMaxParts) return fail(LSTC_ERR_INVALID, "lstc_halo_parts_extract: k = %u parts, 1..%u", k, kMaxParts);
     std::vector<uint32_t> nl(k), no(k);
     std::vector<uint64_t> ml(k);
@@ -516,6 +517,7 @@ extern "C" lstc_status lstc_halo_parts_build(const uint64_t *d_sym_off, const ui
                                              const uint32_t *d_part, uint32_t k, int mode, const lstc_halo_out *outs,
                                              uint32_t *n_local, uint32_t *n_owned, uint64_t *m_local, void *d_ws,
                                              size_t ws_bytes, lstc_stream s) {
+    TraceRange range_("lstc_halo_parts_build");
     if (!n_local || !n_owned || !m_local) return fail(LSTC_ERR_INVALID, "lstc_halo_parts_build: NULL size outputs");
     return hp_extract(d_sym_off, d_sym_col, n, d_part, k, mode, outs, n_local, n_owned, m_local, d_ws, ws_bytes,
                       (hipStream_t)s, "lstc_halo_parts_build");

@@ -159,6 +159,7 @@ extern "C" lstc_status lstc_in_lists_workspace(uint32_t n_rows, uint32_t n_owned
 extern "C" lstc_status lstc_in_lists(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows, uint32_t n_owned,
                                      uint64_t m, uint64_t *d_in_off, uint32_t *d_in_col, uint64_t *n_entries,
                                      void *d_ws, size_t ws_bytes, lstc_stream s) {
+    TraceRange range_("lstc_in_lists");
     hipStream_t st = (hipStream_t)s;
     if (n_owned > n_rows) return fail(LSTC_ERR_INVALID, "lstc_in_lists: n_owned %u > n_rows %u", n_owned, n_rows);
     if (n_rows == 0xFFFFFFFFu) return fail(LSTC_ERR_INVALID, "lstc_in_lists: n_rows must be < 2^32-1");
@@ -204,6 +205,7 @@ extern "C" lstc_status lstc_in_lists(const uint64_t *d_off, const uint32_t *d_co
 extern "C" lstc_status lstc_segments(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows, uint64_t m,
                                      const uint32_t *d_in_col, uint64_t n_in, uint64_t *d_col_seg, uint64_t *d_in_seg,
                                      lstc_stream s) {
+    TraceRange range_("lstc_segments");
     hipStream_t st = (hipStream_t)s;
     if (n_rows == 0xFFFFFFFFu) return fail(LSTC_ERR_INVALID, "lstc_segments: n_rows must be < 2^32-1");
     if (m >= 0xFFFFF000ull)

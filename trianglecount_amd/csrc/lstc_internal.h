@@ -14,6 +14,15 @@ namespace lstc {
 lstc_status fail(lstc_status st, const char *fmt, ...);
 lstc_status hip_fail(hipError_t e, const char *what);
 
+// A named host range for ROCm profilers for the lifetime of the object (api.hip).
+struct TraceRange {
+    explicit TraceRange(const char *name);
+    ~TraceRange();
+    TraceRange(const TraceRange &) = delete;
+    TraceRange &operator=(const TraceRange &) = delete;
+    bool on;
+};
+
 #define LSTC_HIP(expr)                                                  \
     do {                                                                \
         hipError_t _e = (expr);                                         \

@@ -638,6 +638,7 @@ extern "C" lstc_status lstc_clean_orient_split(const uint32_t *d_src, const uint
                                                uint32_t *d_in_col, uint64_t *d_col_seg, uint64_t *d_in_seg,
                                                uint32_t *n_out, uint64_t *m_out, uint64_t *n_in_out, void *d_ws,
                                                size_t ws_bytes, lstc_stream s) {
+    TraceRange range_("lstc_clean_orient_split");
     hipStream_t st = (hipStream_t)s;
     if (!n_out || !m_out || !n_in_out) return fail(LSTC_ERR_INVALID, "lstc_clean_orient_split: NULL size outputs");
     if (!d_in_off || !d_in_col || !d_col_seg || !d_in_seg)
@@ -664,6 +665,7 @@ extern "C" lstc_status lstc_clean_orient(const uint32_t *d_src, const uint32_t *
                                          uint32_t *d_sym_col, uint64_t *d_off, uint32_t *d_col,
                                          uint64_t *d_in_off, uint32_t *d_in_col, uint32_t *n_out,
                                          uint64_t *m_out, void *d_ws, size_t ws_bytes, lstc_stream s) {
+    TraceRange range_("lstc_clean_orient");
     hipStream_t st = (hipStream_t)s;
     if (!n_out || !m_out) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: NULL size outputs");
     const uint64_t *d_sizes = nullptr;

@@ -186,6 +186,7 @@ using namespace lstc;
 extern "C" lstc_status lstc_triangle_support(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows,
                                              uint32_t n_owned, uint64_t m, uint32_t *d_support,
                                              uint64_t *d_per_vertex, lstc_stream s) {
+    TraceRange range_("lstc_triangle_support");
     hipStream_t st = (hipStream_t)s;
     if (n_owned > n_rows) return fail(LSTC_ERR_INVALID, "lstc_triangle_support: n_owned %u > n_rows %u", n_owned, n_rows);
     if (n_rows == 0xFFFFFFFFu) return fail(LSTC_ERR_INVALID, "lstc_triangle_support: n_rows must be < 2^32-1");

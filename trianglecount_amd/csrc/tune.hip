@@ -70,6 +70,7 @@ extern "C" lstc_status lstc_tune_resolve(const lstc_graph_desc *graphs, uint32_t
 
 extern "C" lstc_status lstc_autotune(const lstc_graph_desc *graphs, uint32_t G, uint32_t reps, lstc_tune *best,
                                      double *best_ms, lstc_stream s) {
+    TraceRange range_("lstc_autotune");
     hipStream_t st = (hipStream_t)s;
     if (!graphs || G == 0 || !best) return fail(LSTC_ERR_INVALID, "lstc_autotune: bad arguments");
     if (reps == 0) reps = 3;

@@ -128,6 +128,7 @@ extern "C" lstc_status lstc_work_stats_workspace(uint32_t G, size_t *bytes) {
 
 extern "C" lstc_status lstc_work_stats(const lstc_graph_desc *graphs, uint32_t G, const lstc_tune *tune,
                                        lstc_work *out, void *d_ws, size_t ws_bytes, lstc_stream s) {
+    TraceRange range_("lstc_work_stats");
     hipStream_t st = (hipStream_t)s;
     if (G == 0) return LSTC_OK;
     if (!graphs || !out) return fail(LSTC_ERR_INVALID, "lstc_work_stats: NULL argument");
