@@ -47,8 +47,8 @@ struct RangeApi {
     int (*pop)() = nullptr;
     RangeApi() {
         if (getenv("LSTC_NO_RANGES")) return;
-        void *h = dlopen("libroctx64.so.4", RTLD_NOW | RTLD_LOCAL);
-        if (!h) h = dlopen("libroctx64.so", RTLD_NOW | RTLD_LOCAL);
+        void *h = dlopen("librocprofiler-sdk-roctx.so.1", RTLD_NOW | RTLD_LOCAL);  // what rocprofv3 traces
+        if (!h) h = dlopen("libroctx64.so.4", RTLD_NOW | RTLD_LOCAL);             // the older tracer's library
         if (!h) return;
         push = (int (*)(const char *))dlsym(h, "roctxRangePushA");
         pop = (int (*)())dlsym(h, "roctxRangePop");
