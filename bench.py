@@ -231,30 +231,40 @@ def partitioned(T, g, k, seed, steps, ws):
             "part_owned": [p.owned for p in parts], "part_arcs": [p.m for p in parts]}
 
 
-def support_leg(T, g, tri, st, steps=3):
-    """Per-edge support and per-vertex triangle counts of the headline graph (lstc_triangle_support):
+def support_leg(T, g, tri, st, work, steps=3):
+    """Per-edge support and per-vertex triangle counts of the headline graph: the split rule
+    (lstc_triangle_support_split, the graph's in-lists) and the plain walk (lstc_triangle_support);
     time per call, and the sums every triangle fixes (it adds one to three arcs and three vertices).
-    Algorithmic bytes: every list N+(w), w in N+(u), streamed once per arc u -> w (4 B an entry:
-    sum over vertices of d-(v) d+(v)), N+(u) read once, two row offsets per arc, the support array
-    written once and the per-vertex counts once, and one 8-byte read-modify-write per triangle
-    for the support of its w -> x arc."""
+    Algorithmic bytes: the streamed list entries (4 B each: the count's streamed entries for the
+    split rule, sum over vertices of d-(v) d+(v) for the plain walk), the tables' keys, two row
+    offsets per located list, the outputs written once, and one 8-byte read-modify-write per
+    triangle for the support of its streamed arc."""
     import torch
-    T.triangle_support(g)  # first call outside the timing (output allocations)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(steps):
-        sup, pv = T.triangle_support(g)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / steps
-    s_sup = int(sup.to(torch.int64).sum().item())
-    s_pv = int(pv.sum().item())
-    alg = 4 * st["sum_in_out"] + 4 * g.m + 16 * g.m + 4 * g.m + 8 * g.n + 8 * tri
-    return {"ms": round(ms, 3), "gteps": round(g.m / (ms / 1e3) / 1e9, 4), "steps": steps,
-            "algorithmic_bytes": alg, "achieved_gbs": round(alg / (ms / 1e3) / 1e9, 1),
-            "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-            "support_sum": s_sup, "per_vertex_sum": s_pv, "sums_equal_3x_triangles": s_sup == 3 * tri == s_pv}
+    out = {}
+    for name, split in (("split", True), ("plain", False)):
+        T.triangle_support(g, split=split)  # first call outside the timing (output allocations)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            sup, pv = T.triangle_support(g, split=split)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        s_sup = int(sup.to(torch.int64).sum().item())
+        s_pv = int(pv.sum().item())
+        if split:
+            alg = 4 * (work["streamed"] + work["hashed_keys"]) + 16 * (work["segments"] + work["tiny_segments"])
+        else:
+            alg = 4 * st["sum_in_out"] + 4 * g.m + 16 * g.m
+        alg += 4 * g.m + 8 * g.n + 8 * tri
+        out[name] = {"ms": round(ms, 3), "gteps": round(g.m / (ms / 1e3) / 1e9, 4), "steps": steps,
+                     "algorithmic_bytes": alg, "achieved_gbs": round(alg / (ms / 1e3) / 1e9, 1),
+                     "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "support_sum": s_sup, "per_vertex_sum": s_pv,
+                     "sums_equal_3x_triangles": s_sup == 3 * tri == s_pv}
+        del sup, pv
+    return out
 
 
 def _per_graph_calls(T, graphs, ws, tune=None):
@@ -567,9 +577,9 @@ def main():
     g.sym_off = g.sym_col = None
     torch.cuda.empty_cache()
     if "support" in legs:
-        extra["support"] = support_leg(T, g, tri, st)
+        extra["support"] = support_leg(T, g, tri, st, work)
         log(f"triangle support: {extra['support']}")
-        if not extra["support"]["sums_equal_3x_triangles"]:
+        if not all(v["sums_equal_3x_triangles"] for v in extra["support"].values()):
             raise SystemExit("support / per-vertex sums differ from 3x the count")
         torch.cuda.empty_cache()
     if "mixed" in legs:

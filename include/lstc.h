@@ -239,6 +239,16 @@ lstc_status lstc_count_split_static(const uint64_t *d_off, const uint32_t *d_col
 lstc_status lstc_triangle_support(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows, uint32_t n_owned,
                                   uint64_t m, uint32_t *d_support, uint64_t *d_per_vertex, lstc_stream s);
 
+/* The same outputs from the split rule of lstc_count_split: with the graph's
+ * target-side in-lists (lstc_in_lists, or the preprocessing's), every arc
+ * streams its shorter list.  Identical results; the workspace holds the work
+ * items (lstc_triangle_support_split_workspace).                             */
+lstc_status lstc_triangle_support_split_workspace(uint32_t n_rows, uint64_t m, size_t *bytes);
+lstc_status lstc_triangle_support_split(const uint64_t *d_off, const uint32_t *d_col, const uint64_t *d_in_off,
+                                        const uint32_t *d_in_col, uint32_t n_rows, uint32_t n_owned, uint64_t m,
+                                        uint32_t *d_support, uint64_t *d_per_vertex, void *d_ws, size_t ws_bytes,
+                                        lstc_stream s);
+
 /* Target-side in-lists of an oriented CSR for split counting: for every row w,
  * the owned sources u < n_owned of arcs u -> w with d+(u) <= d+(w) (row lengths
  * of this CSR), ascending.  d_in_off[n_rows+1], d_in_col[m] (at most m entries);

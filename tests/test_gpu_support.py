@@ -22,6 +22,15 @@ def _check(lstc, oracle, og, n_owned=None):
     assert none is None and np.array_equal(s2.cpu().numpy().view(np.uint32), want_sup)
     none, p2 = lstc.triangle_support(g, per_edge=False)
     assert none is None and np.array_equal(p2.cpu().numpy().view(np.uint64), want_pv)
+    # split rule (lstc_triangle_support_split): owned-source in-lists, every arc streams its shorter list
+    lstc.in_lists(g)
+    assert g.split_ready()
+    for pe, pvv in [(True, True), (True, False), (False, True)]:
+        s3, p3 = lstc.triangle_support(g, per_edge=pe, per_vertex=pvv, split=True)
+        if pe:
+            assert np.array_equal(s3.cpu().numpy().view(np.uint32), want_sup)
+        if pvv:
+            assert np.array_equal(p3.cpu().numpy().view(np.uint64), want_pv)
 
 
 @pytest.mark.parametrize("name,builder", [
@@ -33,6 +42,8 @@ def _check(lstc, oracle, og, n_owned=None):
     ("er", lambda O: O.gen_er(20000, 16, seed=4)),
     ("grid", lambda O: O.gen_grid(100, 80, 0.3, 2)),
     ("chung_lu", lambda O: O.gen_chung_lu(30000, 2.1, 30, 4000, seed=5)),
+    ("chung_lu_hubs", lambda O: O.gen_chung_lu(60000, 2.1, 40, 2e4, seed=7)),  # in-lists over 512 sources
+    ("rmat_s15", lambda O: O.gen_rmat(15, 16, seed=8)),
     ("star", lambda O: G.star(500)),
 ])
 def test_support_matches_oracle(lstc, oracle, name, builder):

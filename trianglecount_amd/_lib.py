@@ -147,6 +147,8 @@ SIGNATURES = {
     "lstc_server_poll": (ST, [VP, U64, C.POINTER(Result)]),
     "lstc_server_wait": (ST, [VP, U64, DBL, C.POINTER(Result)]),
     "lstc_server_cancel": (ST, [VP, U64]),
+    "lstc_triangle_support_split_workspace": (ST, [U32, U64, C.POINTER(C.c_size_t)]),
+    "lstc_triangle_support_split": (ST, [VP, VP, VP, VP, U32, U32, U64, VP, VP, VP, C.c_size_t, VP]),
     "lstc_server_get_stats": (ST, [VP, C.POINTER(ServerStats)]),
     "lstc_server_destroy": (ST, [VP]),
 }

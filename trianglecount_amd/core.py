@@ -297,15 +297,30 @@ def count(g: Graph, n_owned: int | None = None, per_row: bool = False, tune: Tun
 
 
 def triangle_support(g: Graph, n_owned: int | None = None, per_edge: bool = True, per_vertex: bool = True,
-                     stream=None):
+                     split: bool | None = None, ws: Workspace | None = None, stream=None):
     """Per-edge support (uint32 per arc of g.col) and per-vertex triangle counts (uint64 per row),
-    over the triangles whose source row is owned (lstc_triangle_support).  Returns device tensors
-    (support, per_vertex), either None when not asked for."""
+    over the triangles whose source row is owned.  With the graph's in-lists (split, the default
+    when it has them) lstc_triangle_support_split, else lstc_triangle_support.  Returns device
+    tensors (support, per_vertex), either None when not asked for."""
     no = g.owned if n_owned is None else int(n_owned)
+    if split is None:
+        split = g.split_ready(no)
+    if split and not g.split_ready(no):
+        raise ValueError("split support needs in-lists of the same owned rows: clean_orient(..., in_lists=True) "
+                         "or in_lists(g)")
     sup = empty_u32(g.m) if per_edge else None
     pv = empty_u64(g.n) if per_vertex else None
-    check(load().lstc_triangle_support(_ptr(g.off), _ptr(g.col), g.n, no, g.m, _ptr(sup), _ptr(pv),
-                                       C.c_void_p(stream_handle(stream))), "lstc_triangle_support")
+    L = load()
+    sh = C.c_void_p(stream_handle(stream))
+    if split:
+        nb = C.c_size_t()
+        check(L.lstc_triangle_support_split_workspace(g.n, g.m, C.byref(nb)), "lstc_triangle_support_split_workspace")
+        wp, wb = (ws or _default_ws).get(nb.value)
+        check(L.lstc_triangle_support_split(_ptr(g.off), _ptr(g.col), _ptr(g.in_off), _ptr(g.in_col), g.n, no, g.m,
+                                            _ptr(sup), _ptr(pv), wp, wb, sh), "lstc_triangle_support_split")
+    else:
+        check(L.lstc_triangle_support(_ptr(g.off), _ptr(g.col), g.n, no, g.m, _ptr(sup), _ptr(pv), sh),
+              "lstc_triangle_support")
     return (sup[: g.m] if sup is not None else None), (pv[: g.n] if pv is not None else None)
 
 
