@@ -121,3 +121,48 @@ def test_in_lists_bit_exact(lstc, oracle, name, builder):
     # split count (each arc streams its shorter list) equals the oracle
     assert lstc.count(g) == oracle.count(og)
     assert lstc.count(g, split=False) == oracle.count(og)
+
+
+@pytest.mark.parametrize("name,n,s,d", [
+    ("no_edges", 7, [], []),
+    ("loops_only", 5, [1, 2, 3], [1, 2, 3]),                      # every pair dropped: no symmetric arcs
+    ("one_edge", 9, [3], [7]),
+    ("duplicates", 9, [8, 3, 8, 3, 3], [3, 8, 3, 8, 8]),
+    ("triangle_and_isolated", 12, [0, 1, 2, 2], [1, 2, 0, 2]),
+])
+def test_split_preprocessing_degenerate(lstc, oracle, name, n, s, d):
+    """lstc_clean_orient_split (in-lists and segment words through the chained scan) on inputs
+    with no, one or only duplicate / looping edges: the same CSR and in-lists as the oracle's."""
+    s = np.array(s, np.uint32)
+    d = np.array(d, np.uint32)
+    ds = torch.from_numpy(s.view(np.int32)).cuda() if len(s) else torch.zeros(0, dtype=torch.int32).cuda()
+    dd = torch.from_numpy(d.view(np.int32)).cuda() if len(d) else torch.zeros(0, dtype=torch.int32).cuda()
+    g = lstc.clean_orient(n, ds, dd, in_lists=True)
+    _, og = oracle.clean_orient(n, s, d)
+    assert g.n == og.n and g.m == og.m
+    off, col = g.to_numpy()
+    assert np.array_equal(off, og.off) and np.array_equal(col, og.col)
+    want_off, want_col = G.owned_in_lists(og.off, og.col, og.n)
+    assert g.meta["in_entries"] == int(want_off[-1])
+    assert np.array_equal(g.in_off[: g.n + 1].cpu().numpy().view(np.uint64), want_off)
+    if len(want_col):
+        assert np.array_equal(g.in_col[: len(want_col)].cpu().numpy().view(np.uint32), want_col)
+    assert lstc.count(g) == oracle.count(og)
+
+
+def test_split_preprocessing_many_tiles(lstc, oracle):
+    """A graph whose unique keys span hundreds of 4096-key tiles, so the in-list positions come
+    through long look-back chains: in-lists and segment words equal the oracle's."""
+    n, s, d = oracle.gen_rmat(17, 16, seed=12)
+    ds = torch.from_numpy(np.ascontiguousarray(s, np.uint32).view(np.int32)).cuda()
+    dd = torch.from_numpy(np.ascontiguousarray(d, np.uint32).view(np.int32)).cuda()
+    g = lstc.clean_orient(n, ds, dd, in_lists=True)
+    _, og = oracle.clean_orient(n, s, d)
+    want_off, want_col = G.owned_in_lists(og.off, og.col, og.n)
+    assert np.array_equal(g.in_off[: g.n + 1].cpu().numpy().view(np.uint64), want_off)
+    assert np.array_equal(g.in_col[: len(want_col)].cpu().numpy().view(np.uint32), want_col)
+    deg = np.diff(og.off.astype(np.int64)).astype(np.uint64)
+    want_cs = (og.off[:-1] & np.uint64(0xFFFFFFFF)) | (deg << np.uint64(32))
+    assert np.array_equal(g.col_seg[: g.m].cpu().numpy().view(np.uint64), want_cs[og.col])
+    assert np.array_equal(g.in_seg[: len(want_col)].cpu().numpy().view(np.uint64), want_cs[want_col])
+    assert lstc.count(g) == oracle.count(og)
