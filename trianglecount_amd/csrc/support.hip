@@ -4,178 +4,338 @@
 // lowest-ranked vertex is u) before it reduces them
 // (src/WithoutPartition/ShareKernelTCV62_update_v1.cu:175-186); local
 // clustering and k-truss need the counts per vertex and per edge instead.
-// Every triangle u -> w -> x (u -> x, u owned) is found once, at the arc u -> w,
-// as an entry x of N+(w) that N+(u) also holds, and adds one to the support of
-// its three arcs and to the count of its three vertices:
+// Every triangle u -> w -> x (u -> x, u owned) is found once, as an entry x of
+// a streamed list that the hashed list also holds, and adds one to the support
+// of its three arcs and to the count of its three vertices.
 //
-//   one wave per owned row u (and per tile of kSupTile keys of N+(u), for longer
-//   rows): the tile goes into an LDS hash table of 4-slot buckets (key -> its
-//   index in the tile), with one counter per key (the arcs u -> x);
-//   the lists N+(w), w in N+(u), are walked flat, 64 segments per window and 64
-//   entries per round, kSupUnroll rounds with their loads in flight together;
-//   a lane's segment is the number of segment starts at or before its position
-//   (a ballot of start marks in LDS), and its entry x is looked up in the table;
-//   a hit adds to the tile counter of x (LDS), to the window counter of w (LDS),
-//   and, with a global atomic, to the support of arc w -> x; window and tile
-//   counters go out with one atomic each, to the supports of arcs u -> w and
-//   u -> x and to the counts of w and x (so a hub vertex receives one atomic per
-//   arc of its, not one per triangle of its).
+// A work item hashes one row r (in tiles of kSupTile keys) into a wave-private
+// LDS table of 4-slot buckets and streams candidate lists through it with the
+// count engine's flat segment walk (segwalk.h): 64 lists per window, rounds of
+// kSupUnroll chunks of 64 entries, a round inside one list addressed from a
+// scalar base, the next round's loads issued before this one is probed.  A hit
+// (entry x of the list of candidate c) is attributed three ways:
+//   arc r -> x : the hit slot's counter (one LDS add; the slot word also holds
+//                the key's index in the tile, so the counters go out to the
+//                supports with one atomic per slot and tile);
+//   arc of c   : hits per list come from a running count of the window's hits
+//                read at each list's end (arc r -> w for a neighbour w of r;
+//                arc u -> r for an in-list source u, located by one binary
+//                search per list with hits), one atomic per list;
+//   the entry  : one global atomic on the support of the streamed arc.
+// Vertex counts take the same three totals (x per slot, c per list, r per item).
 #include "lstc_internal.h"
+#include "segwalk.h"
 #include "wave.h"
 
 namespace lstc {
 
-static constexpr uint32_t kSupTile = 512;            // keys of N+(u) per tile
-static constexpr uint32_t kSupSlots = 2 * kSupTile;  // table slots: load <= 1/2
-static constexpr uint32_t kSupBuckets = kSupSlots / 4;
+static constexpr uint32_t kSupTile = 512;              // keys of N+(r) per tile
+static constexpr uint32_t kSupLog2 = 10;               // slots of a full tile's table: load <= 1/2
+static constexpr uint32_t kSupSlots = 1u << kSupLog2;
 static constexpr int kSupWaves = 4;
-static constexpr int kSupUnroll = 8;             // rounds of 64 entries whose loads are in flight together
-static constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;  // ids are below 2^32 - 1
-
-__device__ __forceinline__ uint32_t sup_home(uint32_t x) { return (x * 0x9E3779B1u) >> 24; }  // 256 buckets
+static constexpr int kSupUnroll = 4;                   // chunks of 64 entries per round
+static constexpr uint32_t kSupEmpty = 0xFFFFFFFFu;     // empty slot; also the end of a lane without a list
+static constexpr uint32_t kSupNoId = 0xFFFFFFFEu;      // past-the-end positions: never a key (ids < 2^32 - 2)
+static constexpr uint32_t kIdxShift = 23;              // slot word: tile index << 23 | triangles found
+static constexpr uint32_t kHitMask = (1u << kIdxShift) - 1u;
+static constexpr uint32_t kFlushCands = 1u << 22;      // candidates between slot flushes (a slot gains <= 1 per list)
 
 struct SupWave {
-    uint4 tab[kSupBuckets];       // keys, 4 slots per bucket, filled in slot order
-    uint16_t tidx[kSupSlots];     // tile index of the key in each slot
-    uint32_t cnt[kSupTile];       // triangles found per tile key (arc u -> x)
-    uint32_t start[64];           // window: flat start of each non-empty segment
-    uint64_t base[64];            //         column index of its first entry
-    uint32_t nb[64];              //         its neighbour index in N+(u) (arc u -> w)
-    uint32_t hits[64];            //         triangles found per segment
-    uint8_t mark[64];             //         segment starts inside the current round
+    uint32_t tab[kSupSlots];        // keys of the tile, 4-slot buckets
+    uint32_t cnt[kSupSlots];        // slot words (arc r -> key)
+    uint32_t base[64];              // window: list j's column start minus its flat start
+    uint32_t end[64];               //         its flat end (exclusive)
+    uint32_t cand[64];              //         its candidate index
+    uint32_t ovf[kSupSlots / 128];  // table build: one bit per home bucket that overflowed
 };
 
-// Tile index of x in the table, or -1.  A key lies in the first bucket from its
-// home that had a free slot when it was inserted, and slots never empty, so the
-// walk ends at the key or at a bucket that is not full; at most half of the
-// slots are taken, so that happens within one lap (the bound).
-__device__ __forceinline__ int sup_find(const SupWave &S, uint32_t x) {
-    uint32_t b = sup_home(x);
-    for (uint32_t step = 0; step < kSupBuckets; ++step) {
-        const uint4 v = S.tab[b];
-        if (v.x == x) return S.tidx[4 * b];
-        if (v.y == x) return S.tidx[4 * b + 1];
-        if (v.z == x) return S.tidx[4 * b + 2];
-        if (v.w == x) return S.tidx[4 * b + 3];
-        if (v.w == kEmptySlot) return -1;
-        b = (b + 1) & (kSupBuckets - 1);
-    }
-    return -1;
+// log2 of the table slots for T keys: at least twice as many slots, 16 at least.
+__device__ __forceinline__ uint32_t sup_tlog2(uint32_t T) {
+    uint32_t l = 4;
+    while ((1u << l) < 2 * T) ++l;
+    return l;
 }
 
-__global__ __launch_bounds__(kSupWaves * 64) void k_support(const uint64_t *__restrict__ off,
-                                                            const uint32_t *__restrict__ col, uint32_t n_rows,
-                                                            uint32_t n_owned, uint32_t *__restrict__ support,
-                                                            unsigned long long *__restrict__ per_vertex) {
+// Builds the table of the tile col[kb, kb + T): a key takes the first free
+// slot of its home bucket or, when that is full, of the buckets after it (a
+// slot is tried once the ones before it are taken; at most half the slots are
+// filled, so one lap finds a free one), and its slot word gets its tile index.
+// A home that overflowed is then marked by the order of its first two slots
+// (such a bucket is full: slot 0 > slot 1; any other: slot 0 < slot 1, or slot
+// 1 empty), so one read of the home tells a probe whether to go on.
+__device__ __forceinline__ void sup_build(SupWave &S, const uint32_t *__restrict__ col, uint64_t kb, uint32_t T,
+                                          uint32_t tlog2, uint32_t lane) {
+    const uint32_t ns = 1u << tlog2, qmask = (ns >> 2) - 1u, qshift = 34 - tlog2;
+    for (uint32_t i = lane; i < ns; i += 64) {
+        S.tab[i] = kSupEmpty;
+        S.cnt[i] = 0;
+    }
+    if (lane < kSupSlots / 128) S.ovf[lane] = 0;
+    for (uint32_t i = lane; i < T; i += 64) {
+        const uint32_t key = gload(col + kb + i);
+        const uint32_t home = hquad(key, qshift);
+        uint32_t h = home;
+        for (uint32_t step = 0; step <= qmask; ++step) {
+            uint32_t *q = S.tab + 4 * h;
+            int sl = -1;
+            if (atomicCAS(q, kSupEmpty, key) == kSupEmpty) sl = 0;
+            else if (atomicCAS(q + 1, kSupEmpty, key) == kSupEmpty) sl = 1;
+            else if (atomicCAS(q + 2, kSupEmpty, key) == kSupEmpty) sl = 2;
+            else if (atomicCAS(q + 3, kSupEmpty, key) == kSupEmpty) sl = 3;
+            if (sl >= 0) {
+                S.cnt[4 * h + sl] = i << kIdxShift;
+                break;
+            }
+            h = (h + 1) & qmask;
+        }
+        if (h != home) atomicOr(&S.ovf[home >> 5], 1u << (home & 31u));
+    }
+    for (uint32_t b = lane; b <= qmask; b += 64) {
+        const uint32_t k0 = S.tab[4 * b], k1 = S.tab[4 * b + 1];
+        const bool over = (S.ovf[b >> 5] >> (b & 31u)) & 1u;
+        if (over ? k0 < k1 : k0 > k1) {
+            const uint32_t w0 = S.cnt[4 * b], w1 = S.cnt[4 * b + 1];
+            S.tab[4 * b] = k1;
+            S.tab[4 * b + 1] = k0;
+            S.cnt[4 * b] = w1;
+            S.cnt[4 * b + 1] = w0;
+        }
+    }
+}
+
+// Sends the slot counters out (arcs tile_arc + tile index, vertices = the keys)
+// and clears them; the tile indexes stay.
+__device__ __forceinline__ void sup_flush(SupWave &S, uint64_t tile_arc, uint32_t tlog2, uint32_t *__restrict__ support,
+                                          unsigned long long *__restrict__ per_vertex, uint32_t lane) {
+    for (uint32_t i = lane; i < (1u << tlog2); i += 64) {
+        const uint32_t w = S.cnt[i];
+        const uint32_t h = w & kHitMask;
+        if (h == 0) continue;
+        if (support) atomicAdd(support + tile_arc + (w >> kIdxShift), h);
+        if (per_vertex) atomicAdd(per_vertex + S.tab[i], (unsigned long long)h);
+        S.cnt[i] = w & ~kHitMask;
+    }
+}
+
+// Adds one to the support of the streamed arc at column position i (a byte
+// offset with NARROW).
+template <bool NARROW>
+__device__ __forceinline__ void sup_entry_add(uint32_t *__restrict__ support, uint32_t i) {
+    LSTC_GLOBAL_AS uint32_t *p = NARROW ? (LSTC_GLOBAL_AS uint32_t *)((LSTC_GLOBAL_AS char *)support + i)
+                                        : (LSTC_GLOBAL_AS uint32_t *)support + i;
+    __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Which slot of bucket v holds x, from the compare ballots of slots 1-3
+// (slot 0 when none: only read for lanes that hit).
+__device__ __forceinline__ uint32_t sup_slot_in(uint64_t h1, uint64_t h2, uint64_t h3) {
+    return lane_in(h1) ? 1u : lane_in(h2) ? 2u : lane_in(h3) ? 3u : 0u;
+}
+
+// Probes the chunks of one round (positions B + 64k + lane, entries x[k] at
+// column positions idx[k]) and attributes the hits: slot counters and the
+// streamed arcs at once; for the lists, `cum` counts the window's hits so far
+// and a list that ends in this round (ends: some list does) records in its
+// lane the count at its end.  FULL: the round lies wholly before `total`.
+template <int U, bool FULL, bool NARROW>
+__device__ __forceinline__ void sup_probe(SupWave &S, const uint32_t (&x)[U], const uint32_t (&idx)[U], uint32_t B,
+                                          uint32_t total, bool ends, uint32_t myend, uint32_t qmask, uint32_t qshift,
+                                          uint32_t *__restrict__ support, uint32_t lane, uint32_t &cum,
+                                          uint32_t &cend) {
+    const uint4 *bk = (const uint4 *)S.tab;
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        const uint32_t lo = B + 64u * k;
+        const bool valid = FULL || lo + lane < total;
+        const uint32_t xv = valid ? x[k] : kSupNoId;
+        const uint32_t p = hquad(xv, qshift);
+        const uint4 v = bk[p];
+        const uint64_t h1 = ballot(v.y == xv), h2 = ballot(v.z == xv), h3 = ballot(v.w == xv);
+        uint64_t H = ballot(v.x == xv) | h1 | h2 | h3;
+        uint32_t slot = 4 * p + sup_slot_in(h1, h2, h3);
+        // no hit and the home overflowed: the chain goes on (bounded by one lap)
+        uint64_t going = ~(H | ballot(v.x <= v.y)) & ballot(valid);
+        uint32_t q = p;
+        for (uint32_t step = 0; going != 0 && step < qmask; ++step) {
+            q = (q + 1) & qmask;
+            const uint4 w = bk[q];
+            const uint64_t g1 = ballot(w.y == xv) & going, g2 = ballot(w.z == xv) & going,
+                           g3 = ballot(w.w == xv) & going;
+            const uint64_t G = (ballot(w.x == xv) & going) | g1 | g2 | g3;
+            if (lane_in(G)) slot = 4 * q + sup_slot_in(g1, g2, g3);
+            H |= G;
+            going &= ~(G | ballot(w.w == kSupEmpty));
+        }
+        if (H != 0 && lane_in(H)) {
+            atomicAdd(&S.cnt[slot], 1u);
+            if (support) sup_entry_add<NARROW>(support, idx[k]);
+        }
+        if (ends) {  // lists ending at lo + d, 1 <= d <= 64: the hits before their end
+            const uint32_t d = myend - lo;
+            if (d - 1u < 64u) cend = cum + (uint32_t)__popcll(H << (64u - d));
+        }
+        cum += (uint32_t)__popcll(H);
+    }
+}
+
+// Column positions of round B (see index_walk / index_single); returns whether
+// a list ends inside the round or at its last position.
+template <int U, bool NARROW>
+__device__ __forceinline__ bool sup_index(uint32_t B, uint32_t total, uint32_t mybase, uint32_t myend, uint32_t lane,
+                                          uint32_t (&idx)[U]) {
+    constexpr uint32_t RB = 64u * U;
+    const uint32_t sg = first_segment(B, myend);
+    if (B + RB > total)
+        index_walk<U, false, NARROW>(B, total, mybase, myend, sg, lane, idx);
+    else if (one_segment<U>(B, myend))
+        index_single<U, NARROW>(read_lane(mybase, sg) + B, lane, idx);
+    else
+        index_walk<U, true, NARROW>(B, total, mybase, myend, sg, lane, idx);
+    return ballot(myend > B && myend <= B + RB) != 0;
+}
+
+// Streams one window: lane c offers the list col[s, s + len) (len 0: none) of
+// candidate c.  Returns in lane j the triangles found in the window's j-th
+// non-empty list (whose candidate index is S.cand[j]); nseg = their number.
+template <int U, bool NARROW>
+__device__ __forceinline__ uint32_t sup_window(SupWave &S, const uint32_t *__restrict__ col,
+                                               uint32_t *__restrict__ support, uint32_t s, uint32_t len, uint32_t c,
+                                               uint32_t qmask, uint32_t qshift, uint32_t lane, uint32_t &nseg) {
+    constexpr uint32_t RB = 64u * U;
+    const uint32_t incl = wave_incl_scan(len, lane);
+    const uint32_t total = read_lane(incl, 63);  // the lists are disjoint ranges of col: total <= m
+    const uint64_t nzm = ballot(len != 0);
+    nseg = (uint32_t)__popcll(nzm);
+    if (total == 0) return 0;
+    if (len != 0) {
+        const uint32_t k = mask_rank(nzm);
+        S.base[k] = s - (incl - len);
+        S.end[k] = incl;
+        S.cand[k] = c;
+    }
+    const uint32_t mybase = S.base[lane];
+    const uint32_t myend = lane < nseg ? S.end[lane] : kSupEmpty;
+    uint32_t cum = 0, cend = 0;
+    uint32_t ia[U], ib[U], xa[U], xb[U];
+    bool ea = sup_index<U, NARROW>(0, total, mybase, myend, lane, ia), eb;
+    load_round<U, NARROW>(col, ia, xa);
+    uint32_t B = 0;
+    for (; B + 3 * RB <= total; B += 2 * RB) {  // rounds B + RB and B + 2 RB are full
+        eb = sup_index<U, NARROW>(B + RB, total, mybase, myend, lane, ib);
+        load_round<U, NARROW>(col, ib, xb);
+        sup_probe<U, true, NARROW>(S, xa, ia, B, total, ea, myend, qmask, qshift, support, lane, cum, cend);
+        ea = sup_index<U, NARROW>(B + 2 * RB, total, mybase, myend, lane, ia);
+        load_round<U, NARROW>(col, ia, xa);
+        sup_probe<U, true, NARROW>(S, xb, ib, B + RB, total, eb, myend, qmask, qshift, support, lane, cum, cend);
+    }
+    // round B is in xa; rounds B + RB and B + 2 RB may follow, the last one partial
+    if (B + RB >= total) {
+        sup_probe<U, false, NARROW>(S, xa, ia, B, total, ea, myend, qmask, qshift, support, lane, cum, cend);
+    } else {
+        eb = sup_index<U, NARROW>(B + RB, total, mybase, myend, lane, ib);
+        load_round<U, NARROW>(col, ib, xb);
+        sup_probe<U, true, NARROW>(S, xa, ia, B, total, ea, myend, qmask, qshift, support, lane, cum, cend);
+        if (B + 2 * RB < total) {
+            ea = sup_index<U, NARROW>(B + 2 * RB, total, mybase, myend, lane, ia);
+            load_round<U, NARROW>(col, ia, xa);
+            sup_probe<U, true, NARROW>(S, xb, ib, B + RB, total, eb, myend, qmask, qshift, support, lane, cum, cend);
+            sup_probe<U, false, NARROW>(S, xa, ia, B + 2 * RB, total, ea, myend, qmask, qshift, support, lane, cum,
+                                        cend);
+        } else {
+            sup_probe<U, false, NARROW>(S, xb, ib, B + RB, total, eb, myend, qmask, qshift, support, lane, cum, cend);
+        }
+    }
+    // list j's triangles: the count at its end minus the count at the end of list j - 1
+    const uint32_t prev = gather(cend, lane - 1u);
+    return lane < nseg ? cend - (lane ? prev : 0u) : 0u;
+}
+
+// Position of key r in the sorted list col[b, e) (r is there).
+__device__ __forceinline__ uint64_t sup_position(const uint32_t *__restrict__ col, uint64_t b, uint64_t e, uint32_t r) {
+    while (e - b > 1) {
+        const uint64_t mid = (b + e) >> 1;
+        if (col[mid] <= r) b = mid; else e = mid;
+    }
+    return b;
+}
+
+// One work item at row r (N+(r) = col[rb, rb + K), hashed tile by tile).
+// Candidates [0, Ka): kind A, the c-th neighbour w of r, N+(w) streamed (only
+// when d+(w) < below, if below != 0); candidates [Ka, Ka + Kb): kind B, the
+// in-list source u = src[c - Ka], N+(u) streamed.
+template <int U, bool NARROW>
+__device__ __forceinline__ void sup_item(SupWave &S, const uint64_t *__restrict__ off, const uint32_t *__restrict__ col,
+                                         const uint32_t *__restrict__ src, uint32_t n_rows, uint32_t r, uint64_t rb,
+                                         uint32_t K, uint32_t Ka, uint32_t Kb, uint32_t below,
+                                         uint32_t *__restrict__ support, unsigned long long *__restrict__ per_vertex,
+                                         uint32_t lane) {
+    const uint32_t C = Ka + Kb;
+    unsigned long long row_tri = 0;
+    for (uint32_t t0 = 0; t0 < K; t0 += kSupTile) {
+        const uint32_t T = min(K - t0, kSupTile);
+        const uint32_t tl = sup_tlog2(T), qmask = (1u << (tl - 2)) - 1u, qshift = 34 - tl;
+        sup_build(S, col, rb + t0, T, tl, lane);
+        const bool tiled = K > kSupTile;
+        const uint32_t lo = gload(col + rb + t0), hi = gload(col + rb + t0 + T - 1);
+        for (uint32_t c0 = 0; c0 < C; c0 += 64) {
+            if (c0 != 0 && (c0 & (kFlushCands - 1u)) == 0) sup_flush(S, rb + t0, tl, support, per_vertex, lane);
+            const uint32_t c = c0 + lane;
+            uint32_t s = 0, len = 0;
+            if (c < Ka) {
+                const uint32_t w = gload(col + rb + c);
+                if (w < n_rows) {
+                    const uint64_t s0 = gload(off + w);
+                    s = (uint32_t)s0;
+                    len = (uint32_t)(gload(off + w + 1) - s0);
+                    if (below && len >= below) len = 0;  // that arc streams N+(r), at w
+                }
+            } else if (c < C) {
+                const uint32_t u = gload(src + (c - Ka));
+                if (u < n_rows) {
+                    const uint64_t s0 = gload(off + u);
+                    s = (uint32_t)s0;
+                    len = (uint32_t)(gload(off + u + 1) - s0);
+                }
+            }
+            // a list that cannot meet this tile is skipped (only when N+(r) is tiled)
+            if (tiled && len && (gload(col + s + len - 1) < lo || gload(col + s) > hi)) len = 0;
+            uint32_t nseg;
+            const uint32_t h = sup_window<U, NARROW>(S, col, support, s, len, c, qmask, qshift, lane, nseg);
+            if (lane < nseg && h != 0) {
+                const uint32_t cc = S.cand[lane];
+                if (cc < Ka) {  // arc r -> w
+                    const uint64_t a = rb + cc;
+                    if (support) atomicAdd(support + a, h);
+                    if (per_vertex) atomicAdd(per_vertex + gload(col + a), (unsigned long long)h);
+                } else {  // arc u -> r
+                    const uint32_t u = gload(src + (cc - Ka));
+                    if (support) atomicAdd(support + sup_position(col, gload(off + u), gload(off + u + 1), r), h);
+                    if (per_vertex) atomicAdd(per_vertex + u, (unsigned long long)h);
+                }
+                row_tri += h;
+            }
+        }
+        sup_flush(S, rb + t0, tl, support, per_vertex, lane);
+    }
+    const unsigned long long tot = wave_sum64(row_tri);
+    if (per_vertex && lane == 0 && tot) atomicAdd(per_vertex + r, tot);
+}
+
+// Plain walk: one item per owned row u with d+(u) >= 2, every neighbour kind A.
+template <bool NARROW>
+__global__ __launch_bounds__(kSupWaves * 64, 4) void k_support(const uint64_t *__restrict__ off,
+                                                               const uint32_t *__restrict__ col, uint32_t n_rows,
+                                                               uint32_t n_owned, uint32_t *__restrict__ support,
+                                                               unsigned long long *__restrict__ per_vertex) {
     __shared__ SupWave s_w[kSupWaves];
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    SupWave &S = s_w[wv];
-    uint32_t *slot = (uint32_t *)S.tab;
-    const uint64_t le_mask = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
     const uint64_t nw = (uint64_t)gridDim.x * kSupWaves;
     for (uint64_t u = (uint64_t)blockIdx.x * kSupWaves + wv; u < n_owned; u += nw) {
         const uint64_t rb = off[u];
         const uint32_t K = (uint32_t)(off[u + 1] - rb);
         if (K < 2) continue;
-        unsigned long long row_tri = 0;
-        for (uint32_t t0 = 0; t0 < K; t0 += kSupTile) {
-            const uint32_t T = min(K - t0, kSupTile);
-            for (uint32_t i = lane; i < kSupSlots; i += 64) slot[i] = kEmptySlot;
-            for (uint32_t i = lane; i < T; i += 64) S.cnt[i] = 0;
-            for (uint32_t i = lane; i < T; i += 64) {
-                const uint32_t key = col[rb + t0 + i];
-                uint32_t b = sup_home(key);
-                for (uint32_t step = 0; step < kSupBuckets; ++step) {  // a slot is tried once the ones before it are taken
-                    uint32_t *q = slot + 4 * b;
-                    int sl = -1;
-                    if (atomicCAS(q, kEmptySlot, key) == kEmptySlot) sl = 0;
-                    else if (atomicCAS(q + 1, kEmptySlot, key) == kEmptySlot) sl = 1;
-                    else if (atomicCAS(q + 2, kEmptySlot, key) == kEmptySlot) sl = 2;
-                    else if (atomicCAS(q + 3, kEmptySlot, key) == kEmptySlot) sl = 3;
-                    if (sl >= 0) {
-                        S.tidx[4 * b + sl] = (uint16_t)i;
-                        break;
-                    }
-                    b = (b + 1) & (kSupBuckets - 1);
-                }
-            }
-            const uint32_t lo = col[rb + t0], hi = col[rb + t0 + T - 1];
-            const bool tiled = K > kSupTile;
-            for (uint32_t j0 = 0; j0 < K; j0 += 64) {
-                const uint32_t j = j0 + lane;
-                uint64_t wb = 0;
-                uint32_t len = 0;
-                if (j < K) {
-                    const uint32_t w = col[rb + j];
-                    if (w < n_rows) {
-                        wb = off[w];
-                        len = (uint32_t)(off[w + 1] - wb);
-                        // a list that cannot meet this tile is skipped (only when N+(u) is tiled)
-                        if (tiled && len && (col[wb + len - 1] < lo || col[wb] > hi)) len = 0;
-                    }
-                }
-                const uint64_t ne = ballot(len > 0);
-                if (ne == 0) continue;
-                const uint32_t incl = wave_incl_scan(len, lane);
-                const uint32_t total = read_lane(incl, 63);
-                const uint32_t st = incl - len;  // this lane's segment start (when non-empty)
-                if (len > 0) {                   // non-empty segments, compacted in order
-                    const uint32_t c = mask_rank(ne);
-                    S.start[c] = st;
-                    S.base[c] = wb;
-                    S.nb[c] = j;
-                    S.hits[c] = 0;
-                }
-                S.mark[lane] = 0;
-                for (uint32_t B = 0; B < total; B += 64 * kSupUnroll) {
-                    uint64_t pos[kSupUnroll];
-                    uint32_t sg[kSupUnroll], x[kSupUnroll];
-                    bool ok[kSupUnroll];
-#pragma unroll
-                    for (int r = 0; r < kSupUnroll; ++r) {
-                        const uint32_t Br = B + 64 * r;
-                        // segment of position Br + lane: the last one starting at or before it
-                        const uint32_t j0r = (uint32_t)__popcll(ballot(len > 0 && st <= Br)) - 1u;
-                        const bool in = len > 0 && st > Br && st < Br + 64;
-                        if (in) S.mark[st - Br] = 1;
-                        const uint64_t marks = ballot(S.mark[lane] != 0);
-                        if (in) S.mark[st - Br] = 0;
-                        sg[r] = j0r + (uint32_t)__popcll(marks & le_mask);
-                        ok[r] = Br + lane < total;
-                        pos[r] = ok[r] ? S.base[sg[r]] + (Br + lane - S.start[sg[r]]) : 0u;
-                    }
-#pragma unroll
-                    for (int r = 0; r < kSupUnroll; ++r) x[r] = ok[r] ? col[pos[r]] : kEmptySlot;
-#pragma unroll
-                    for (int r = 0; r < kSupUnroll; ++r) {
-                        if (x[r] < lo || x[r] > hi) continue;  // also every position past the end
-                        const int i = sup_find(S, x[r]);
-                        if (i >= 0) {
-                            atomicAdd(&S.cnt[i], 1u);                       // arc u -> x
-                            atomicAdd(&S.hits[sg[r]], 1u);                  // arc u -> w
-                            if (support) atomicAdd(&support[pos[r]], 1u);  // arc w -> x
-                        }
-                    }
-                }
-                if (lane < (uint32_t)__popcll(ne)) {
-                    const uint32_t h = S.hits[lane];
-                    if (h) {
-                        const uint64_t a = rb + S.nb[lane];
-                        if (support) atomicAdd(&support[a], h);
-                        if (per_vertex) atomicAdd(&per_vertex[col[a]], (unsigned long long)h);
-                        row_tri += h;
-                    }
-                }
-            }
-            for (uint32_t i = lane; i < T; i += 64) {
-                const uint32_t c = S.cnt[i];
-                if (!c) continue;
-                if (support) atomicAdd(&support[rb + t0 + i], c);                                // arc u -> x
-                if (per_vertex) atomicAdd(&per_vertex[col[rb + t0 + i]], (unsigned long long)c);  // vertex x
-            }
-        }
-        const unsigned long long tot = wave_sum64(row_tri);
-        if (per_vertex && lane == 0 && tot) atomicAdd(&per_vertex[u], tot);
+        sup_item<kSupUnroll, NARROW>(s_w[wv], off, col, nullptr, n_rows, (uint32_t)u, rb, K, K, 0, 0, support,
+                                     per_vertex, lane);
     }
 }
 
@@ -184,14 +344,10 @@ __global__ __launch_bounds__(kSupWaves * 64) void k_support(const uint64_t *__re
 // count (count_engine.hip): arc u -> w is handled at u (kind A: N+(u) in the
 // table, N+(w) streamed) when d+(w) < d+(u), else at w (kind B: N+(w) in the
 // table, N+(u) streamed for u in w's in-list), so every arc streams its shorter
-// list (at RMAT s23 2.9e10 entries instead of 7.3e10).  A triangle u -> w -> x
-// found at row r adds to the tile counter of x (arc r -> x), to the segment
-// counter of the streamed list (arc r -> w for kind A; arc u -> r for kind B,
-// whose position in N+(u) one binary search finds per segment with hits), and
-// with a global atomic to the streamed entry (arc w -> x, or u -> x).
-// Work items: chunk 0 of a row holds its kind A side, chunk c its in-list
-// sources [512c, 512c + 512), so a hub's long in-list is spread over waves;
-// persistent waves take items from a ticket.
+// list (at RMAT s23 2.9e10 entries instead of 7.3e10).  Work items: chunk 0 of
+// a row holds its kind A side, chunk c its in-list sources [512c, 512c + 512),
+// so a hub's long in-list is spread over waves; persistent waves take items
+// from a ticket.
 static constexpr uint32_t kSupChunk = 512;
 
 __global__ void k_sup_items(const uint64_t *__restrict__ off, const uint64_t *__restrict__ in_off, uint32_t n_rows,
@@ -210,35 +366,14 @@ __global__ void k_sup_item_rows(const uint32_t *__restrict__ items, const uint64
         for (uint32_t c = 0; c < items[r]; ++c) item_row[item_off[r] + c] = (uint32_t)r;
 }
 
-// Tile index of x, the first bucket read inline; the walk on is rare (a full home).
-__device__ __forceinline__ int sup_lookup(const SupWave &S, uint32_t x) {
-    const uint32_t b = sup_home(x);
-    const uint4 v = S.tab[b];
-    const int sl = v.x == x ? 0 : v.y == x ? 1 : v.z == x ? 2 : v.w == x ? 3 : -1;
-    if (sl >= 0) return S.tidx[4 * b + sl];
-    if (v.w == kEmptySlot) return -1;
-    return sup_find(S, x);
-}
-
-// Position of key r in the sorted list col[b, e) (r is there).
-__device__ __forceinline__ uint64_t sup_position(const uint32_t *__restrict__ col, uint64_t b, uint64_t e, uint32_t r) {
-    while (e - b > 1) {
-        const uint64_t mid = (b + e) >> 1;
-        if (col[mid] <= r) b = mid; else e = mid;
-    }
-    return b;
-}
-
-__global__ __launch_bounds__(kSupWaves * 64) void k_support_split(
+template <bool NARROW>
+__global__ __launch_bounds__(kSupWaves * 64, 4) void k_support_split(
     const uint64_t *__restrict__ off, const uint32_t *__restrict__ col, const uint64_t *__restrict__ in_off,
     const uint32_t *__restrict__ in_col, uint32_t n_rows, uint32_t n_owned, const uint64_t *__restrict__ item_off,
     const uint32_t *__restrict__ item_row, uint32_t *__restrict__ ticket, uint32_t *__restrict__ support,
     unsigned long long *__restrict__ per_vertex) {
     __shared__ SupWave s_w[kSupWaves];
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    SupWave &S = s_w[wv];
-    uint32_t *slot = (uint32_t *)S.tab;
-    const uint64_t le_mask = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
     const uint64_t total = item_off[n_rows];
     for (;;) {
         uint32_t t = 0;
@@ -253,118 +388,9 @@ __global__ __launch_bounds__(kSupWaves * 64) void k_support_split(
         const uint32_t Ka = (chunk == 0 && r < n_owned && K >= 2) ? K : 0u;
         const uint64_t b0 = (uint64_t)chunk * kSupChunk;
         const uint32_t Kb = b0 < kb ? (uint32_t)min(kb - b0, (uint64_t)kSupChunk) : 0u;
-        const uint32_t C = Ka + Kb;  // candidates: kind A neighbours, then kind B sources
-        if (C == 0) continue;
-        unsigned long long row_tri = 0;
-        for (uint32_t t0 = 0; t0 < K; t0 += kSupTile) {
-            const uint32_t T = min(K - t0, kSupTile);
-            for (uint32_t i = lane; i < kSupSlots; i += 64) slot[i] = kEmptySlot;
-            for (uint32_t i = lane; i < T; i += 64) S.cnt[i] = 0;
-            for (uint32_t i = lane; i < T; i += 64) {
-                const uint32_t key = col[rb + t0 + i];
-                uint32_t b = sup_home(key);
-                for (uint32_t step = 0; step < kSupBuckets; ++step) {  // a slot is tried once the ones before it are taken
-                    uint32_t *q = slot + 4 * b;
-                    int sl = -1;
-                    if (atomicCAS(q, kEmptySlot, key) == kEmptySlot) sl = 0;
-                    else if (atomicCAS(q + 1, kEmptySlot, key) == kEmptySlot) sl = 1;
-                    else if (atomicCAS(q + 2, kEmptySlot, key) == kEmptySlot) sl = 2;
-                    else if (atomicCAS(q + 3, kEmptySlot, key) == kEmptySlot) sl = 3;
-                    if (sl >= 0) {
-                        S.tidx[4 * b + sl] = (uint16_t)i;
-                        break;
-                    }
-                    b = (b + 1) & (kSupBuckets - 1);
-                }
-            }
-            const uint32_t lo = col[rb + t0], hi = col[rb + t0 + T - 1];
-            const bool tiled = K > kSupTile;
-            for (uint32_t c0 = 0; c0 < C; c0 += 64) {
-                const uint32_t c = c0 + lane;
-                uint64_t wb = 0;
-                uint32_t len = 0;
-                if (c < Ka) {  // kind A: N+(w) when it is the shorter list
-                    const uint32_t w = col[rb + c];
-                    if (w < n_rows) {
-                        wb = off[w];
-                        len = (uint32_t)(off[w + 1] - wb);
-                        if (len >= K) len = 0;
-                    }
-                } else if (c < C) {  // kind B: N+(u) of an in-list source u (d+(u) <= d+(r))
-                    const uint32_t u = in_col[ib + b0 + (c - Ka)];
-                    wb = off[u];
-                    len = (uint32_t)(off[u + 1] - wb);
-                }
-                // a list that cannot meet this tile is skipped (only when N+(r) is tiled)
-                if (tiled && len && (col[wb + len - 1] < lo || col[wb] > hi)) len = 0;
-                const uint64_t ne = ballot(len > 0);
-                if (ne == 0) continue;
-                const uint32_t incl = wave_incl_scan(len, lane);
-                const uint32_t total_e = read_lane(incl, 63);
-                const uint32_t st = incl - len;
-                if (len > 0) {
-                    const uint32_t k = mask_rank(ne);
-                    S.start[k] = st;
-                    S.base[k] = wb;
-                    S.nb[k] = c;
-                    S.hits[k] = 0;
-                }
-                S.mark[lane] = 0;
-                for (uint32_t B = 0; B < total_e; B += 64 * kSupUnroll) {
-                    uint64_t pos[kSupUnroll];
-                    uint32_t sg[kSupUnroll], x[kSupUnroll];
-                    bool ok[kSupUnroll];
-#pragma unroll
-                    for (int q = 0; q < kSupUnroll; ++q) {
-                        const uint32_t Br = B + 64 * q;
-                        const uint32_t j0r = (uint32_t)__popcll(ballot(len > 0 && st <= Br)) - 1u;
-                        const bool in = len > 0 && st > Br && st < Br + 64;
-                        if (in) S.mark[st - Br] = 1;
-                        const uint64_t marks = ballot(S.mark[lane] != 0);
-                        if (in) S.mark[st - Br] = 0;
-                        sg[q] = j0r + (uint32_t)__popcll(marks & le_mask);
-                        ok[q] = Br + lane < total_e;
-                        pos[q] = ok[q] ? S.base[sg[q]] + (Br + lane - S.start[sg[q]]) : 0u;
-                    }
-#pragma unroll
-                    for (int q = 0; q < kSupUnroll; ++q) x[q] = ok[q] ? col[pos[q]] : kEmptySlot;
-#pragma unroll
-                    for (int q = 0; q < kSupUnroll; ++q) {
-                        if (x[q] < lo || x[q] > hi) continue;  // also every position past the end
-                        const int i = sup_lookup(S, x[q]);
-                        if (i >= 0) {
-                            atomicAdd(&S.cnt[i], 1u);                       // arc r -> x
-                            atomicAdd(&S.hits[sg[q]], 1u);                  // arc r -> w, or u -> r
-                            if (support) atomicAdd(&support[pos[q]], 1u);  // arc w -> x, or u -> x
-                        }
-                    }
-                }
-                if (lane < (uint32_t)__popcll(ne)) {
-                    const uint32_t h = S.hits[lane];
-                    if (h) {
-                        const uint32_t cc = S.nb[lane];
-                        if (cc < Ka) {
-                            const uint64_t a = rb + cc;
-                            if (support) atomicAdd(&support[a], h);
-                            if (per_vertex) atomicAdd(&per_vertex[col[a]], (unsigned long long)h);
-                        } else {
-                            const uint32_t u = in_col[ib + b0 + (cc - Ka)];
-                            if (support) atomicAdd(&support[sup_position(col, off[u], off[u + 1], r)], h);
-                            if (per_vertex) atomicAdd(&per_vertex[u], (unsigned long long)h);
-                        }
-                        row_tri += h;
-                    }
-                }
-            }
-            for (uint32_t i = lane; i < T; i += 64) {
-                const uint32_t c = S.cnt[i];
-                if (!c) continue;
-                if (support) atomicAdd(&support[rb + t0 + i], c);                                // arc r -> x
-                if (per_vertex) atomicAdd(&per_vertex[col[rb + t0 + i]], (unsigned long long)c);  // vertex x
-            }
-        }
-        const unsigned long long tot = wave_sum64(row_tri);
-        if (per_vertex && lane == 0 && tot) atomicAdd(&per_vertex[r], tot);
+        if (Ka + Kb == 0) continue;
+        sup_item<kSupUnroll, NARROW>(s_w[wv], off, col, in_col + ib + b0, n_rows, r, rb, K, Ka, Kb, K, support,
+                                     per_vertex, lane);
     }
 }
 
@@ -402,13 +428,19 @@ extern "C" lstc_status lstc_triangle_support(const uint64_t *d_off, const uint32
     if (n_rows && !d_off) return fail(LSTC_ERR_INVALID, "lstc_triangle_support: NULL offsets");
     if (m && !d_col) return fail(LSTC_ERR_INVALID, "lstc_triangle_support: NULL columns");
     if (!d_support && !d_per_vertex) return fail(LSTC_ERR_INVALID, "lstc_triangle_support: no output");
+    if (m >= 0xFFFFF000ull)  // 32-bit column positions in the walk
+        return fail(LSTC_ERR_INVALID, "lstc_triangle_support: m = %llu arcs exceeds 2^32-4096", (unsigned long long)m);
     if (d_support && m) LSTC_TRY(fill_u32(d_support, 0, m, st));
     if (d_per_vertex && n_rows) LSTC_TRY(fill_u64(d_per_vertex, 0, n_rows, st));
     if (n_owned == 0 || m == 0) return LSTC_OK;
     const uint32_t grid = (uint32_t)hmin<uint64_t>(((uint64_t)n_owned + kSupWaves - 1) / kSupWaves,
                                                    (uint64_t)device_cus() * 4);
-    hipLaunchKernelGGL(k_support, dim3(grid), dim3(kSupWaves * 64), 0, st, d_off, d_col, n_rows, n_owned, d_support,
-                       (unsigned long long *)d_per_vertex);
+    if (m < (1ull << 30))  // column positions as 32-bit byte offsets
+        hipLaunchKernelGGL(k_support<true>, dim3(grid), dim3(kSupWaves * 64), 0, st, d_off, d_col, n_rows, n_owned,
+                           d_support, (unsigned long long *)d_per_vertex);
+    else
+        hipLaunchKernelGGL(k_support<false>, dim3(grid), dim3(kSupWaves * 64), 0, st, d_off, d_col, n_rows, n_owned,
+                           d_support, (unsigned long long *)d_per_vertex);
     LSTC_LAUNCHED("k_support");
     return LSTC_OK;
 }
@@ -431,6 +463,9 @@ extern "C" lstc_status lstc_triangle_support_split(const uint64_t *d_off, const 
     if (n_rows && (!d_off || !d_in_off)) return fail(LSTC_ERR_INVALID, "lstc_triangle_support_split: NULL offsets");
     if (m && (!d_col || !d_in_col)) return fail(LSTC_ERR_INVALID, "lstc_triangle_support_split: NULL columns");
     if (!d_support && !d_per_vertex) return fail(LSTC_ERR_INVALID, "lstc_triangle_support_split: no output");
+    if (m >= 0xFFFFF000ull)  // 32-bit column positions in the walk
+        return fail(LSTC_ERR_INVALID, "lstc_triangle_support_split: m = %llu arcs exceeds 2^32-4096",
+                    (unsigned long long)m);
     if (2 * (uint64_t)n_rows + m / kSupChunk + 1 >= 0xFFFFFFFFull)
         return fail(LSTC_ERR_INVALID, "lstc_triangle_support_split: too many work items");
     const SupLayout L = sup_layout(d_ws, ws_bytes, n_rows, m);
@@ -447,9 +482,14 @@ extern "C" lstc_status lstc_triangle_support_split(const uint64_t *d_off, const 
     LSTC_LAUNCHED("k_sup_item_rows");
     LSTC_TRY(fill_u32(L.ticket, 0, 1, st));
     const uint32_t grid = (uint32_t)device_cus() * 4;  // persistent: four workgroups of four waves per CU (LDS)
-    hipLaunchKernelGGL(k_support_split, dim3(grid), dim3(kSupWaves * 64), 0, st, d_off, d_col, d_in_off, d_in_col,
-                       n_rows, n_owned, (const uint64_t *)L.item_off, (const uint32_t *)L.item_row, L.ticket, d_support,
-                       (unsigned long long *)d_per_vertex);
+    if (m < (1ull << 30))  // column positions as 32-bit byte offsets
+        hipLaunchKernelGGL(k_support_split<true>, dim3(grid), dim3(kSupWaves * 64), 0, st, d_off, d_col, d_in_off,
+                           d_in_col, n_rows, n_owned, (const uint64_t *)L.item_off, (const uint32_t *)L.item_row,
+                           L.ticket, d_support, (unsigned long long *)d_per_vertex);
+    else
+        hipLaunchKernelGGL(k_support_split<false>, dim3(grid), dim3(kSupWaves * 64), 0, st, d_off, d_col, d_in_off,
+                           d_in_col, n_rows, n_owned, (const uint64_t *)L.item_off, (const uint32_t *)L.item_row,
+                           L.ticket, d_support, (unsigned long long *)d_per_vertex);
     LSTC_LAUNCHED("k_support_split");
     return LSTC_OK;
 }
