@@ -67,3 +67,21 @@ def test_support_empty_and_invalid(lstc, oracle):
     assert int(sup.sum().item()) == 0 and int(pv.sum().item()) == 0
     with pytest.raises(lstc.LstcError):
         lstc.triangle_support(g, per_edge=False, per_vertex=False)
+
+
+def test_support_tiled_workgroup_rows(lstc, oracle):
+    """K_2500: rows up to d+ = 2499 are hashed in two workgroup tiles of 2048 keys.  Closed
+    forms: every arc's support is n - 2, every vertex's count (n - 1)(n - 2) / 2, on the plain
+    walk and on the split rule."""
+    n = 2500
+    _, og = oracle.clean_orient(*G.complete(n))
+    assert int(np.diff(og.off).max()) == n - 1
+    g = lstc.Graph.from_numpy(og.off, og.col)
+    for split in (False, True):
+        if split:
+            lstc.in_lists(g)
+        sup, pv = lstc.triangle_support(g, split=split)
+        s = sup.cpu().numpy().view(np.uint32)
+        p = pv.cpu().numpy().view(np.uint64)
+        assert s.shape == (n * (n - 1) // 2,) and bool((s == n - 2).all()), split
+        assert bool((p == (n - 1) * (n - 2) // 2).all()), split
