@@ -29,12 +29,16 @@
 
 namespace lstc {
 
-static constexpr int kSupWaves = 4;
-static constexpr uint32_t kSupWaveLog2 = 10;                              // slots of a wave's table
+// A workgroup of WAVES waves (4 for the split rule, 8 for the plain walk: the
+// faster of the two for each at RMAT s23) holds one table of 2^kSupGroupLog2
+// slots, or one slice of it per wave.
 static constexpr uint32_t kSupGroupLog2 = 12;                             // slots of a workgroup's table
-static_assert((kSupWaves << kSupWaveLog2) == (1u << kSupGroupLog2), "a workgroup table is its waves' tables");
-static constexpr uint32_t kSupWaveTile = (1u << kSupWaveLog2) / 2;        // keys per tile: load <= 1/2
-static constexpr uint32_t kSupGroupTile = (1u << kSupGroupLog2) / 2;
+static constexpr uint32_t kSupGroupTile = (1u << kSupGroupLog2) / 2;      // keys per tile: load <= 1/2
+template <int WAVES>
+constexpr uint32_t sup_wave_log2() { return kSupGroupLog2 - (WAVES == 8 ? 3 : 2); }  // slots of a wave's table
+template <int WAVES>
+constexpr uint32_t sup_wave_tile() { return (1u << sup_wave_log2<WAVES>()) / 2; }
+static constexpr int kSupSplitWaves = 4, kSupPlainWaves = 8;
 static constexpr int kSupUnroll = 4;                                       // chunks of 64 entries per round
 static constexpr uint32_t kSupEmpty = 0xFFFFFFFFu;     // empty slot; also the end of a lane without a list
 static constexpr uint32_t kSupNoId = 0xFFFFFFFEu;      // past-the-end positions: never a key (ids < 2^32 - 2)
@@ -49,9 +53,10 @@ struct SupTab {
     uint32_t *tab, *cnt, *ovf;
 };
 // A window's non-empty lists: j-th list's column start minus its flat start,
-// its flat end (exclusive), its candidate index.
+// its flat end (exclusive), its candidate's lane in the window.
 struct SupWin {
-    uint32_t *base, *end, *cand;
+    uint32_t *base, *end;
+    uint8_t *cand;
 };
 
 // log2 of the table slots for T keys: at least twice as many slots, 16 at least.
@@ -212,12 +217,12 @@ __device__ __forceinline__ uint64_t sup_index(uint32_t B, uint32_t total, uint32
     return ballot(myend > B && myend <= B + RB);
 }
 
-// Streams one window: lane c offers the list col[s, s + len) (len 0: none) of
-// candidate c.  Returns in lane j the triangles found in the window's j-th
-// non-empty list (whose candidate index is W.cand[j]); nseg = their number.
+// Streams one window: each lane offers the list col[s, s + len) (len 0: none)
+// of its candidate.  Returns in lane j the triangles found in the window's j-th
+// non-empty list (whose candidate's lane is W.cand[j]); nseg = their number.
 template <int U, bool NARROW>
 __device__ __forceinline__ uint32_t sup_window(const SupTab &Q, const SupWin &W, const uint32_t *__restrict__ col,
-                                               uint32_t *__restrict__ support, uint32_t s, uint32_t len, uint32_t c,
+                                               uint32_t *__restrict__ support, uint32_t s, uint32_t len,
                                                uint32_t qmask, uint32_t qshift, uint32_t lane, uint32_t &nseg) {
     constexpr uint32_t RB = 64u * U;
     const uint32_t incl = wave_incl_scan(len, lane);
@@ -229,7 +234,7 @@ __device__ __forceinline__ uint32_t sup_window(const SupTab &Q, const SupWin &W,
         const uint32_t k = mask_rank(nzm);
         W.base[k] = s - (incl - len);
         W.end[k] = incl;
-        W.cand[k] = c;
+        W.cand[k] = (uint8_t)lane;
     }
     const uint32_t mybase = W.base[lane];
     const uint32_t myend = lane < nseg ? W.end[lane] : kSupEmpty;
@@ -278,20 +283,20 @@ __device__ __forceinline__ uint64_t sup_position(const uint32_t *__restrict__ co
 }
 
 // One work item at row r (N+(r) = col[rb, rb + K)), run by NT threads (one
-// wave, or the workgroup sharing one table; wave wv then takes every
-// (NT / 64)-th window).  Candidates [0, Ka): kind A, the c-th neighbour w of r,
+// wave, or the workgroup sharing one table, whose waves claim windows from the
+// counter *next).  Candidates [0, Ka): kind A, the c-th neighbour w of r,
 // N+(w) streamed (only when d+(w) < below, if below != 0); candidates
 // [Ka, Ka + Kb): kind B, the in-list source u = src[c - Ka], N+(u) streamed.
-// N+(r) is hashed in tiles of TILE keys; the candidates go in passes of at most
-// kSupPass, each followed by a flush of the slot counters.
-template <int U, bool NARROW, int NT>
+// N+(r) is hashed in tiles of TILE keys (a wave tile, or kSupGroupTile); the
+// candidates go in passes of at most kSupPass, each followed by a flush of the
+// slot counters.
+template <int U, bool NARROW, int NT, uint32_t TILE>
 __device__ __forceinline__ void sup_item(const SupTab &Q, const SupWin &W, const uint64_t *__restrict__ off,
                                          const uint32_t *__restrict__ col, const uint32_t *__restrict__ src,
                                          uint32_t n_rows, uint32_t r, uint64_t rb, uint32_t K, uint32_t Ka,
                                          uint32_t Kb, uint32_t below, uint32_t *__restrict__ support,
-                                         unsigned long long *__restrict__ per_vertex, uint32_t tid) {
-    constexpr uint32_t TILE = NT > 64 ? kSupGroupTile : kSupWaveTile;
-    const uint32_t lane = tid & 63, wv = tid >> 6;
+                                         unsigned long long *__restrict__ per_vertex, uint32_t *next, uint32_t tid) {
+    const uint32_t lane = tid & 63;
     const uint32_t C = Ka + Kb;
     unsigned long long row_tri = 0;
     for (uint32_t t0 = 0; t0 < K; t0 += TILE) {
@@ -302,7 +307,17 @@ __device__ __forceinline__ void sup_item(const SupTab &Q, const SupWin &W, const
         const uint32_t lo = gload(col + rb + t0), hi = gload(col + rb + t0 + T - 1);
         for (uint32_t p0 = 0; p0 < C; p0 += kSupPass) {
             const uint32_t p1 = min(C, p0 + kSupPass);
-            for (uint32_t c0 = p0 + 64 * wv; c0 < p1; c0 += NT) {
+            if (NT > 64) {
+                if (tid == 0) *next = p0;
+                __syncthreads();
+            }
+            for (uint32_t c0 = p0;; c0 += 64) {
+                if (NT > 64) {  // the workgroup's next window
+                    uint32_t t = 0;
+                    if (lane == 0) t = atomicAdd(next, 64u);
+                    c0 = read_lane(t, 0);
+                }
+                if (c0 >= p1) break;
                 const uint32_t c = c0 + lane;
                 uint32_t s = 0, len = 0;
                 if (c < min(Ka, p1)) {
@@ -324,9 +339,9 @@ __device__ __forceinline__ void sup_item(const SupTab &Q, const SupWin &W, const
                 // a list that cannot meet this tile is skipped (only when N+(r) is tiled)
                 if (tiled && len && (gload(col + s + len - 1) < lo || gload(col + s) > hi)) len = 0;
                 uint32_t nseg;
-                const uint32_t h = sup_window<U, NARROW>(Q, W, col, support, s, len, c, qmask, qshift, lane, nseg);
+                const uint32_t h = sup_window<U, NARROW>(Q, W, col, support, s, len, qmask, qshift, lane, nseg);
                 if (lane < nseg && h != 0) {
-                    const uint32_t cc = W.cand[lane];
+                    const uint32_t cc = c0 + W.cand[lane];
                     if (cc < Ka) {  // arc r -> w
                         const uint64_t a = rb + cc;
                         if (support) atomicAdd(support + a, h);
@@ -346,59 +361,70 @@ __device__ __forceinline__ void sup_item(const SupTab &Q, const SupWin &W, const
     if (per_vertex && lane == 0 && tot) atomicAdd(per_vertex + r, tot);
 }
 
-// LDS of a support workgroup: the four wave tables, or one table of all of
-// them; one window per wave.
+// LDS of a support workgroup: the wave tables, or one table of all of them;
+// one window per wave.  Under 40 KB: four workgroups per CU.
+static constexpr uint32_t kSupScanRows = 256;  // rows checked per step for workgroup rows (plain walk)
+template <int WAVES>
 struct SupShared {
-    uint32_t tab[kSupWaves << kSupWaveLog2];
-    uint32_t cnt[kSupWaves << kSupWaveLog2];
-    uint32_t ovf[(kSupWaves << kSupWaveLog2) / 128];
-    uint32_t base[kSupWaves][64], end[kSupWaves][64], cand[kSupWaves][64];
-    uint32_t rows[kSupWaves * 64];
-    uint32_t nrows, ticket;
+    uint32_t tab[1u << kSupGroupLog2];
+    uint32_t cnt[1u << kSupGroupLog2];
+    uint32_t ovf[(1u << kSupGroupLog2) / 128];
+    uint32_t base[WAVES][64], end[WAVES][64];
+    uint8_t cand[WAVES][64];
+    uint32_t rows[kSupScanRows];
+    uint32_t nrows, ticket, next;
 };
 
-__device__ __forceinline__ SupTab sup_group_tab(SupShared &L) { return SupTab{L.tab, L.cnt, L.ovf}; }
-__device__ __forceinline__ SupTab sup_wave_tab(SupShared &L, uint32_t wv) {
-    return SupTab{L.tab + (wv << kSupWaveLog2), L.cnt + (wv << kSupWaveLog2), L.ovf + (wv << kSupWaveLog2) / 128};
+template <int WAVES>
+__device__ __forceinline__ SupTab sup_group_tab(SupShared<WAVES> &L) { return SupTab{L.tab, L.cnt, L.ovf}; }
+template <int WAVES>
+__device__ __forceinline__ SupTab sup_wave_tab(SupShared<WAVES> &L, uint32_t wv) {
+    constexpr uint32_t l2 = sup_wave_log2<WAVES>();
+    return SupTab{L.tab + (wv << l2), L.cnt + (wv << l2), L.ovf + (wv << l2) / 128};
 }
-__device__ __forceinline__ SupWin sup_win(SupShared &L, uint32_t wv) { return SupWin{L.base[wv], L.end[wv], L.cand[wv]}; }
+template <int WAVES>
+__device__ __forceinline__ SupWin sup_win(SupShared<WAVES> &L, uint32_t wv) {
+    return SupWin{L.base[wv], L.end[wv], L.cand[wv]};
+}
 
 // Plain walk: every neighbour kind A.  Rows with d+ above a wave tile are found
-// 256 at a time and run by the whole workgroup; then each wave takes the rows
-// with 2 <= d+ <= kSupWaveTile.
-template <bool NARROW>
-__global__ __launch_bounds__(kSupWaves * 64, 4) void k_support(const uint64_t *__restrict__ off,
-                                                               const uint32_t *__restrict__ col, uint32_t n_rows,
-                                                               uint32_t n_owned, uint32_t *__restrict__ support,
-                                                               unsigned long long *__restrict__ per_vertex) {
-    __shared__ SupShared L;
+// kSupScanRows at a time and run by the whole workgroup; then each wave takes
+// the rows with 2 <= d+ <= the wave tile.
+template <bool NARROW, int WAVES>
+__global__ __launch_bounds__(WAVES * 64, WAVES == 8 ? 8 : 4) void k_support(const uint64_t *__restrict__ off,
+                                                                           const uint32_t *__restrict__ col,
+                                                                           uint32_t n_rows, uint32_t n_owned,
+                                                                           uint32_t *__restrict__ support,
+                                                                           unsigned long long *__restrict__ per_vertex) {
+    constexpr uint32_t WT = sup_wave_tile<WAVES>();
+    __shared__ SupShared<WAVES> L;
     const uint32_t tid = threadIdx.x, wv = tid >> 6;
     const SupWin W = sup_win(L, wv);
-    for (uint64_t b0 = (uint64_t)blockIdx.x * (kSupWaves * 64); b0 < n_owned;
-         b0 += (uint64_t)gridDim.x * (kSupWaves * 64)) {
+    for (uint64_t b0 = (uint64_t)blockIdx.x * kSupScanRows; b0 < n_owned; b0 += (uint64_t)gridDim.x * kSupScanRows) {
         if (tid == 0) L.nrows = 0;
         __syncthreads();
         const uint64_t u = b0 + tid;
-        if (u < n_owned && off[u + 1] - off[u] > kSupWaveTile) L.rows[atomicAdd(&L.nrows, 1u)] = (uint32_t)u;
+        if (tid < kSupScanRows && u < n_owned && off[u + 1] - off[u] > WT)
+            L.rows[atomicAdd(&L.nrows, 1u)] = (uint32_t)u;
         __syncthreads();
         const uint32_t nb = L.nrows;
         for (uint32_t i = 0; i < nb; ++i) {
             const uint32_t r = L.rows[i];
             const uint64_t rb = off[r];
             const uint32_t K = (uint32_t)(off[r + 1] - rb);
-            sup_item<kSupUnroll, NARROW, kSupWaves * 64>(sup_group_tab(L), W, off, col, nullptr, n_rows, r, rb, K, K, 0,
-                                                         0, support, per_vertex, tid);
+            sup_item<kSupUnroll, NARROW, WAVES * 64, kSupGroupTile>(sup_group_tab(L), W, off, col, nullptr, n_rows, r,
+                                                                    rb, K, K, 0, 0, support, per_vertex, &L.next, tid);
         }
         __syncthreads();
     }
     const SupTab Q = sup_wave_tab(L, wv);
-    const uint64_t nw = (uint64_t)gridDim.x * kSupWaves;
-    for (uint64_t u = (uint64_t)blockIdx.x * kSupWaves + wv; u < n_owned; u += nw) {
+    const uint64_t nw = (uint64_t)gridDim.x * WAVES;
+    for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wv; u < n_owned; u += nw) {
         const uint64_t rb = off[u];
         const uint32_t K = (uint32_t)(off[u + 1] - rb);
-        if (K < 2 || K > kSupWaveTile) continue;
-        sup_item<kSupUnroll, NARROW, 64>(Q, W, off, col, nullptr, n_rows, (uint32_t)u, rb, K, K, 0, 0, support,
-                                         per_vertex, tid & 63);
+        if (K < 2 || K > WT) continue;
+        sup_item<kSupUnroll, NARROW, 64, WT>(Q, W, off, col, nullptr, n_rows, (uint32_t)u, rb, K, K, 0, 0, support,
+                                             per_vertex, nullptr, tid & 63);
     }
 }
 
@@ -416,14 +442,15 @@ static constexpr uint32_t kSupChunk = 512;
 
 // Items per row, group rows and wave rows counted apart.
 __global__ void k_sup_items(const uint64_t *__restrict__ off, const uint64_t *__restrict__ in_off, uint32_t n_rows,
-                            uint32_t n_owned, uint32_t *__restrict__ items_g, uint32_t *__restrict__ items_w) {
+                            uint32_t n_owned, uint32_t wave_tile, uint32_t *__restrict__ items_g,
+                            uint32_t *__restrict__ items_w) {
     for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_rows; r += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t K = off[r + 1] - off[r], kb = in_off[r + 1] - in_off[r];
         const uint32_t nb = (uint32_t)((kb + kSupChunk - 1) / kSupChunk);
         const uint32_t a = (r < n_owned && K >= 2) ? 1u : 0u;
         const uint32_t it = K == 0 ? 0u : (nb > a ? nb : a);
-        items_g[r] = K > kSupWaveTile ? it : 0u;
-        items_w[r] = K > kSupWaveTile ? 0u : it;
+        items_g[r] = K > wave_tile ? it : 0u;
+        items_w[r] = K > wave_tile ? 0u : it;
     }
 }
 
@@ -438,12 +465,13 @@ __global__ void k_sup_item_rows(const uint32_t *__restrict__ items_g, const uint
     }
 }
 
-template <bool NARROW, int NT>
+template <bool NARROW, int NT, uint32_t TILE>
 __device__ __forceinline__ void sup_split_item(const SupTab &Q, const SupWin &W, const uint64_t *__restrict__ off,
                                                const uint32_t *__restrict__ col, const uint64_t *__restrict__ in_off,
                                                const uint32_t *__restrict__ in_col, uint32_t n_rows, uint32_t n_owned,
                                                uint32_t r, uint32_t chunk, uint32_t *__restrict__ support,
-                                               unsigned long long *__restrict__ per_vertex, uint32_t tid) {
+                                               unsigned long long *__restrict__ per_vertex, uint32_t *next,
+                                               uint32_t tid) {
     const uint64_t rb = off[r];
     const uint32_t K = (uint32_t)(off[r + 1] - rb);
     const uint64_t ib = in_off[r], kb = in_off[r + 1] - ib;
@@ -451,17 +479,17 @@ __device__ __forceinline__ void sup_split_item(const SupTab &Q, const SupWin &W,
     const uint64_t b0 = (uint64_t)chunk * kSupChunk;
     const uint32_t Kb = b0 < kb ? (uint32_t)min(kb - b0, (uint64_t)kSupChunk) : 0u;
     if (Ka + Kb == 0) return;
-    sup_item<kSupUnroll, NARROW, NT>(Q, W, off, col, in_col + ib + b0, n_rows, r, rb, K, Ka, Kb, K, support,
-                                     per_vertex, tid);
+    sup_item<kSupUnroll, NARROW, NT, TILE>(Q, W, off, col, in_col + ib + b0, n_rows, r, rb, K, Ka, Kb, K, support,
+                                           per_vertex, next, tid);
 }
 
-template <bool NARROW>
-__global__ __launch_bounds__(kSupWaves * 64, 4) void k_support_split(
+template <bool NARROW, int WAVES>
+__global__ __launch_bounds__(WAVES * 64, WAVES == 8 ? 8 : 4) void k_support_split(
     const uint64_t *__restrict__ off, const uint32_t *__restrict__ col, const uint64_t *__restrict__ in_off,
     const uint32_t *__restrict__ in_col, uint32_t n_rows, uint32_t n_owned, const uint64_t *__restrict__ off_g,
     const uint64_t *__restrict__ off_w, const uint32_t *__restrict__ item_row, uint32_t *__restrict__ ticket,
     uint32_t *__restrict__ support, unsigned long long *__restrict__ per_vertex) {
-    __shared__ SupShared L;
+    __shared__ SupShared<WAVES> L;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const SupWin W = sup_win(L, wv);
     const uint32_t n_g = (uint32_t)off_g[n_rows], n_w = (uint32_t)off_w[n_rows];
@@ -472,8 +500,9 @@ __global__ __launch_bounds__(kSupWaves * 64, 4) void k_support_split(
         __syncthreads();
         if (t >= n_g) break;
         const uint32_t r = item_row[t];
-        sup_split_item<NARROW, kSupWaves * 64>(sup_group_tab(L), W, off, col, in_off, in_col, n_rows, n_owned, r,
-                                               t - (uint32_t)off_g[r], support, per_vertex, tid);
+        sup_split_item<NARROW, WAVES * 64, kSupGroupTile>(sup_group_tab(L), W, off, col, in_off, in_col, n_rows,
+                                                          n_owned, r, t - (uint32_t)off_g[r], support, per_vertex,
+                                                          &L.next, tid);
     }
     const SupTab Q = sup_wave_tab(L, wv);
     for (;;) {  // wave items (every wave reaches this break once the items are taken)
@@ -482,8 +511,8 @@ __global__ __launch_bounds__(kSupWaves * 64, 4) void k_support_split(
         t = read_lane(t, 0);
         if (t >= n_w) break;
         const uint32_t r = item_row[n_g + t];
-        sup_split_item<NARROW, 64>(Q, W, off, col, in_off, in_col, n_rows, n_owned, r, t - (uint32_t)off_w[r],
-                                   support, per_vertex, lane);
+        sup_split_item<NARROW, 64, sup_wave_tile<WAVES>()>(Q, W, off, col, in_off, in_col, n_rows, n_owned, r,
+                                                           t - (uint32_t)off_w[r], support, per_vertex, nullptr, lane);
     }
 }
 
@@ -528,13 +557,15 @@ extern "C" lstc_status lstc_triangle_support(const uint64_t *d_off, const uint32
     if (d_support && m) LSTC_TRY(fill_u32(d_support, 0, m, st));
     if (d_per_vertex && n_rows) LSTC_TRY(fill_u64(d_per_vertex, 0, n_rows, st));
     if (n_owned == 0 || m == 0) return LSTC_OK;
-    const uint32_t grid = (uint32_t)hmin<uint64_t>(((uint64_t)n_owned + kSupWaves - 1) / kSupWaves,
+    const uint32_t grid = (uint32_t)hmin<uint64_t>(((uint64_t)n_owned + kSupPlainWaves - 1) / kSupPlainWaves,
                                                    (uint64_t)device_cus() * 4);  // four workgroups per CU (LDS)
     if (m < (1ull << 30))  // column positions as 32-bit byte offsets
-        hipLaunchKernelGGL(k_support<true>, dim3(grid), dim3(kSupWaves * 64), 0, st, d_off, d_col, n_rows, n_owned,
+        hipLaunchKernelGGL((k_support<true, kSupPlainWaves>), dim3(grid), dim3(kSupPlainWaves * 64), 0, st, d_off, d_col,
+                           n_rows, n_owned,
                            d_support, (unsigned long long *)d_per_vertex);
     else
-        hipLaunchKernelGGL(k_support<false>, dim3(grid), dim3(kSupWaves * 64), 0, st, d_off, d_col, n_rows, n_owned,
+        hipLaunchKernelGGL((k_support<false, kSupPlainWaves>), dim3(grid), dim3(kSupPlainWaves * 64), 0, st, d_off,
+                           d_col, n_rows, n_owned,
                            d_support, (unsigned long long *)d_per_vertex);
     LSTC_LAUNCHED("k_support");
     return LSTC_OK;
@@ -570,8 +601,8 @@ extern "C" lstc_status lstc_triangle_support_split(const uint64_t *d_off, const 
     if (d_per_vertex && n_rows) LSTC_TRY(fill_u64(d_per_vertex, 0, n_rows, st));
     if (n_rows == 0 || m == 0) return LSTC_OK;
     const uint32_t grid_n = (uint32_t)hmin<uint64_t>(((uint64_t)n_rows + 255) / 256, (uint64_t)device_cus() * 16);
-    hipLaunchKernelGGL(k_sup_items, dim3(grid_n), dim3(256), 0, st, d_off, d_in_off, n_rows, n_owned, L.items_g,
-                       L.items_w);
+    hipLaunchKernelGGL(k_sup_items, dim3(grid_n), dim3(256), 0, st, d_off, d_in_off, n_rows, n_owned,
+                       sup_wave_tile<kSupSplitWaves>(), L.items_g, L.items_w);
     LSTC_LAUNCHED("k_sup_items");
     LSTC_TRY(scan_exclusive_u32_to_u64(L.items_g, L.off_g, n_rows, L.off_g + n_rows, L.scan_ws, L.scan_bytes, st));
     LSTC_TRY(scan_exclusive_u32_to_u64(L.items_w, L.off_w, n_rows, L.off_w + n_rows, L.scan_ws, L.scan_bytes, st));
@@ -579,13 +610,15 @@ extern "C" lstc_status lstc_triangle_support_split(const uint64_t *d_off, const 
                        n_rows, L.item_row);
     LSTC_LAUNCHED("k_sup_item_rows");
     LSTC_TRY(fill_u32(L.ticket, 0, 2, st));
-    const uint32_t grid = (uint32_t)device_cus() * 4;  // persistent: four workgroups of four waves per CU (LDS)
+    const uint32_t grid = (uint32_t)device_cus() * 4;  // persistent: four workgroups per CU (LDS)
     if (m < (1ull << 30))  // column positions as 32-bit byte offsets
-        hipLaunchKernelGGL(k_support_split<true>, dim3(grid), dim3(kSupWaves * 64), 0, st, d_off, d_col, d_in_off,
+        hipLaunchKernelGGL((k_support_split<true, kSupSplitWaves>), dim3(grid), dim3(kSupSplitWaves * 64), 0, st, d_off,
+                           d_col, d_in_off,
                            d_in_col, n_rows, n_owned, (const uint64_t *)L.off_g, (const uint64_t *)L.off_w,
                            (const uint32_t *)L.item_row, L.ticket, d_support, (unsigned long long *)d_per_vertex);
     else
-        hipLaunchKernelGGL(k_support_split<false>, dim3(grid), dim3(kSupWaves * 64), 0, st, d_off, d_col, d_in_off,
+        hipLaunchKernelGGL((k_support_split<false, kSupSplitWaves>), dim3(grid), dim3(kSupSplitWaves * 64), 0, st,
+                           d_off, d_col, d_in_off,
                            d_in_col, n_rows, n_owned, (const uint64_t *)L.off_g, (const uint64_t *)L.off_w,
                            (const uint32_t *)L.item_row, L.ticket, d_support, (unsigned long long *)d_per_vertex);
     LSTC_LAUNCHED("k_support_split");
