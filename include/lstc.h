@@ -235,7 +235,8 @@ lstc_status lstc_count_split_static(const uint64_t *d_off, const uint32_t *d_col
  * n_owned = n_rows): d_support[m] (uint32, may be NULL) receives for arc j of
  * the oriented CSR the number of triangles containing that edge, d_per_vertex
  * [n_rows] (uint64, may be NULL) the number containing each vertex.  Each sums
- * to 3x the count.  Stream-ordered, no workspace.                            */
+ * to 3x the count.  m < 2^32 - 4096 (32-bit positions inside the walk, as
+ * lstc_count).  Stream-ordered, no workspace.                               */
 lstc_status lstc_triangle_support(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows, uint32_t n_owned,
                                   uint64_t m, uint32_t *d_support, uint64_t *d_per_vertex, lstc_stream s);
 
