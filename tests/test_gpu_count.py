@@ -155,7 +155,8 @@ def test_in_lists_of_any_csr(lstc, oracle, cut_frac):
 def test_count_above_2_30_arcs(lstc):
     """m > 2^30 arcs, so column byte offsets need more than 32 bits: disjoint K64
     copies with id-ordered arcs, built in HBM directly.  Closed form: copies * C(64, 3),
-    plain and split (in-lists and segment words)."""
+    plain and split (in-lists and segment words); per-edge support k - 2 on every arc and
+    per-vertex counts C(k - 1, 2) on every vertex, from both support walks."""
     k = 64
     per = k * (k - 1) // 2
     copies = (1 << 30) // per + 1
@@ -177,3 +178,8 @@ def test_count_above_2_30_arcs(lstc):
     lstc.in_lists(g)
     assert lstc.count(g) == want
     assert lstc.count(g, tune=lstc.Tune(unroll=8)) == want
+    for split in (False, True):
+        sup, pv = lstc.triangle_support(g, split=split)
+        assert sup.shape[0] == m and bool((sup == k - 2).all()), split
+        assert pv.shape[0] == n and bool((pv == (k - 1) * (k - 2) // 2).all()), split
+        del sup, pv
