@@ -386,7 +386,10 @@ def _serve(T, reqs, budget_gb, t_scale=1.0):
                  "mean_ms": round(float(lat.mean()) * 1e3, 3), "max_ms": round(float(lat.max()) * 1e3, 3),
                  "batches": int(st["batches"]), "graphs_per_launch": round(len(reqs) / max(1, st["batches"]), 2),
                  "count_kernel_busy": round(kt.ms / 1e3 / wall, 3),
-                 "max_batch_graphs": int(st["max_batch_graphs"]), "rejected": int(st["rejected"])}
+                 "max_batch_graphs": int(st["max_batch_graphs"]), "rejected": int(st["rejected"]),
+                 "prep_groups": int(st["prep_groups"]),
+                 "graphs_per_prep": round(len(reqs) / max(1, st["prep_groups"]), 2),
+                 "max_prep_graphs": int(st["max_prep_graphs"])}
 
 
 def serving(T, n_req, rate, budget_gb, seed, verify):

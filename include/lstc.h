@@ -402,6 +402,7 @@ typedef struct lstc_server_stats {
     uint64_t submitted, completed, rejected, batches, max_batch_graphs;
     uint64_t pool_bytes, pool_used_bytes, edges, triangles;
     double p50_s, p99_s, mean_s, max_s;
+    uint64_t prep_groups, max_prep_graphs; /* preparations (groups of requests) and the largest */
 } lstc_server_stats;
 
 typedef struct lstc_server lstc_server;

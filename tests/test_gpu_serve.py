@@ -144,3 +144,33 @@ def test_cancel_queued_request(lstc):
         assert st["completed"] == 3
     finally:
         srv.close()
+
+
+def test_grouped_preparation_exact(lstc, oracle):
+    """A burst of generated requests is prepared in groups (their union through one
+    clean/orient pipeline, then split back into one CSR per graph with local ids):
+    every count and edge total equals the oracle's, with caller CSRs interleaved
+    (a group stops at one) and a budget that forces some groups to be halved."""
+    rng = np.random.default_rng(21)
+    reqs = _req_list(rng, 150, 1e3, 5e4)
+    want = [_oracle_count(oracle, *r) for r in reqs]
+    _, og = oracle.clean_orient(*oracle.gen_rmat(10, 8, seed=4))
+    g = lstc.Graph.from_numpy(og.off, og.col)
+    csr_want = (oracle.count(og), og.m)
+    for budget in (4 << 30, 96 << 20):
+        srv = lstc.Server(budget)
+        try:
+            ids, exp = [], []
+            for i, r in enumerate(reqs):
+                ids.append(_submit(srv, r))
+                exp.append(want[i])
+                if i in (40, 100):
+                    ids.append(srv.submit_graph(g))
+                    exp.append(csr_want)
+            got = [srv.wait(x, 120) for x in ids]
+            assert [(r["count"], r["edges"]) for r in got] == exp, budget
+            st = srv.stats()
+            assert st["completed"] == len(ids) and st["pool_used_bytes"] == 0
+            assert st["max_prep_graphs"] > 1 and st["prep_groups"] < len(reqs), st
+        finally:
+            srv.close()

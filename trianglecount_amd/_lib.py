@@ -80,7 +80,8 @@ class ServerStats(C.Structure):
     """lstc_server_stats (include/lstc.h)."""
     _fields_ = [(k, C.c_uint64) for k in ("submitted", "completed", "rejected", "batches", "max_batch_graphs",
                                           "pool_bytes", "pool_used_bytes", "edges", "triangles")] + \
-               [(k, C.c_double) for k in ("p50_s", "p99_s", "mean_s", "max_s")]
+               [(k, C.c_double) for k in ("p50_s", "p99_s", "mean_s", "max_s")] + \
+               [(k, C.c_uint64) for k in ("prep_groups", "max_prep_graphs")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

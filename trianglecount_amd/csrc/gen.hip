@@ -25,8 +25,10 @@ static inline uint64_t prob_to_u64(double p) {
 
 // RMAT: one 32-bit uniform per recursion level, two levels per 64-bit draw,
 // quadrants a:(0,0) b:(0,1) c:(1,0) d:(1,1), level 0 = most significant bit.  Stream 1.
+// Ids are written plus id0 (0 but in the server's grouped preparation, where
+// each graph takes its own range of a union's ids).
 __global__ void k_gen_rmat(uint32_t scale, uint64_t m_raw, uint32_t tA, uint32_t tAB,
-                           uint32_t tABC, uint64_t key, uint32_t *__restrict__ src,
+                           uint32_t tABC, uint64_t key, uint32_t id0, uint32_t *__restrict__ src,
                            uint32_t *__restrict__ dst) {
     const uint32_t nh = (scale + 1) / 2;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m_raw;
@@ -41,19 +43,19 @@ __global__ void k_gen_rmat(uint32_t scale, uint64_t m_raw, uint32_t tA, uint32_t
             if (q & 1u) d |= bit;
             if (q & 2u) s |= bit;
         }
-        src[i] = s;
-        dst[i] = d;
+        src[i] = id0 + s;
+        dst[i] = id0 + d;
     }
 }
 
 // Uniform pairs (G(n, M) sampling).  Stream 2.
-__global__ void k_gen_er(uint32_t n, uint64_t m_raw, uint64_t key, uint32_t *__restrict__ src,
+__global__ void k_gen_er(uint32_t n, uint64_t m_raw, uint64_t key, uint32_t id0, uint32_t *__restrict__ src,
                          uint32_t *__restrict__ dst) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m_raw;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t r = rng64(key, i);
-        src[i] = mulhi32((uint32_t)r, n);
-        dst[i] = mulhi32((uint32_t)(r >> 32), n);
+        src[i] = id0 + mulhi32((uint32_t)r, n);
+        dst[i] = id0 + mulhi32((uint32_t)(r >> 32), n);
     }
 }
 
@@ -132,6 +134,32 @@ static bool chung_lu_table(uint32_t n, double gamma, double mean_deg, double w_m
     return true;
 }
 
+lstc_status gen_rmat_launch(uint32_t scale, uint64_t m_raw, double a, double b, double c, uint64_t seed, uint32_t id0,
+                            uint32_t *d_src, uint32_t *d_dst, hipStream_t s) {
+    if (scale == 0 || scale > 31) return fail(LSTC_ERR_INVALID, "lstc_gen_rmat: scale %u not in 1..31", scale);
+    if (!(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0))
+        return fail(LSTC_ERR_INVALID, "lstc_gen_rmat: bad a/b/c");
+    if ((uint64_t)id0 + (1ull << scale) > 0xFFFFFFFFull) return fail(LSTC_ERR_INVALID, "lstc_gen_rmat: ids overflow");
+    if (m_raw == 0) return LSTC_OK;
+    if (!d_src || !d_dst) return fail(LSTC_ERR_INVALID, "lstc_gen_rmat: NULL output");
+    hipLaunchKernelGGL(k_gen_rmat, dim3(gen_grid_dim(m_raw)), dim3(256), 0, s, scale, m_raw, prob_to_u32(a),
+                       prob_to_u32(a + b), prob_to_u32(a + b + c), stream_key(seed, 1), id0, d_src, d_dst);
+    LSTC_LAUNCHED("k_gen_rmat");
+    return LSTC_OK;
+}
+
+lstc_status gen_er_launch(uint32_t n, uint64_t m_raw, uint64_t seed, uint32_t id0, uint32_t *d_src, uint32_t *d_dst,
+                          hipStream_t s) {
+    if (n == 0 || n == 0xFFFFFFFFu) return fail(LSTC_ERR_INVALID, "lstc_gen_er: bad n");
+    if ((uint64_t)id0 + n > 0xFFFFFFFFull) return fail(LSTC_ERR_INVALID, "lstc_gen_er: ids overflow");
+    if (m_raw == 0) return LSTC_OK;
+    if (!d_src || !d_dst) return fail(LSTC_ERR_INVALID, "lstc_gen_er: NULL output");
+    hipLaunchKernelGGL(k_gen_er, dim3(gen_grid_dim(m_raw)), dim3(256), 0, s, n, m_raw, stream_key(seed, 2), id0,
+                       d_src, d_dst);
+    LSTC_LAUNCHED("k_gen_er");
+    return LSTC_OK;
+}
+
 }  // namespace lstc
 
 using namespace lstc;
@@ -140,28 +168,13 @@ extern "C" lstc_status lstc_gen_rmat(uint32_t scale, uint64_t m_raw, double a, d
                                      uint64_t seed, uint32_t *d_src, uint32_t *d_dst,
                                      lstc_stream s) {
     TraceRange range_("lstc_gen_rmat");
-    if (scale == 0 || scale > 31) return fail(LSTC_ERR_INVALID, "lstc_gen_rmat: scale %u not in 1..31", scale);
-    if (!(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0))
-        return fail(LSTC_ERR_INVALID, "lstc_gen_rmat: bad a/b/c");
-    if (m_raw == 0) return LSTC_OK;
-    if (!d_src || !d_dst) return fail(LSTC_ERR_INVALID, "lstc_gen_rmat: NULL output");
-    hipLaunchKernelGGL(k_gen_rmat, dim3(gen_grid_dim(m_raw)), dim3(256), 0, (hipStream_t)s, scale,
-                       m_raw, prob_to_u32(a), prob_to_u32(a + b), prob_to_u32(a + b + c),
-                       stream_key(seed, 1), d_src, d_dst);
-    LSTC_LAUNCHED("k_gen_rmat");
-    return LSTC_OK;
+    return gen_rmat_launch(scale, m_raw, a, b, c, seed, 0, d_src, d_dst, (hipStream_t)s);
 }
 
 extern "C" lstc_status lstc_gen_er(uint32_t n, uint64_t m_raw, uint64_t seed, uint32_t *d_src,
                                    uint32_t *d_dst, lstc_stream s) {
     TraceRange range_("lstc_gen_er");
-    if (n == 0 || n == 0xFFFFFFFFu) return fail(LSTC_ERR_INVALID, "lstc_gen_er: bad n");
-    if (m_raw == 0) return LSTC_OK;
-    if (!d_src || !d_dst) return fail(LSTC_ERR_INVALID, "lstc_gen_er: NULL output");
-    hipLaunchKernelGGL(k_gen_er, dim3(gen_grid_dim(m_raw)), dim3(256), 0, (hipStream_t)s, n, m_raw,
-                       stream_key(seed, 2), d_src, d_dst);
-    LSTC_LAUNCHED("k_gen_er");
-    return LSTC_OK;
+    return gen_er_launch(n, m_raw, seed, 0, d_src, d_dst, (hipStream_t)s);
 }
 
 extern "C" lstc_status lstc_gen_chung_lu_workspace(uint32_t n, size_t *bytes) {

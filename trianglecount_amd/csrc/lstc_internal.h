@@ -108,6 +108,14 @@ lstc_status clean_orient_launch(const uint32_t *d_src, const uint32_t *d_dst, ui
                                 uint64_t *d_in_seg, void *d_ws, size_t ws_bytes, hipStream_t st,
                                 const uint64_t **d_sizes, const uint32_t **d_bad);
 
+// Seeded generators (gen.hip) writing every id plus id0: lstc_gen_rmat and
+// lstc_gen_er are these with id0 = 0; the server's grouped preparation gives
+// each graph its own id range of one union.
+lstc_status gen_rmat_launch(uint32_t scale, uint64_t m_raw, double a, double b, double c, uint64_t seed, uint32_t id0,
+                            uint32_t *d_src, uint32_t *d_dst, hipStream_t s);
+lstc_status gen_er_launch(uint32_t n, uint64_t m_raw, uint64_t seed, uint32_t id0, uint32_t *d_src, uint32_t *d_dst,
+                          hipStream_t s);
+
 // Fill kernels (used instead of hipMemsetAsync so callers may capture them in graphs).
 lstc_status fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s);
 lstc_status fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);

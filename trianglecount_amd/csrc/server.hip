@@ -7,10 +7,17 @@
 //             memory and their preparation (seeded generation + clean/orient +
 //             in-lists + segment words) enqueued on one of kPrepStreams prep
 //             streams, round robin, so a large request does not hold up the
-//             small ones behind it;
-//   collect : prepared requests (prep event done) release their transient
-//             memory, have their out-of-range flag checked and wait for
-//             counting; their CSR stays resident;
+//             small ones behind it.  Generated requests waiting together are
+//             prepared as one group (up to kGroupMax): each generates into its
+//             own id range of one union edge list, the union goes through one
+//             clean/orient pipeline (graphs never share an edge, so cleaning and
+//             orientation act on each alone), and two kernels copy every
+//             graph's rows back into its own CSR with local ids -- one launch
+//             sequence for the group instead of ~45 launches per request, which
+//             is what bounds a host thread under load;
+//   collect : prepared groups (prep event done) release their transient
+//             memory, have their out-of-range flag checked and their requests
+//             wait for counting; each CSR stays resident;
 //   launch  : while one of the kSlots batch slots is free, every prepared
 //             request is counted by ONE persistent batched launch
 //             (count_engine.hip) on the count stream, results copied to pinned
@@ -106,19 +113,22 @@ struct Req {
     lstc_request spec;
     Clock::time_point t_submit;
     // device memory
-    char *transient = nullptr, *resident = nullptr;
-    uint32_t *src = nullptr, *dst = nullptr;
+    char *resident = nullptr;
     uint64_t *off = nullptr;
     uint32_t *col = nullptr;
     uint64_t *in_off = nullptr;  // target-side in-lists (null for caller CSRs)
     uint32_t *in_col = nullptr;
     uint64_t *col_seg = nullptr;  // their segment words
     uint64_t *in_seg = nullptr;
-    void *ws = nullptr;
-    size_t ws_bytes = 0;
     uint32_t n_rows = 0;
     uint64_t m_cap = 0;
-    hipEvent_t prep_done = nullptr;
+};
+
+// Generated requests prepared together (one or more).
+struct PrepGroup {
+    std::vector<Req *> reqs;
+    char *transient = nullptr;  // raw edges, cleaning workspace (and for several graphs their union's CSR)
+    hipEvent_t done = nullptr;
     const uint32_t *d_bad = nullptr;
     int flag_slot = -1;  // pinned host word that receives *d_bad at the end of the preparation
 };
@@ -136,9 +146,100 @@ __global__ void k_gather_edges(const GraphDev *__restrict__ desc, uint32_t G, un
         out[g] = desc[g].off[desc[g].n_rows];
 }
 
-static size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+__host__ __device__ inline size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
 
-static constexpr int kPrepStreams = 3;  // preparation streams (round robin over requests)
+// A request's resident block: oriented CSR (offsets at 0), its in-lists, then
+// the segment words of both (byte offsets; arrays sized for m_raw arcs).
+struct ResLayout {
+    size_t col, in_off, in_col, col_seg, in_seg, bytes;
+};
+__host__ __device__ inline ResLayout res_layout(uint32_t n, uint64_t m_raw) {
+    const size_t mr = m_raw ? m_raw : 1;
+    const size_t offb = round256(8 * ((size_t)n + 1)), half = offb + round256(4 * mr);
+    return ResLayout{offb, half, half + offb, 2 * half, 2 * half + round256(8 * mr), 2 * half + 2 * round256(8 * mr)};
+}
+
+static constexpr int kPrepStreams = 3;  // preparation streams (round robin over groups)
+static constexpr uint32_t kGroupMax = 64;            // requests prepared together at most
+static constexpr uint64_t kGroupRawMax = 1ull << 25;  // their raw edges at most
+static constexpr uint64_t kGroupRowsMax = 1ull << 28; // their vertices at most
+
+// Where a group's graphs go: graph g holds union ids [base[g], base[g + 1])
+// and its resident block starts at slab offset res[g] (sized for mcap[g] raw
+// edges).  Passed by value as a kernel argument.
+struct GroupTable {
+    uint32_t G;
+    uint32_t base[kGroupMax + 1];
+    uint64_t res[kGroupMax];
+    uint64_t mcap[kGroupMax];
+};
+
+// The graph g with base[g] <= x < base[g + 1] (x below base[G]) for the bounds
+// b[0..G]; ascending.
+__device__ __forceinline__ uint32_t group_find(const uint64_t *b, uint32_t G, uint64_t x) {
+    uint32_t lo = 0, hi = G;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (b[mid] <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Rows of a group's union into each graph: offsets relative to the graph's
+// first arc (and first in-list entry), the closing offset by the graph's last row.
+__global__ void k_group_rows(GroupTable T, char *__restrict__ slab, const uint64_t *__restrict__ uoff,
+                             const uint64_t *__restrict__ uin_off) {
+    __shared__ uint64_t s_base[kGroupMax + 1];
+    for (uint32_t g = threadIdx.x; g <= T.G; g += blockDim.x) s_base[g] = T.base[g];
+    __syncthreads();
+    const uint32_t N = T.base[T.G];
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        const uint32_t g = group_find(s_base, T.G, v);
+        const uint32_t b = T.base[g], n = T.base[g + 1] - b, i = v - b;
+        const ResLayout L = res_layout(n, T.mcap[g]);
+        char *blk = slab + T.res[g];
+        uint64_t *off = (uint64_t *)blk, *in_off = (uint64_t *)(blk + L.in_off);
+        const uint64_t a0 = uoff[b], i0 = uin_off[b];
+        off[i] = uoff[v] - a0;
+        in_off[i] = uin_off[v] - i0;
+        if (i + 1 == n) {
+            off[n] = uoff[v + 1] - a0;
+            in_off[n] = uin_off[v + 1] - i0;
+        }
+    }
+}
+
+// Arcs and in-list entries of the union into each graph: ids minus the graph's
+// first id, segment-word starts minus its first arc (lengths stay).
+__global__ void k_group_arcs(GroupTable T, char *__restrict__ slab, const uint64_t *__restrict__ uoff,
+                             const uint32_t *__restrict__ ucol, const uint64_t *__restrict__ useg,
+                             const uint64_t *__restrict__ uin_off, const uint32_t *__restrict__ uin_col,
+                             const uint64_t *__restrict__ uin_seg) {
+    __shared__ uint64_t s_arc[kGroupMax + 1], s_in[kGroupMax + 1];
+    for (uint32_t g = threadIdx.x; g <= T.G; g += blockDim.x) {
+        s_arc[g] = uoff[T.base[g]];
+        s_in[g] = uin_off[T.base[g]];
+    }
+    __syncthreads();
+    const uint64_t M = s_arc[T.G], MI = s_in[T.G];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < M; p += stride) {
+        const uint32_t g = group_find(s_arc, T.G, p);
+        const ResLayout L = res_layout(T.base[g + 1] - T.base[g], T.mcap[g]);
+        char *blk = slab + T.res[g];
+        const uint64_t j = p - s_arc[g];
+        ((uint32_t *)(blk + L.col))[j] = ucol[p] - T.base[g];
+        ((uint64_t *)(blk + L.col_seg))[j] = useg[p] - s_arc[g];
+    }
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < MI; q += stride) {
+        const uint32_t g = group_find(s_in, T.G, q);
+        const ResLayout L = res_layout(T.base[g + 1] - T.base[g], T.mcap[g]);
+        char *blk = slab + T.res[g];
+        const uint64_t j = q - s_in[g];
+        ((uint32_t *)(blk + L.in_col))[j] = uin_col[q] - T.base[g];
+        ((uint64_t *)(blk + L.in_seg))[j] = uin_seg[q] - s_arc[g];
+    }
+}
 static constexpr int kSlots = 2;        // batches that may be queued on the count stream at once
 static constexpr int kFlagSlots = 4096; // requests in preparation at once (pinned flag words)
 
@@ -171,7 +272,7 @@ struct lstc_server {
     uint32_t next_prep = 0;
     BatchSlot slot[kSlots];
     uint32_t next_slot = 0;    // the slot the next launch takes (slots complete in launch order)
-    uint32_t *h_flags = nullptr;  // pinned out-of-range flags of requests in preparation
+    uint32_t *h_flags = nullptr;  // pinned out-of-range flags of groups in preparation
     std::vector<int> free_flags;
 
     std::mutex mu;
@@ -183,13 +284,14 @@ struct lstc_server {
     uint64_t next_id = 1, n_submitted = 0, n_rejected = 0, n_completed = 0, n_batches = 0, edges_done = 0;
     uint64_t triangles_done = 0;
     uint64_t max_batch_seen = 0;
+    uint64_t n_groups = 0, max_group_seen = 0;  // preparation groups
     bool stop = false;
     lstc_status fatal = LSTC_OK;
     std::thread worker;
 
     // worker-private state
     std::deque<Req *> pending, ready;
-    std::vector<Req *> preparing;
+    std::vector<PrepGroup *> preparing;
 };
 
 namespace lstc {
@@ -202,11 +304,16 @@ static size_t prep_bytes(uint32_t n, uint64_t m_raw, size_t *ws_out) {
     *ws_out = ws;
     return round256(8 * m_raw) + ws;  // raw src/dst + clean workspace
 }
-// resident per request: oriented CSR plus its target-side in-lists (split count)
-// oriented CSR, in-lists (both at most m_raw arcs) and their segment words
-static size_t resident_bytes(uint32_t n, uint64_t m_raw) {
-    return 2 * (round256(8 * ((size_t)n + 1)) + round256(4 * (m_raw ? m_raw : 1))) +
-           2 * round256(8 * (m_raw ? m_raw : 1));
+// resident per request: oriented CSR, its target-side in-lists (both at most
+// m_raw arcs) and their segment words (split count)
+static size_t resident_bytes(uint32_t n, uint64_t m_raw) { return res_layout(n, m_raw).bytes; }
+// transient of a group of several graphs: raw union edges, the union's cleaning
+// workspace and its CSR (with in-lists and segment words)
+static size_t group_bytes(uint32_t N, uint64_t M, size_t *ws_out) {
+    size_t ws = 0;
+    lstc_clean_orient_workspace(N, M, &ws);
+    *ws_out = ws;
+    return round256(8 * M) + round256(ws) + res_layout(N, M).bytes;
 }
 static uint32_t request_rows(const lstc_request &r) {
     if (r.kind == LSTC_REQ_RMAT) return 1u << r.scale;
@@ -245,54 +352,134 @@ static lstc_status admit(lstc_server *S) {
             S->ready.push_back(r);
             continue;
         }
-        if (S->free_flags.empty()) break;  // enough requests in preparation for now
-        const uint32_t n = request_rows(r->spec);
-        size_t ws = 0;
-        const size_t tb = prep_bytes(n, r->spec.m_raw, &ws), rb = resident_bytes(n, r->spec.m_raw);
-        char *t = S->pool.alloc(tb);
-        if (!t) break;
-        char *res = S->pool.alloc(rb);
-        if (!res) { S->pool.release(t); break; }
+        if (S->free_flags.empty()) break;  // enough groups in preparation for now
+        // the group: the head and the generated requests right behind it (up to
+        // the first caller CSR or cancelled request), within the group limits
+        uint32_t G = 1;
+        uint64_t N = request_rows(r->spec), M = r->spec.m_raw;
         {
             std::lock_guard<std::mutex> lk(S->mu);
-            S->queued.erase(r->id);  // from here on the request runs to its result
-            S->cancelled.erase(r->id);
+            while (G < kGroupMax && G < S->pending.size()) {
+                const Req *q = S->pending[G];
+                if (q->spec.kind == LSTC_REQ_CSR || S->cancelled.count(q->id)) break;
+                const uint64_t nq = request_rows(q->spec);
+                if (N + nq > kGroupRowsMax || M + q->spec.m_raw > kGroupRawMax) break;
+                N += nq;
+                M += q->spec.m_raw;
+                ++G;
+            }
         }
-        S->pending.pop_front();
-        r->transient = t;
-        r->resident = res;
-        r->src = (uint32_t *)t;
-        r->dst = (uint32_t *)(t + 4 * r->spec.m_raw);
-        r->ws = t + round256(8 * r->spec.m_raw);
-        r->ws_bytes = ws;
-        const size_t half = round256(8 * ((size_t)n + 1)) + round256(4 * (r->spec.m_raw ? r->spec.m_raw : 1));
-        r->off = (uint64_t *)res;
-        r->col = (uint32_t *)(res + round256(8 * ((size_t)n + 1)));
-        r->in_off = (uint64_t *)(res + half);
-        r->in_col = (uint32_t *)(res + half + round256(8 * ((size_t)n + 1)));
-        r->col_seg = (uint64_t *)(res + 2 * half);
-        r->in_seg = (uint64_t *)(res + 2 * half + round256(8 * (r->spec.m_raw ? r->spec.m_raw : 1)));
-        r->n_rows = n;
-        r->m_cap = r->spec.m_raw;
-        r->flag_slot = S->free_flags.back();
+        // memory for the group, halved until it fits (one request alone, or wait)
+        char *t = nullptr;
+        std::vector<char *> res;
+        size_t ws = 0;
+        for (;;) {
+            t = S->pool.alloc(G == 1 ? prep_bytes((uint32_t)N, M, &ws) : group_bytes((uint32_t)N, M, &ws));
+            bool ok = t != nullptr;
+            for (uint32_t g = 0; ok && g < G; ++g) {
+                const Req *q = S->pending[g];
+                char *blk = S->pool.alloc(resident_bytes(request_rows(q->spec), q->spec.m_raw));
+                if (blk) res.push_back(blk); else ok = false;
+            }
+            if (ok) break;
+            for (char *blk : res) S->pool.release(blk);
+            res.clear();
+            S->pool.release(t);
+            t = nullptr;
+            if (G == 1) break;
+            G /= 2;
+            N = M = 0;
+            for (uint32_t g = 0; g < G; ++g) {
+                N += request_rows(S->pending[g]->spec);
+                M += S->pending[g]->spec.m_raw;
+            }
+        }
+        if (!t) break;
+        PrepGroup *grp = new PrepGroup();
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            for (uint32_t g = 0; g < G; ++g) {  // from here on these requests run to their results
+                S->queued.erase(S->pending[g]->id);
+                S->cancelled.erase(S->pending[g]->id);
+            }
+            S->n_groups++;
+            S->max_group_seen = std::max<uint64_t>(S->max_group_seen, G);
+        }
+        for (uint32_t g = 0; g < G; ++g) {
+            Req *q = S->pending.front();
+            S->pending.pop_front();
+            const uint32_t nq = request_rows(q->spec);
+            const ResLayout L = res_layout(nq, q->spec.m_raw);
+            char *blk = res[g];
+            q->resident = blk;
+            q->off = (uint64_t *)blk;
+            q->col = (uint32_t *)(blk + L.col);
+            q->in_off = (uint64_t *)(blk + L.in_off);
+            q->in_col = (uint32_t *)(blk + L.in_col);
+            q->col_seg = (uint64_t *)(blk + L.col_seg);
+            q->in_seg = (uint64_t *)(blk + L.in_seg);
+            q->n_rows = nq;
+            q->m_cap = q->spec.m_raw;
+            grp->reqs.push_back(q);
+        }
+        grp->transient = t;
+        grp->flag_slot = S->free_flags.back();
         S->free_flags.pop_back();
-        S->preparing.push_back(r);  // from here on fail_all releases it
+        S->preparing.push_back(grp);  // from here on fail_all releases its requests
         hipStream_t st = S->prep_stream[S->next_prep];
         S->next_prep = (S->next_prep + 1) % kPrepStreams;
-        lstc_status rc;
-        if (r->spec.kind == LSTC_REQ_RMAT)
-            rc = lstc_gen_rmat(r->spec.scale, r->spec.m_raw, r->spec.a, r->spec.b, r->spec.c, r->spec.seed, r->src,
-                               r->dst, st);
-        else
-            rc = lstc_gen_er(n, r->spec.m_raw, r->spec.seed, r->src, r->dst, st);
-        if (rc != LSTC_OK) return rc;
-        // compact = 0 keeps n rows, so the count needs no host-side sizes
-        LSTC_TRY(clean_orient_launch(r->src, r->dst, r->spec.m_raw, n, 0, nullptr, nullptr, r->off, r->col, r->in_off,
-                                     r->in_col, r->col_seg, r->in_seg, r->ws, r->ws_bytes, st, nullptr, &r->d_bad));
+        uint32_t *src = (uint32_t *)t, *dst = (uint32_t *)(t + 4 * M);
+        char *wsp = t + round256(8 * M);
+        uint32_t id0 = 0;
+        uint64_t e0 = 0;
+        for (Req *q : grp->reqs) {  // graph g generates ids [id0, id0 + n) into its slice of the union
+            if (q->spec.kind == LSTC_REQ_RMAT)
+                LSTC_TRY(gen_rmat_launch(q->spec.scale, q->spec.m_raw, q->spec.a, q->spec.b, q->spec.c, q->spec.seed,
+                                         id0, src + e0, dst + e0, st));
+            else
+                LSTC_TRY(gen_er_launch(q->n_rows, q->spec.m_raw, q->spec.seed, id0, src + e0, dst + e0, st));
+            id0 += q->n_rows;
+            e0 += q->spec.m_raw;
+        }
+        // compact = 0 keeps every id, so the count needs no host-side sizes (and in a
+        // union every graph keeps its id range)
+        if (G == 1) {
+            Req *q = grp->reqs[0];
+            LSTC_TRY(clean_orient_launch(src, dst, M, (uint32_t)N, 0, nullptr, nullptr, q->off, q->col, q->in_off,
+                                         q->in_col, q->col_seg, q->in_seg, wsp, ws, st, nullptr, &grp->d_bad));
+        } else {
+            char *u = wsp + round256(ws);
+            const ResLayout UL = res_layout((uint32_t)N, M);
+            uint64_t *uoff = (uint64_t *)u, *uin_off = (uint64_t *)(u + UL.in_off);
+            uint32_t *ucol = (uint32_t *)(u + UL.col), *uin_col = (uint32_t *)(u + UL.in_col);
+            uint64_t *useg = (uint64_t *)(u + UL.col_seg), *uin_seg = (uint64_t *)(u + UL.in_seg);
+            LSTC_TRY(clean_orient_launch(src, dst, M, (uint32_t)N, 0, nullptr, nullptr, uoff, ucol, uin_off, uin_col,
+                                         useg, uin_seg, wsp, ws, st, nullptr, &grp->d_bad));
+            GroupTable T{};
+            T.G = G;
+            uint32_t b = 0;
+            for (uint32_t g = 0; g < G; ++g) {
+                T.base[g] = b;
+                T.res[g] = (uint64_t)(grp->reqs[g]->resident - S->slab);
+                T.mcap[g] = grp->reqs[g]->m_cap;
+                b += grp->reqs[g]->n_rows;
+            }
+            T.base[G] = b;
+            const uint32_t cus = (uint32_t)device_cus();
+            const uint32_t grid_r = (uint32_t)hmin<uint64_t>((N + 255) / 256, (uint64_t)cus * 8);
+            const uint32_t grid_a = (uint32_t)hmin<uint64_t>((M + 255) / 256, (uint64_t)cus * 16);
+            hipLaunchKernelGGL(k_group_rows, dim3(grid_r), dim3(256), 0, st, T, S->slab, (const uint64_t *)uoff,
+                               (const uint64_t *)uin_off);
+            LSTC_LAUNCHED("k_group_rows");
+            hipLaunchKernelGGL(k_group_arcs, dim3(grid_a), dim3(256), 0, st, T, S->slab, (const uint64_t *)uoff,
+                               (const uint32_t *)ucol, (const uint64_t *)useg, (const uint64_t *)uin_off,
+                               (const uint32_t *)uin_col, (const uint64_t *)uin_seg);
+            LSTC_LAUNCHED("k_group_arcs");
+        }
         // the out-of-range flag of the cleaning comes back with the preparation
-        LSTC_HIP(hipMemcpyAsync(S->h_flags + r->flag_slot, r->d_bad, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        LSTC_HIP(hipEventCreateWithFlags(&r->prep_done, hipEventDisableTiming));
-        LSTC_HIP(hipEventRecord(r->prep_done, st));
+        LSTC_HIP(hipMemcpyAsync(S->h_flags + grp->flag_slot, grp->d_bad, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        LSTC_HIP(hipEventCreateWithFlags(&grp->done, hipEventDisableTiming));
+        LSTC_HIP(hipEventRecord(grp->done, st));
     }
     return LSTC_OK;
 }
@@ -304,32 +491,32 @@ static void publish_failed(lstc_server *S, Req *r, lstc_status st) {
     S->n_completed++;
 }
 
-// Requests on different prep streams finish in any order: every one whose
-// event has completed is collected.
+// Groups on different prep streams finish in any order: every one whose event
+// has completed is collected.
 static lstc_status collect_prepared(lstc_server *S) {
     bool failed_any = false;
     for (size_t i = 0; i < S->preparing.size();) {
-        Req *r = S->preparing[i];
-        hipError_t e = hipEventQuery(r->prep_done);
+        PrepGroup *grp = S->preparing[i];
+        hipError_t e = hipEventQuery(grp->done);
         if (e == hipErrorNotReady) { ++i; continue; }
         if (e != hipSuccess) return hip_fail(e, "server: prep event");
-        (void)hipEventDestroy(r->prep_done);
-        r->prep_done = nullptr;
-        S->pool.release(r->transient);
-        r->transient = nullptr;
-        const uint32_t bad = S->h_flags[r->flag_slot];
-        S->free_flags.push_back(r->flag_slot);
-        r->flag_slot = -1;
+        (void)hipEventDestroy(grp->done);
+        S->pool.release(grp->transient);
+        const uint32_t bad = S->h_flags[grp->flag_slot];
+        S->free_flags.push_back(grp->flag_slot);
         S->preparing[i] = S->preparing.back();
         S->preparing.pop_back();
-        if (bad) {  // an id out of range (or a failed in-list scan): the graph is not counted
-            S->pool.release(r->resident);
-            publish_failed(S, r, (bad & 2u) ? LSTC_ERR_DEVICE : LSTC_ERR_INVALID);
-            delete r;
-            failed_any = true;
-            continue;
+        for (Req *r : grp->reqs) {
+            if (bad) {  // an id out of range (or a failed in-list scan): the graphs are not counted
+                S->pool.release(r->resident);
+                publish_failed(S, r, (bad & 2u) ? LSTC_ERR_DEVICE : LSTC_ERR_INVALID);
+                delete r;
+                failed_any = true;
+            } else {
+                S->ready.push_back(r);
+            }
         }
-        S->ready.push_back(r);
+        delete grp;
     }
     if (failed_any) S->cv_done.notify_all();
     return LSTC_OK;
@@ -435,7 +622,10 @@ static void fail_all(lstc_server *S, lstc_status st) {
         delete r;
     };
     for (Req *r : S->pending) drop(r);
-    for (Req *r : S->preparing) drop(r);
+    for (PrepGroup *g : S->preparing) {
+        for (Req *r : g->reqs) drop(r);
+        delete g;
+    }
     for (Req *r : S->ready) drop(r);
     for (BatchSlot &B : S->slot) {
         for (Req *r : B.batch) drop(r);
@@ -639,6 +829,8 @@ extern "C" lstc_status lstc_server_get_stats(lstc_server *S, lstc_server_stats *
     double s = 0;
     for (double x : l) s += x;
     out->mean_s = l.empty() ? 0.0 : s / (double)l.size();
+    out->prep_groups = S->n_groups;
+    out->max_prep_graphs = S->max_group_seen;
     return LSTC_OK;
 }
 
