@@ -161,6 +161,8 @@ __host__ __device__ inline ResLayout res_layout(uint32_t n, uint64_t m_raw) {
 
 static constexpr int kPrepStreams = 3;  // preparation streams (round robin over groups)
 static constexpr uint32_t kGroupMax = 64;            // requests prepared together at most
+static constexpr size_t kGroupsInFlight = 2 * kPrepStreams;  // groups in preparation at once: under load
+                                                             // the requests behind them form larger groups
 static constexpr uint64_t kGroupRawMax = 1ull << 25;  // their raw edges at most
 static constexpr uint64_t kGroupRowsMax = 1ull << 28; // their vertices at most
 
@@ -352,7 +354,7 @@ static lstc_status admit(lstc_server *S) {
             S->ready.push_back(r);
             continue;
         }
-        if (S->free_flags.empty()) break;  // enough groups in preparation for now
+        if (S->free_flags.empty() || S->preparing.size() >= kGroupsInFlight) break;  // enough in preparation for now
         // the group: the head and the generated requests right behind it (up to
         // the first caller CSR or cancelled request), within the group limits
         uint32_t G = 1;
