@@ -262,6 +262,19 @@ lstc_status lstc_in_lists(const uint64_t *d_off, const uint32_t *d_col, uint32_t
                           uint64_t m, uint64_t *d_in_off, uint32_t *d_in_col, uint64_t *n_entries, void *d_ws,
                           size_t ws_bytes, lstc_stream s);
 
+/* lstc_in_lists and lstc_segments in one call: the in-lists plus the segment
+ * words of every entry of d_col[m] and d_in_col (d_col_seg[m], d_in_seg[m]; the
+ * in-lists hold at most m entries), built in the same passes over the arcs
+ * (the in-list pass writes the column words, the in_col pass the in-list
+ * words), so a bare oriented CSR -- the reference's input,
+ * src/WithoutPartition/ShareKernelTCV62_update_v1.cu:207-230 -- is made ready
+ * for lstc_count_split with one gather of row offsets per entry.  m < 2^32 -
+ * 4096.  Workspace: lstc_in_lists_workspace.  The stream is synchronised.     */
+lstc_status lstc_in_lists_segments(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows, uint32_t n_owned,
+                                   uint64_t m, uint64_t *d_in_off, uint32_t *d_in_col, uint64_t *d_col_seg,
+                                   uint64_t *d_in_seg, uint64_t *n_entries, void *d_ws, size_t ws_bytes,
+                                   lstc_stream s);
+
 /* Segment words of a graph prepared for split counting: for every entry v of
  * d_col[m] and of d_in_col[n_in] (n_in = d_in_off[n_rows]), the word
  * start | length << 32 of v's row (start = d_off[v], a column index below

@@ -131,7 +131,8 @@ def test_split_count_matches_oracle(lstc, oracle, name, builder):
 
 @pytest.mark.parametrize("cut_frac", [0.0, 0.3, 1.0])
 def test_in_lists_of_any_csr(lstc, oracle, cut_frac):
-    """lstc_in_lists on an uploaded CSR with n_owned rows: definition, and split = plain count."""
+    """lstc_in_lists_segments (and lstc_in_lists + lstc_segments) on an uploaded CSR with n_owned
+    rows: definition, segment words of every entry, and split = plain count."""
     _, og = oracle.clean_orient(*oracle.gen_rmat(13, 16, seed=12))
     cut = int(og.n * cut_frac)
     g = lstc.Graph.from_numpy(og.off, og.col, n_owned=cut)
@@ -150,6 +151,14 @@ def test_in_lists_of_any_csr(lstc, oracle, cut_frac):
         return (og.off[ids].astype(np.uint64) & np.uint64(0xFFFFFFFF)) | (d[ids].astype(np.uint64) << np.uint64(32))
     assert np.array_equal(g.col_seg[: og.m].cpu().numpy().view(np.uint64), words(og.col))
     assert np.array_equal(g.in_seg[: len(want_col)].cpu().numpy().view(np.uint64), words(want_col))
+    # the same arrays from lstc_in_lists alone followed by lstc_segments
+    g2 = lstc.Graph.from_numpy(og.off, og.col, n_owned=cut)
+    lstc.in_lists(g2, segment_words=False)
+    assert g2.col_seg is None and lstc.count(g2) == want
+    lstc.segments(g2)
+    for a, b in ((g.in_off, g2.in_off), (g.in_col[: len(want_col)], g2.in_col[: len(want_col)]),
+                 (g.col_seg[: og.m], g2.col_seg[: og.m]), (g.in_seg[: len(want_col)], g2.in_seg[: len(want_col)])):
+        assert torch.equal(a, b)
 
 
 def test_count_above_2_30_arcs(lstc):

@@ -127,6 +127,7 @@ SIGNATURES = {
     "lstc_triangle_support": (ST, [VP, VP, U32, U32, U64, VP, VP, VP]),
     "lstc_in_lists_workspace": (ST, [U32, U32, U64, PSZ]),
     "lstc_in_lists": (ST, [VP, VP, U32, U32, U64, VP, VP, PU64, VP, SZ, VP]),
+    "lstc_in_lists_segments": (ST, [VP, VP, U32, U32, U64, VP, VP, VP, VP, PU64, VP, SZ, VP]),
     "lstc_segments": (ST, [VP, VP, U32, U64, VP, U64, VP, VP, VP]),
     "lstc_count_batch_workspace": (ST, [C.POINTER(GraphDesc), U32, C.POINTER(Tune), PSZ]),
     "lstc_count_batch": (ST, [C.POINTER(GraphDesc), U32, VP, C.POINTER(Tune), VP, SZ, VP]),

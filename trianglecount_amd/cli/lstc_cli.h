@@ -69,9 +69,9 @@ struct Workspace {
     }
 };
 
-// Owned-source target-side in-lists of an uploaded CSR (lstc_in_lists) and the
-// segment words of col and the in-lists (lstc_segments), so the graph is
-// counted split; the caller frees the four arrays.
+// Owned-source target-side in-lists of an uploaded CSR and the segment words of
+// col and the in-lists, built together (lstc_in_lists_segments), so the graph
+// is counted split; the caller frees the four arrays.
 struct SplitLists {
     uint64_t *in_off = nullptr;
     uint32_t *in_col = nullptr;
@@ -89,14 +89,14 @@ inline SplitLists build_in_lists(const uint64_t *d_off, const uint32_t *d_col, u
     SplitLists sl;
     CLI_HIP(hipMalloc((void **)&sl.in_off, sizeof(uint64_t) * ((size_t)n_rows + 1)));
     CLI_HIP(hipMalloc((void **)&sl.in_col, sizeof(uint32_t) * (m ? m : 1)));
+    CLI_HIP(hipMalloc((void **)&sl.col_seg, sizeof(uint64_t) * (m ? m : 1)));
+    CLI_HIP(hipMalloc((void **)&sl.in_seg, sizeof(uint64_t) * (m ? m : 1)));
     size_t need = 0;
     CLI_CHECK(lstc_in_lists_workspace(n_rows, n_owned, m, &need));
     void *w = ws.get(need);
     uint64_t entries = 0;
-    CLI_CHECK(lstc_in_lists(d_off, d_col, n_rows, n_owned, m, sl.in_off, sl.in_col, &entries, w, ws.bytes, nullptr));
-    CLI_HIP(hipMalloc((void **)&sl.col_seg, sizeof(uint64_t) * (m ? m : 1)));
-    CLI_HIP(hipMalloc((void **)&sl.in_seg, sizeof(uint64_t) * (entries ? entries : 1)));
-    CLI_CHECK(lstc_segments(d_off, d_col, n_rows, m, sl.in_col, entries, sl.col_seg, sl.in_seg, nullptr));
+    CLI_CHECK(lstc_in_lists_segments(d_off, d_col, n_rows, n_owned, m, sl.in_off, sl.in_col, sl.col_seg, sl.in_seg,
+                                     &entries, w, ws.bytes, nullptr));
     CLI_HIP(hipDeviceSynchronize());
     return sl;
 }

@@ -245,8 +245,10 @@ def segments(g: Graph, stream=None) -> Graph:
     return g
 
 
-def in_lists(g: Graph, ws: Workspace | None = None, stream=None) -> Graph:
-    """Attach target-side in-lists (owned sources only, lstc_in_lists) so ``g`` is counted split."""
+def in_lists(g: Graph, ws: Workspace | None = None, stream=None, segment_words: bool = True) -> Graph:
+    """Attach target-side in-lists (owned sources only) so ``g`` is counted split: with their
+    segment words and those of col in the same passes (lstc_in_lists_segments), or the in-lists
+    alone (lstc_in_lists)."""
     L = load()
     no = g.owned
     nb = C.c_size_t()
@@ -255,11 +257,20 @@ def in_lists(g: Graph, ws: Workspace | None = None, stream=None) -> Graph:
     in_off = empty_u64(g.n + 1)
     in_col = empty_u32(max(g.m, 1))
     k = C.c_uint64()
-    check(L.lstc_in_lists(_ptr(g.off), _ptr(g.col), g.n, no, g.m, _ptr(in_off), _ptr(in_col), C.byref(k), wp, wb,
-                          C.c_void_p(stream_handle(stream))), "lstc_in_lists")
+    sh = C.c_void_p(stream_handle(stream))
+    if segment_words:
+        col_seg = empty_u64(max(g.m, 1))
+        in_seg = empty_u64(max(g.m, 1))
+        check(L.lstc_in_lists_segments(_ptr(g.off), _ptr(g.col), g.n, no, g.m, _ptr(in_off), _ptr(in_col),
+                                       _ptr(col_seg), _ptr(in_seg), C.byref(k), wp, wb, sh), "lstc_in_lists_segments")
+        g.col_seg, g.in_seg = col_seg, in_seg[: max(k.value, 1)]
+    else:
+        check(L.lstc_in_lists(_ptr(g.off), _ptr(g.col), g.n, no, g.m, _ptr(in_off), _ptr(in_col), C.byref(k), wp,
+                              wb, sh), "lstc_in_lists")
+        g.col_seg = g.in_seg = None
     g.in_off, g.in_col, g.in_owned = in_off, in_col, no
     g.meta["in_entries"] = k.value
-    return segments(g, stream=stream)
+    return g
 
 
 # ------------------------------------------------------------------ (c) count

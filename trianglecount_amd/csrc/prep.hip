@@ -22,6 +22,7 @@
 //            16-byte record per endpoint and counts the in-list arcs
 //   in     : reverse keys of kept arcs with d+(d) <= d+(s) -> target-side
 //            in-lists and their segment words
+#include "lookback.h"
 #include "lstc_internal.h"
 #include "wave.h"
 
@@ -283,15 +284,6 @@ struct OrientInPass {
     uint64_t *in_total;
     uint32_t *flag;
 };
-static constexpr uint64_t kTsAgg = 1ull << 62, kTsIncl = 2ull << 62, kTsCount = (1ull << 62) - 1;
-static constexpr uint32_t kLookBackPolls = 1u << 20;
-
-__device__ __forceinline__ uint64_t status_load(const uint64_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 __global__ __launch_bounds__(kCT) void k_orient_in_scatter(OrientInPass op, const uint64_t *__restrict__ tile_off) {
     __shared__ uint32_t s_w[2][kCT / 64];        // oriented arcs per wave of a round (two rounds in flight)
@@ -363,33 +355,7 @@ __global__ __launch_bounds__(kCT) void k_orient_in_scatter(OrientInPass op, cons
         const uint32_t incl = wave_incl_scan(v, lane);
         s_in[lane] = incl - v;
         const uint64_t c = read_lane(incl, 63);
-        uint64_t excl = 0;
-        if (tile == 0) {
-            if (lane == 0) status_store(op.status, kTsIncl | c);
-        } else {
-            if (lane == 0) status_store(op.status + tile, kTsAgg | c);
-            int64_t t = (int64_t)tile - 1;  // lane j examines tile t - j
-            uint32_t polls = 0;
-            while (true) {
-                const int64_t tj = t - (int64_t)lane;
-                const uint64_t w = tj >= 0 ? status_load(op.status + tj) : kTsIncl;  // before tile 0: prefix 0
-                const uint64_t mi = ballot((w >> 62) == 2), mz = ballot((w >> 62) == 0);
-                const uint64_t first = mi & (0ull - mi);           // nearest tile with its inclusive prefix
-                const uint64_t below = first ? first - 1 : ~0ull;  // the tiles between it and this one
-                if (mz & below) {                                  // one of them has not published yet
-                    if (++polls > kLookBackPolls) {
-                        if (lane == 0) atomicOr(op.flag, 2u);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                excl += wave_sum64(lane_in(below | first) ? (w & kTsCount) : 0ull);
-                if (first) break;
-                t -= 64;
-            }
-            if (lane == 0) status_store(op.status + tile, kTsIncl | (excl + c));
-        }
+        const uint64_t excl = tile_lookback(op.status, tile, c, op.flag, lane);
         if (lane == 0) {
             s_excl = excl;
             if (tile0 + kCTile >= n) *op.in_total = excl + c;  // the last tile holds the total
