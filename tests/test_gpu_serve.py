@@ -66,6 +66,9 @@ def test_over_budget_rejected(lstc, oracle):
             srv.submit_rmat(20, 16 << 20, 1)
         assert e.value.status == -3  # LSTC_ERR_OVER_BUDGET
         assert srv.stats()["rejected"] == 1
+        with pytest.raises(lstc.LstcError) as e:  # invalid quadrant probabilities: refused, the server runs on
+            srv.submit_rmat(10, 1000, 1, a=0.6, b=0.3, c=0.3)
+        assert e.value.status == -1  # LSTC_ERR_INVALID
         rid = srv.submit_er(500, 3000, 7)  # a small one still runs
         assert srv.wait(rid, 60)["count"] == _oracle_count(oracle, "er", 500, 3000, 7)[0]
     finally:

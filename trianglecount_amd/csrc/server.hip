@@ -725,6 +725,8 @@ extern "C" lstc_status lstc_server_submit(lstc_server *S, const lstc_request *re
         return fail(LSTC_ERR_INVALID, "lstc_server_submit: unknown request kind %u", r.kind);
     if (r.kind == LSTC_REQ_RMAT && (r.scale == 0 || r.scale > 31))
         return fail(LSTC_ERR_INVALID, "lstc_server_submit: RMAT scale %u not in 1..31", r.scale);
+    if (r.kind == LSTC_REQ_RMAT && !(r.a >= 0 && r.b >= 0 && r.c >= 0 && r.a + r.b + r.c <= 1.0))
+        return fail(LSTC_ERR_INVALID, "lstc_server_submit: RMAT probabilities a, b, c must be >= 0 with a+b+c <= 1");
     if (r.kind == LSTC_REQ_ER && (r.n == 0 || r.n == 0xFFFFFFFFu))
         return fail(LSTC_ERR_INVALID, "lstc_server_submit: bad ER vertex count");
     if (r.kind == LSTC_REQ_CSR) {
