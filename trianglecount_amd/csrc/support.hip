@@ -8,9 +8,11 @@
 // a streamed list that the hashed list also holds, and adds one to the support
 // of its three arcs and to the count of its three vertices.
 //
-// A work item hashes one row r (in tiles of kSupTile keys) into a wave-private
-// LDS table of 4-slot buckets and streams candidate lists through it with the
-// count engine's flat segment walk (segwalk.h): 64 lists per window, rounds of
+// A work item hashes one row r (in tiles of at most half its table's slots)
+// into an LDS table of 4-slot buckets -- a wave's slice of the workgroup's
+// table, or for rows longer than a slice holds the whole table shared by the
+// workgroup's waves -- and streams candidate lists through it with the count
+// engine's flat segment walk (segwalk.h): 64 lists per window, rounds of
 // kSupUnroll chunks of 64 entries, a round inside one list addressed from a
 // scalar base, the next round's loads issued before this one is probed.  A hit
 // (entry x of the list of candidate c) is attributed three ways:
