@@ -328,7 +328,10 @@ static uint32_t request_rows(const lstc_request &r) {
 static lstc_status admit(lstc_server *S) {
     while (!S->pending.empty()) {
         Req *r = S->pending.front();
+        const bool csr = r->spec.kind == LSTC_REQ_CSR;
         {
+            // a cancel is honoured exactly when it lands before the request leaves
+            // `queued`, which happens under the same lock as this check
             std::lock_guard<std::mutex> lk(S->mu);
             if (S->cancelled.erase(r->id)) {
                 S->queued.erase(r->id);
@@ -340,12 +343,9 @@ static lstc_status admit(lstc_server *S) {
                 S->cv_done.notify_all();
                 continue;
             }
+            if (csr) S->queued.erase(r->id);
         }
-        if (r->spec.kind == LSTC_REQ_CSR) {  // caller-resident CSR: nothing to prepare
-            {
-                std::lock_guard<std::mutex> lk(S->mu);
-                S->queued.erase(r->id);
-            }
+        if (csr) {  // caller-resident CSR: nothing to prepare
             r->off = (uint64_t *)r->spec.off;
             r->col = (uint32_t *)r->spec.col;
             r->n_rows = r->spec.n_rows;
@@ -397,16 +397,38 @@ static lstc_status admit(lstc_server *S) {
             }
         }
         if (!t) break;
-        PrepGroup *grp = new PrepGroup();
+        // commit: the requests leave `queued` (a later cancel is refused) unless
+        // a cancel arrived meanwhile, where the group stops before that request
+        uint32_t keep = G;
         {
             std::lock_guard<std::mutex> lk(S->mu);
-            for (uint32_t g = 0; g < G; ++g) {  // from here on these requests run to their results
-                S->queued.erase(S->pending[g]->id);
-                S->cancelled.erase(S->pending[g]->id);
+            for (uint32_t g = 0; g < G; ++g)
+                if (S->cancelled.count(S->pending[g]->id)) {
+                    keep = g;
+                    break;
+                }
+            for (uint32_t g = 0; g < keep; ++g) S->queued.erase(S->pending[g]->id);
+            if (keep) {
+                S->n_groups++;
+                S->max_group_seen = std::max<uint64_t>(S->max_group_seen, keep);
             }
-            S->n_groups++;
-            S->max_group_seen = std::max<uint64_t>(S->max_group_seen, G);
         }
+        if (keep < G) {
+            for (uint32_t g = keep; g < G; ++g) S->pool.release(res[g]);
+            res.resize(keep);
+            if (keep == 0) {  // the head was cancelled: the top of the loop drops it
+                S->pool.release(t);
+                continue;
+            }
+            // the transient block and workspace size stay those of the larger group (they cover the smaller one)
+            G = keep;
+            N = M = 0;
+            for (uint32_t g = 0; g < G; ++g) {
+                N += request_rows(S->pending[g]->spec);
+                M += S->pending[g]->spec.m_raw;
+            }
+        }
+        PrepGroup *grp = new PrepGroup();
         for (uint32_t g = 0; g < G; ++g) {
             Req *q = S->pending.front();
             S->pending.pop_front();
