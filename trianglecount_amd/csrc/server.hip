@@ -32,6 +32,7 @@
 // host flags, so every kernel runs to completion on its own and a lost or
 // stalled host never leaves a spinning kernel on the GPU.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -99,13 +100,14 @@ class SlabPool {
         free_[start] = bytes;
     }
     size_t total() const { return total_; }
-    size_t used() const { return used_; }
+    size_t used() const { return used_.load(std::memory_order_relaxed); }  // read by other threads (stats)
 
    private:
     char *base_ = nullptr;
     std::map<size_t, size_t> free_;
     std::unordered_map<size_t, size_t> sizes_;
-    size_t total_ = 0, used_ = 0;
+    size_t total_ = 0;
+    std::atomic<size_t> used_{0};
 };
 
 struct Req {
