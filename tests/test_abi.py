@@ -110,6 +110,36 @@ def test_batch_and_in_lists_reject_bad_shapes_on_host():
     assert lib.lstc_in_lists(_FAKE, _FAKE, 10, 11, 20, _FAKE, _FAKE, ctypes.byref(k), _FAKE, ctypes.c_size_t(1 << 20),
                              None) == _lib.LSTC_ERR_INVALID
     assert lib.lstc_in_lists_workspace(10, 10, 20, ctypes.byref(nb)) == _lib.LSTC_OK and nb.value > 0
+    # the fused build: the same checks, plus the segment words' 32-bit starts and both arrays
+    big_ws = ctypes.c_size_t(1 << 20)
+    assert lib.lstc_in_lists_segments(_FAKE, _FAKE, 10, 11, 20, _FAKE, _FAKE, _FAKE, _FAKE, ctypes.byref(k), _FAKE,
+                                      big_ws, None) == _lib.LSTC_ERR_INVALID
+    assert lib.lstc_in_lists_segments(_FAKE, _FAKE, 10, 10, 0xFFFFF000, _FAKE, _FAKE, _FAKE, _FAKE, ctypes.byref(k),
+                                      _FAKE, big_ws, None) == _lib.LSTC_ERR_INVALID
+    assert lib.lstc_in_lists_segments(_FAKE, _FAKE, 10, 10, 20, _FAKE, _FAKE, _FAKE, None, ctypes.byref(k), _FAKE,
+                                      big_ws, None) == _lib.LSTC_ERR_INVALID
+    assert "segment" in lib.lstc_last_error().decode()
+    assert lib.lstc_in_lists_segments(_FAKE, _FAKE, 10, 10, 20, _FAKE, _FAKE, _FAKE, _FAKE, ctypes.byref(k), _FAKE,
+                                      ctypes.c_size_t(16), None) == _lib.LSTC_ERR_WORKSPACE
+
+
+def test_support_rejects_bad_shapes_on_host():
+    lib = _lib.load()
+    ws = ctypes.c_size_t(1 << 30)
+    for args in ((10, 11, 20, _FAKE, _FAKE),             # n_owned > n_rows
+                 (10, 10, 20, None, None),               # no output
+                 (10, 10, 0xFFFFF000, _FAKE, None)):     # m beyond the walk's 32-bit positions
+        n, no, m, sup, pv = args
+        assert lib.lstc_triangle_support(_FAKE, _FAKE, n, no, m, sup, pv, None) == _lib.LSTC_ERR_INVALID, args
+        assert lib.lstc_triangle_support_split(_FAKE, _FAKE, _FAKE, _FAKE, n, no, m, sup, pv, _FAKE, ws,
+                                               None) == _lib.LSTC_ERR_INVALID, args
+    # the split walk needs the in-lists and its workspace
+    assert lib.lstc_triangle_support_split(_FAKE, _FAKE, None, None, 10, 10, 20, _FAKE, None, _FAKE, ws,
+                                           None) == _lib.LSTC_ERR_INVALID
+    nb = ctypes.c_size_t()
+    assert lib.lstc_triangle_support_split_workspace(10, 20, ctypes.byref(nb)) == _lib.LSTC_OK and nb.value > 0
+    assert lib.lstc_triangle_support_split(_FAKE, _FAKE, _FAKE, _FAKE, 10, 10, 20, _FAKE, None, _FAKE,
+                                           ctypes.c_size_t(nb.value - 1), None) == _lib.LSTC_ERR_WORKSPACE
 
 
 def _descs(sizes):
