@@ -262,14 +262,18 @@ lstc_status lstc_in_lists(const uint64_t *d_off, const uint32_t *d_col, uint32_t
                           uint64_t m, uint64_t *d_in_off, uint32_t *d_in_col, uint64_t *n_entries, void *d_ws,
                           size_t ws_bytes, lstc_stream s);
 
-/* lstc_in_lists and lstc_segments in one call: the in-lists plus the segment
- * words of every entry of d_col[m] and d_in_col (d_col_seg[m], d_in_seg[m]; the
- * in-lists hold at most m entries), built in the same passes over the arcs
- * (the in-list pass writes the column words, the in_col pass the in-list
- * words), so a bare oriented CSR -- the reference's input,
- * src/WithoutPartition/ShareKernelTCV62_update_v1.cu:207-230 -- is made ready
- * for lstc_count_split with one gather of row offsets per entry.  m < 2^32 -
- * 4096.  Workspace: lstc_in_lists_workspace.  The stream is synchronised.     */
+/* lstc_in_lists and the segment words the split count reads, in one call and
+ * the same passes over the arcs (the in-list pass writes the column words, the
+ * in_col pass the in-list words), so a bare oriented CSR -- the reference's
+ * input, src/WithoutPartition/ShareKernelTCV62_update_v1.cu:207-230 -- is made
+ * ready for lstc_count_split with one gather of row offsets per entry.  Words
+ * are written for every in-list entry (d_in_seg) and for the d_col entries of
+ * owned rows (d_col_seg; rows < n_owned come first, so these are positions
+ * [0, d_off[n_owned]): every entry for a whole graph; the count reads no other
+ * column word).  In-list entries are arcs of owned rows, so d_in_col, d_col_seg
+ * and d_in_seg need room for d_off[n_owned] entries (m for a whole graph).
+ * m < 2^32 - 4096.  Workspace: lstc_in_lists_workspace.  The stream is
+ * synchronised.                                                              */
 lstc_status lstc_in_lists_segments(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows, uint32_t n_owned,
                                    uint64_t m, uint64_t *d_in_off, uint32_t *d_in_col, uint64_t *d_col_seg,
                                    uint64_t *d_in_seg, uint64_t *n_entries, void *d_ws, size_t ws_bytes,

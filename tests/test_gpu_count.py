@@ -149,16 +149,20 @@ def test_in_lists_of_any_csr(lstc, oracle, cut_frac):
     def words(ids):
         ids = ids.astype(np.int64)
         return (og.off[ids].astype(np.uint64) & np.uint64(0xFFFFFFFF)) | (d[ids].astype(np.uint64) << np.uint64(32))
-    assert np.array_equal(g.col_seg[: og.m].cpu().numpy().view(np.uint64), words(og.col))
+    # the fused build writes the column words of owned rows (the ones the count reads)
+    owned_arcs = int(og.off[cut])
+    assert np.array_equal(g.col_seg[:owned_arcs].cpu().numpy().view(np.uint64), words(og.col[:owned_arcs]))
     assert np.array_equal(g.in_seg[: len(want_col)].cpu().numpy().view(np.uint64), words(want_col))
-    # the same arrays from lstc_in_lists alone followed by lstc_segments
+    # lstc_in_lists alone followed by lstc_segments: the same in-lists, words of every entry
     g2 = lstc.Graph.from_numpy(og.off, og.col, n_owned=cut)
     lstc.in_lists(g2, segment_words=False)
     assert g2.col_seg is None and lstc.count(g2) == want
     lstc.segments(g2)
+    assert np.array_equal(g2.col_seg[: og.m].cpu().numpy().view(np.uint64), words(og.col))
     for a, b in ((g.in_off, g2.in_off), (g.in_col[: len(want_col)], g2.in_col[: len(want_col)]),
-                 (g.col_seg[: og.m], g2.col_seg[: og.m]), (g.in_seg[: len(want_col)], g2.in_seg[: len(want_col)])):
+                 (g.col_seg[:owned_arcs], g2.col_seg[:owned_arcs]), (g.in_seg[: len(want_col)], g2.in_seg[: len(want_col)])):
         assert torch.equal(a, b)
+    assert lstc.count(g2) == want
 
 
 def test_count_above_2_30_arcs(lstc):

@@ -254,13 +254,16 @@ def in_lists(g: Graph, ws: Workspace | None = None, stream=None, segment_words: 
     nb = C.c_size_t()
     check(L.lstc_in_lists_workspace(g.n, no, g.m, C.byref(nb)), "lstc_in_lists_workspace")
     wp, wb = (ws or _default_ws).get(nb.value)
+    # in-list entries are arcs of owned rows, which come first: off[n_owned] of them at most (m for a
+    # whole graph), and the column words the count reads are theirs too
+    mo = g.m if no == g.n else int(g.off[no].item())
     in_off = empty_u64(g.n + 1)
-    in_col = empty_u32(max(g.m, 1))
+    in_col = empty_u32(max(mo, 1))
     k = C.c_uint64()
     sh = C.c_void_p(stream_handle(stream))
     if segment_words:
-        col_seg = empty_u64(max(g.m, 1))
-        in_seg = empty_u64(max(g.m, 1))
+        col_seg = empty_u64(max(mo, 1))
+        in_seg = empty_u64(max(mo, 1))
         check(L.lstc_in_lists_segments(_ptr(g.off), _ptr(g.col), g.n, no, g.m, _ptr(in_off), _ptr(in_col),
                                        _ptr(col_seg), _ptr(in_seg), C.byref(k), wp, wb, sh), "lstc_in_lists_segments")
         g.col_seg, g.in_seg = col_seg, in_seg[: max(k.value, 1)]

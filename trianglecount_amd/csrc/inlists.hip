@@ -21,12 +21,11 @@
 namespace lstc {
 
 // The kept arcs (u -> w with u < n_owned, w a row and d+(u) <= d+(w)) of a
-// block's kInlRows consecutive rows of [0, n_walk) (pass 0: every row when it
-// writes segment words; pass 1: the owned rows), walked flat in row order,
+// block's kInlRows consecutive rows of [0, n_walk) (the owned rows), walked flat in row order,
 // 256 arcs at a time: each arc finds its row by binary search of the rows'
 // offsets in LDS, so loads of col are coalesced however long the rows are.
-// Pass 0 counts the kept arcs per row and, with SEG, writes every arc's segment
-// word (the pass gathers the target's row offsets anyway); pass 1 writes their
+// Pass 0 counts the kept arcs per row and, with SEG, writes the segment word of
+// every arc it walks (it gathers the target's row offsets anyway); pass 1 writes their
 // keys u << 32 | w in arc order from pos[first row of the block] on (the kept
 // arcs of consecutive rows are consecutive), with SEG taking each target's
 // length from the word pass 0 wrote (coalesced) instead of gathering it again.
@@ -169,7 +168,8 @@ static size_t inl_ws(uint32_t n_rows, uint64_t m, size_t *parts) {
 }
 
 // In-lists, and with col_seg / in_seg (both or neither) the segment words of
-// col and in_col in the same passes.
+// the col entries of owned rows (the ones the split count reads) and of every
+// in-list entry, in the same passes.
 static lstc_status in_lists_impl(const char *who, const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows,
                                  uint32_t n_owned, uint64_t m, uint64_t *d_in_off, uint32_t *d_in_col,
                                  uint64_t *d_col_seg, uint64_t *d_in_seg, uint64_t *n_entries, void *d_ws,
@@ -188,7 +188,7 @@ static lstc_status in_lists_impl(const char *who, const uint64_t *d_off, const u
     char *tail = (char *)d_ws + head;
     const size_t tail_bytes = ws_bytes - head;
     const bool seg = d_col_seg != nullptr;
-    const uint32_t n_walk = seg ? n_rows : n_owned;  // segment words cover every row's arcs
+    const uint32_t n_walk = n_owned;  // the kept arcs and the segment words the count reads: owned rows
     const uint32_t grid_rows =
         (uint32_t)hmin<uint64_t>(((uint64_t)n_walk + kInlRows) / kInlRows, (uint64_t)device_cus() * 16);
     if (seg)
