@@ -271,6 +271,14 @@ __global__ __launch_bounds__(256) void k_hp_rows(const uint64_t *__restrict__ of
                     pd = x.part;
                 }
             }
+            // with at most 8 parts, the target's local ids in all its parts (side by side in
+            // lids) are loaded together here rather than one after another in the part loop
+            uint32_t l8[8];
+            if (FILL && k <= 8) {
+                const uint32_t np = (uint32_t)__popcll(md ? dmem : 0ull);
+#pragma unroll
+                for (uint32_t r = 0; r < 8; ++r) l8[r] = r < np ? lids[dbase + r] : 0u;
+            }
             for (unsigned long long rest = ms; rest;) {
                 const uint32_t p = (uint32_t)__builtin_ctzll(rest);
                 rest &= rest - 1;
@@ -287,7 +295,15 @@ __global__ __launch_bounds__(256) void k_hp_rows(const uint64_t *__restrict__ of
                         const uint64_t bp = read_lane64(posp, p), bh = read_lane64(posh, p);
                         uint32_t *oc = (uint32_t *)read_lane64((uint64_t)ocol, p);
                         if (in) {
-                            const uint32_t ld = lids[dbase + lid_rank(dmem, p)];
+                            const uint32_t rk = lid_rank(dmem, p);
+                            uint32_t ld;
+                            if (k <= 8) {
+                                ld = l8[0];
+#pragma unroll
+                                for (uint32_t r = 1; r < 8; ++r) ld = rk == r ? l8[r] : ld;
+                            } else {
+                                ld = lids[dbase + rk];
+                            }
                             if (inp) oc[bp + mask_rank(mP)] = ld;
                             else oc[bh + mask_rank(mH)] = ld;
                         }
