@@ -177,3 +177,24 @@ def test_grouped_preparation_exact(lstc, oracle):
             assert st["max_prep_graphs"] > 1 and st["prep_groups"] < len(reqs), st
         finally:
             srv.close()
+
+
+def test_grouped_preparation_degenerate_requests(lstc, oracle):
+    """Requests with no raw edges, only loops or a single edge, prepared in the same groups as
+    ordinary ones (a burst): every count and edge total equals the oracle's."""
+    odd = [("rmat", 1, 0, 5), ("er", 1, 5, 6), ("rmat", 3, 1, 7), ("er", 2, 0, 8), ("rmat", 2, 3, 9)]
+    rng = np.random.default_rng(31)
+    reqs = []
+    for i, r in enumerate(_req_list(rng, 40, 1e3, 2e4)):
+        reqs.append(r)
+        reqs.append(odd[i % len(odd)])
+    want = [_oracle_count(oracle, *r) for r in reqs]
+    srv = lstc.Server(1 << 30)
+    try:
+        ids = [_submit(srv, r) for r in reqs]
+        got = [srv.wait(x, 120) for x in ids]
+        assert [(r["count"], r["edges"]) for r in got] == want
+        st = srv.stats()
+        assert st["completed"] == len(reqs) and st["pool_used_bytes"] == 0 and st["max_prep_graphs"] > 1, st
+    finally:
+        srv.close()
