@@ -1,5 +1,6 @@
-"""CPU model of the count kernel's LDS hash tables: every insert and every probe
-chain ends, and membership is exact.
+"""CPU model of the LDS hash tables of the count and support kernels: every
+insert and every probe chain ends, membership is exact, and a key's slot word
+stays with it.
 
 The kernel (trianglecount_amd/csrc/count_engine.hip: table_insert, table_mark,
 probe_round, drain_queue) keeps a row's keys in 2^tl slots grouped into
@@ -20,8 +21,13 @@ taken is full.  Tables are filled to at most half of their slots, so at most
 half of the buckets are full and every lap passes a bucket with an empty last
 slot.  The model checks that by simulation over random interleavings of the
 individual compare-and-swap steps (any schedule of the wave's atomics is one of
-them), at every table size the kernel uses, loads up to 1/2, and adversarial
+them), at every table size the kernels use, loads up to 1/2, and adversarial
 keys (all keys on one home at the end of the table, so chains wrap around).
+
+The support kernels (trianglecount_amd/csrc/support.hip: sup_build, sup_probe)
+use the same buckets, marks and probe walk with tables from 16 slots
+(sup_tlog2) up to the workgroup's 2^12, and give every slot a word holding the
+key's tile index; their marking swaps the words of slots 0 and 1 with the keys.
 """
 import random
 
@@ -34,10 +40,19 @@ EMPTY = 0xFFFFFFFF
 NO_ID = 0xFFFFFFFE  # the id that past-the-end positions probe
 SLICE_LOG2 = 10
 BIG_LOG2 = {2: 11, 4: 12, 8: 13}  # workgroup table of 1024 slots per wave, by waves per workgroup
+SUP_GROUP_LOG2 = 12  # support.hip kSupGroupLog2: the workgroup table; waves take 1/4 or 1/8 of it
 
 
 def hquad(x, tl):
     return ((x * PHI) & 0xFFFFFFFF) >> (34 - tl)
+
+
+def sup_tlog2(t):
+    """support.hip sup_tlog2: smallest l >= 4 with 2^l >= 2 * t."""
+    l = 4
+    while (1 << l) < 2 * t:
+        l += 1
+    return l
 
 
 def table_log2_for(keys, scale, max_log2):
@@ -54,6 +69,7 @@ class Table:
         self.nb = 1 << (tl - 2)
         self.qmask = self.nb - 1
         self.slot = [EMPTY] * (4 * self.nb)
+        self.word = [None] * (4 * self.nb)  # the support kernels' slot word: the key's tile index
         self.ovf = [False] * self.nb
 
     def insert_all(self, keys, threads, rng):
@@ -61,19 +77,20 @@ class Table:
 
         Each inserter follows table_insert: for step in 0..qmask, try slots 0..3 of bucket h in
         order, then h+1.  Returns the largest number of buckets any insert visited."""
-        pending = list(keys)
-        live = []  # [key, home, h, slot, buckets_visited]
+        pending = list(enumerate(keys))
+        live = []  # [key, home, h, slot, buckets_visited, tile index]
         worst = 0
         while pending or live:
             while pending and len(live) < threads:
-                k = pending.pop()
+                idx, k = pending.pop()
                 home = hquad(k, self.tl)
-                live.append([k, home, home, 0, 1])
+                live.append([k, home, home, 0, 1, idx])
             i = rng.randrange(len(live))
             st = live[i]
-            k, home, h, s, visited = st
+            k, home, h, s, visited, idx = st
             if self.slot[4 * h + s] == EMPTY:  # the CAS succeeds
                 self.slot[4 * h + s] = k
+                self.word[4 * h + s] = idx
                 if h != home:
                     self.ovf[home] = True
                 worst = max(worst, visited)
@@ -89,27 +106,29 @@ class Table:
                 st[4] = visited + 1
         return worst
 
-    def mark(self):
+    def mark(self, words=True):
         for b in range(self.nb):
             x, y = self.slot[4 * b], self.slot[4 * b + 1]
             if (x < y) if self.ovf[b] else (x > y):
                 self.slot[4 * b], self.slot[4 * b + 1] = y, x
+                if words:
+                    self.word[4 * b], self.word[4 * b + 1] = self.word[4 * b + 1], self.word[4 * b]
 
     def probe(self, x):
-        """(found, buckets read) as probe_round + drain_queue compute them."""
+        """(found, buckets read, slot) as probe_round + drain_queue (sup_probe) compute them."""
         home = hquad(x, self.tl)
         v = self.slot[4 * home: 4 * home + 4]
         if x in v:
-            return True, 1
+            return True, 1, 4 * home + v.index(x)
         if not v[0] > v[1]:  # not marked overflowed: the chain ends at home
-            return False, 1
+            return False, 1, None
         q = (home + 1) & self.qmask
         for step in range(self.qmask):  # drain_queue's lap bound
             w = self.slot[4 * q: 4 * q + 4]
             if x in w:
-                return True, step + 2
+                return True, step + 2, 4 * q + w.index(x)
             if w[3] == EMPTY:
-                return False, step + 2
+                return False, step + 2, None
             q = (q + 1) & self.qmask
         raise AssertionError(f"probe chain of {x:#x} did not end within one lap")
 
@@ -145,8 +164,9 @@ def check_table(tl, keys, threads, rng):
     assert sum(full) <= t.nb // 2
     t.mark()
     keyset = set(keys)
-    for k in keys:
-        assert t.probe(k)[0]
+    for i, k in enumerate(keys):
+        found, _, slot = t.probe(k)
+        assert found and t.word[slot] == i
     absent = [NO_ID] + [x for x in rng.sample(range(NO_ID), 200) if x not in keyset]
     absent += [x for x in keys_with_home(hquad(keys[0], tl), tl, 16, rng) if x not in keyset]
     for x in absent:
@@ -154,7 +174,7 @@ def check_table(tl, keys, threads, rng):
     return worst_insert
 
 
-@pytest.mark.parametrize("tl", [6, 7, 8, 9, 10, 11, 12, 13])
+@pytest.mark.parametrize("tl", [4, 5, 6, 7, 8, 9, 10, 11, 12, 13])
 def test_chains_end_and_membership_exact(tl):
     rng = random.Random(tl)
     slots = 1 << tl
@@ -168,7 +188,7 @@ def test_chains_end_and_membership_exact(tl):
 def test_full_home_chain_wraps_around():
     """Half the slots, every key on the last bucket: the chain runs over the end of the table."""
     rng = random.Random(7)
-    for tl in (6, 10, 13):
+    for tl in (4, 6, 10, 13):
         nb = 1 << (tl - 2)
         keys = keys_with_home(nb - 1, tl, 2 * nb, rng)
         worst = check_table(tl, keys, 64, rng)
@@ -191,10 +211,20 @@ def test_kernel_loads_stay_at_or_below_half():
                 assert 2 * tk <= (1 << tl), (scale, waves, tk, tl)
 
 
+def test_support_loads_stay_at_or_below_half():
+    """Every support table is at most half full: tiles of up to half the table the kernel gives
+    them (the workgroup's 2^12 slots, or a wave's quarter or eighth of it), at least 16 slots."""
+    for table_log2 in (SUP_GROUP_LOG2, SUP_GROUP_LOG2 - 2, SUP_GROUP_LOG2 - 3):
+        tile = (1 << table_log2) // 2
+        for t in range(1, tile + 1):
+            tl = sup_tlog2(t)
+            assert 4 <= tl <= table_log2 and 2 * t <= (1 << tl), (table_log2, t, tl)
+
+
 def test_hash_matches_kernel_arithmetic():
     """hquad is the high bits of the 32-bit product (numpy uint32 wrap-around as on the device)."""
     x = np.array([0, 1, 2, 12345, 0xFFFFFFFD, NO_ID], np.uint32)
-    for tl in (6, 10, 12):
+    for tl in (4, 6, 10, 12):
         dev = ((x * np.uint32(PHI)).astype(np.uint32) >> np.uint32(34 - tl)).tolist()
         assert dev == [hquad(int(v), tl) for v in x]
         assert max(dev) < (1 << (tl - 2))
@@ -214,3 +244,18 @@ def test_model_catches_a_planted_mistake():
     assert len(missed) == 20
     t.mark()
     assert all(t.probe(k)[0] for k in keys)
+
+
+def test_model_catches_unswapped_words():
+    """With the marks swapping keys but not their words, every bucket whose first two keys the
+    marking reorders reports each other's tile index for them: the word check has to see it."""
+    rng = random.Random(5)
+    tl = 6
+    keys = keys_with_home(1, tl, 12, rng)  # 12 keys on one home: it overflows into the next ones
+    t = Table(tl)
+    t.insert_all(keys, 64, rng)
+    before = list(t.slot)
+    t.mark(words=False)
+    swaps = sum(t.slot[4 * b] != before[4 * b] for b in range(t.nb))
+    wrong = [k for i, k in enumerate(keys) if t.word[t.probe(k)[2]] != i]
+    assert swaps >= 1 and len(wrong) == 2 * swaps
