@@ -96,7 +96,16 @@ lstc_status scan_exclusive_u32_to_u64(const uint32_t *in, uint64_t *out, uint64_
 // entries; the result ends in *result (either keys or alt).
 size_t radix_sort_ws_bytes(uint64_t n);
 lstc_status radix_sort_u64(uint64_t *keys, uint64_t *alt, uint64_t n, int key_bits,
-                           uint64_t **result, void *ws, size_t ws_bytes, hipStream_t s);
+                           uint64_t **result, void *ws, size_t ws_bytes, hipStream_t s,
+                           bool first_counts_ready = false);
+// The sort's passes work on blocks of kRadixBlock consecutive keys.  A kernel
+// that writes the keys block by block can count the first digit (the low 8
+// bits) itself: the count of digit d in block t goes to
+// radix_first_counts(ws, n)[d * radix_blocks(n) + t], and the sort is then
+// called with first_counts_ready, which skips its first counting pass.
+constexpr uint64_t kRadixBlock = 16384;
+inline uint64_t radix_blocks(uint64_t n) { return (n + kRadixBlock - 1) / kRadixBlock; }
+uint32_t *radix_first_counts(void *ws, uint64_t n);
 
 // (a) clean + orient without synchronising (prep.hip); sizes stay on the device:
 // (*d_sizes)[1] = vertices after cleaning, (*d_sizes)[2] = undirected edges,

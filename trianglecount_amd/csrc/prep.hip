@@ -6,7 +6,8 @@
 //
 // Pipeline (all stream-ordered, sizes kept on the device until the end):
 //   keys  : each raw pair (s, d), s != d, emits (s<<b | d) and (d<<b | s);
-//           loops and out-of-range ids emit a sentinel that sorts last
+//           loops and out-of-range ids emit a sentinel that sorts last;
+//           the same kernel counts the sort's first digit per sort block
 //   sort  : LSD radix sort over 2b bits (prims.hip)
 //   then compactions of the sorted list, each a count (per tile of 4096 keys),
 //   a scan of the tile counts and a scatter that recomputes the predicate and
@@ -39,21 +40,49 @@ static int bits_for(uint64_t n) {  // bits to represent values < n
 #define GRID_STRIDE(i, n) \
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
 
-__global__ void k_make_keys(const uint32_t *__restrict__ src, const uint32_t *__restrict__ dst,
-                            uint64_t m_raw, uint32_t n, uint32_t b, uint64_t *__restrict__ keys,
-                            uint32_t *__restrict__ bad) {
-    GRID_STRIDE(i, m_raw) {
-        const uint32_t s = src[i], d = dst[i];
-        uint64_t k0 = kSentinel, k1 = kSentinel;
-        if (s >= n || d >= n) {
-            *bad = 1;
-        } else if (s != d) {
-            k0 = ((uint64_t)s << b) | d;
-            k1 = ((uint64_t)d << b) | s;
+// Keys of both directions of every raw pair (pair i gives keys 2i and 2i + 1;
+// loops and out-of-range ids the sentinel), one block of the radix sort per
+// workgroup: the workgroup also counts its block's first digit (the low 8
+// bits) into counts[d * blocks + blockIdx.x], the sort's first histogram.
+static constexpr uint32_t kKeyPairs = (uint32_t)(kRadixBlock / 2);  // raw pairs per workgroup
+static constexpr uint32_t kKeyBatch = 8;                             // rounds whose loads go out together
+
+__global__ __launch_bounds__(256) void k_make_keys(const uint32_t *__restrict__ src, const uint32_t *__restrict__ dst,
+                                                   uint64_t m_raw, uint32_t n, uint32_t b, uint64_t *__restrict__ keys,
+                                                   uint32_t blocks, uint32_t *__restrict__ counts,
+                                                   uint32_t *__restrict__ bad) {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t p0 = (uint64_t)blockIdx.x * kKeyPairs;
+    bool out_of_range = false;
+    for (uint32_t r0 = 0; r0 < kKeyPairs / 256; r0 += kKeyBatch) {
+        uint32_t s[kKeyBatch], d[kKeyBatch];
+#pragma unroll
+        for (uint32_t r = 0; r < kKeyBatch; ++r) {
+            const uint64_t i = p0 + (uint64_t)(r0 + r) * 256 + threadIdx.x;
+            s[r] = i < m_raw ? src[i] : 0;
+            d[r] = i < m_raw ? dst[i] : 0;
         }
-        keys[2 * i] = k0;
-        keys[2 * i + 1] = k1;
+#pragma unroll
+        for (uint32_t r = 0; r < kKeyBatch; ++r) {
+            const uint64_t i = p0 + (uint64_t)(r0 + r) * 256 + threadIdx.x;
+            if (i >= m_raw) break;
+            uint64_t k0 = kSentinel, k1 = kSentinel;
+            if (s[r] >= n || d[r] >= n) {
+                out_of_range = true;
+            } else if (s[r] != d[r]) {
+                k0 = ((uint64_t)s[r] << b) | d[r];
+                k1 = ((uint64_t)d[r] << b) | s[r];
+            }
+            *(ulonglong2 *)(keys + 2 * i) = make_ulonglong2(k0, k1);
+            atomicAdd(&cnt[(uint32_t)k0 & 255u], 1u);
+            atomicAdd(&cnt[(uint32_t)k1 & 255u], 1u);
+        }
     }
+    if (out_of_range) *bad = 1;
+    __syncthreads();
+    counts[(uint64_t)threadIdx.x * blocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
 // ------------------------------------------------------------------ compactions
@@ -527,10 +556,14 @@ lstc_status lstc::clean_orient_launch(const uint32_t *d_src, const uint32_t *d_d
     LSTC_TRY(fill_u32(L.dout, 0, (uint64_t)n + 1, st));
     LSTC_TRY(fill_u32(L.din, 0, (uint64_t)n + 1, st));
     if (m_raw) {
-        hipLaunchKernelGGL(k_make_keys, dim3(grid_k), dim3(256), 0, st, d_src, d_dst, m_raw, n, b, L.keys, L.bad);
+        const uint64_t kblocks = radix_blocks(nk);
+        if (kblocks > 0x7FFFFFFFull) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: too many edges");
+        hipLaunchKernelGGL(k_make_keys, dim3((uint32_t)kblocks), dim3(256), 0, st, d_src, d_dst, m_raw, n, b, L.keys,
+                           (uint32_t)kblocks, radix_first_counts(L.sort_ws, nk), L.bad);
         LSTC_LAUNCHED("k_make_keys");
         uint64_t *sorted = nullptr;
-        LSTC_TRY(radix_sort_u64(L.keys, L.alt, nk, (int)(2 * b > 64 ? 64 : 2 * b), &sorted, L.sort_ws, L.sort_bytes, st));
+        LSTC_TRY(radix_sort_u64(L.keys, L.alt, nk, (int)(2 * b > 64 ? 64 : 2 * b), &sorted, L.sort_ws, L.sort_bytes, st,
+                                true));
         uint64_t *uk = (sorted == L.keys) ? L.alt : L.keys;
         // unique arcs and the symmetric row bounds (m2 -> scal[0])
         LSTC_TRY(fill_u64(L.first, 0, (uint64_t)n + 1, st));

@@ -171,6 +171,7 @@ static constexpr int kSortRounds = 16;
 static constexpr uint32_t kSortSub = 4;
 static constexpr uint64_t kSortTile = kSortWaves * kSortRounds * 64;  // keys per sub-tile
 static constexpr uint64_t kSortBlock = kSortTile * kSortSub;          // keys per block
+static_assert(kSortBlock == kRadixBlock, "the sort's block is the one key producers count by");
 
 __global__ __launch_bounds__(kSortWaves * 64) void k_radix_hist(const uint64_t *__restrict__ keys,
                                                                 uint64_t n, uint32_t shift,
@@ -322,8 +323,15 @@ size_t radix_sort_ws_bytes(uint64_t n) {
     return radix_layout(n, &h, &g, &sw, &sb, nullptr, 0);
 }
 
+uint32_t *radix_first_counts(void *ws, uint64_t n) {
+    uint32_t *h; uint64_t *g; void *sw; size_t sb;
+    radix_layout(n, &h, &g, &sw, &sb, ws, SIZE_MAX);
+    return h;
+}
+
 lstc_status radix_sort_u64(uint64_t *keys, uint64_t *alt, uint64_t n, int key_bits,
-                           uint64_t **result, void *ws, size_t ws_bytes, hipStream_t s) {
+                           uint64_t **result, void *ws, size_t ws_bytes, hipStream_t s,
+                           bool first_counts_ready) {
     *result = keys;
     if (n <= 1 || key_bits <= 0) return LSTC_OK;
     uint32_t *hist; uint64_t *goff; void *sws; size_t sb;
@@ -333,9 +341,11 @@ lstc_status radix_sort_u64(uint64_t *keys, uint64_t *alt, uint64_t n, int key_bi
     if (tiles > 0x7FFFFFFFull) return fail(LSTC_ERR_INVALID, "radix sort: too many keys");
     uint64_t *in = keys, *out = alt;
     for (int shift = 0; shift < key_bits; shift += 8) {
-        hipLaunchKernelGGL(k_radix_hist, dim3((uint32_t)tiles), dim3(kSortWaves * 64), 0, s, in, n,
-                           (uint32_t)shift, (uint32_t)tiles, hist);
-        LSTC_LAUNCHED("k_radix_hist");
+        if (shift > 0 || !first_counts_ready) {
+            hipLaunchKernelGGL(k_radix_hist, dim3((uint32_t)tiles), dim3(kSortWaves * 64), 0, s, in, n,
+                               (uint32_t)shift, (uint32_t)tiles, hist);
+            LSTC_LAUNCHED("k_radix_hist");
+        }
         LSTC_TRY(scan_exclusive_u32_to_u64(hist, goff, tiles * 256, nullptr, sws, sb, s));
         hipLaunchKernelGGL(k_radix_scatter, dim3((uint32_t)tiles), dim3(kSortWaves * 64), 0, s, in,
                            out, n, (uint32_t)shift, (uint32_t)tiles, goff);
