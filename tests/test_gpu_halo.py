@@ -116,6 +116,8 @@ def _parts_equal_oracle(lstc, oracle, g, sym, part_dev, part_host, k, mode, tag,
     ("rmat_s12", lambda O: O.gen_rmat(12, 16, seed=2), 8),
     ("rmat_s12_k64", lambda O: O.gen_rmat(12, 16, seed=2), 64),
     ("rmat_s13_k13", lambda O: O.gen_rmat(13, 8, seed=6), 13),
+    ("rmat_s13_k17", lambda O: O.gen_rmat(13, 8, seed=7), 17),
+    ("rmat_s12_k32", lambda O: O.gen_rmat(12, 8, seed=8), 32),
     ("grid", lambda O: O.gen_grid(30, 30, 0.3, 4), 4),
     ("K100", lambda O: G.complete(100), 2),
     ("K100_k1", lambda O: G.complete(100), 1),
