@@ -174,8 +174,8 @@ static constexpr uint64_t kSortBlock = kSortTile * kSortSub;          // keys pe
 static_assert(kSortBlock == kRadixBlock, "the sort's block is the one key producers count by");
 
 // Digit counts of each block.  The keys are loaded with the non-temporal hint:
-// each is read once here, and the pass runs 12% faster so (0.40 vs 0.45 ms
-// per pass on 268 M keys); the scatter's own loads and stores keep the
+// each is read once here, and the pass measured 12% faster with it (0.40 vs
+// 0.45 ms per pass on 268 M keys); the scatter's own loads and stores keep the
 // default policy, where the hint measured slower.
 __global__ __launch_bounds__(kSortWaves * 64) void k_radix_hist(const uint64_t *__restrict__ keys,
                                                                 uint64_t n, uint32_t shift,
