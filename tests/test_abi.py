@@ -92,6 +92,28 @@ def test_count_rejects_bad_shapes_on_host():
     assert lib.lstc_segments(_FAKE, _FAKE, 10, 0xFFFFF000, None, 0, _FAKE, None, None) == _lib.LSTC_ERR_INVALID
 
 
+def test_stream_copy_entries_reject_bad_shapes_on_host():
+    """lstc_clean_orient_stream / lstc_count_split_stream refuse, before any device call, a missing
+    copy, a copy without segment words and sizes beyond their 32-bit positions."""
+    lib = _lib.load()
+    ws = ctypes.c_size_t(1 << 20)
+    assert lib.lstc_stream_copy_capacity(10, 100) == 100 + 31 * 10
+    u32, u64, u64b, u64c = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    args = lambda scol, m_raw, n: (_FAKE, _FAKE, m_raw, n, 1, None, None, _FAKE, _FAKE, _FAKE, _FAKE, _FAKE, _FAKE,
+                                   scol, ctypes.byref(u32), ctypes.byref(u64), ctypes.byref(u64b),
+                                   ctypes.byref(u64c), _FAKE, ws, None)
+    assert lib.lstc_clean_orient_stream(*args(None, 100, 10)) == _lib.LSTC_ERR_INVALID
+    assert "streaming copy" in lib.lstc_last_error().decode()
+    assert lib.lstc_clean_orient_stream(*args(_FAKE, 1 << 32, 1 << 27)) == _lib.LSTC_ERR_INVALID
+    assert "2^32" in lib.lstc_last_error().decode()
+    cs = lambda scol, seg, scol_len: (_FAKE, _FAKE, _FAKE, _FAKE, seg, seg, scol, scol_len, 10, 10, 20, _FAKE, None,
+                                      _FAKE, ws, None)
+    assert lib.lstc_count_split_stream(*cs(None, _FAKE, 40)) == _lib.LSTC_ERR_INVALID
+    assert lib.lstc_count_split_stream(*cs(_FAKE, None, 40)) == _lib.LSTC_ERR_INVALID
+    assert lib.lstc_count_split_stream(*cs(_FAKE, _FAKE, 0xFFFFF000)) == _lib.LSTC_ERR_INVALID
+    assert "streaming copy" in lib.lstc_last_error().decode()
+
+
 def test_batch_and_in_lists_reject_bad_shapes_on_host():
     lib = _lib.load()
     descs = (_lib.GraphDesc * 2)()

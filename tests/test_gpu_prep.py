@@ -166,3 +166,55 @@ def test_split_preprocessing_many_tiles(lstc, oracle):
     assert np.array_equal(g.col_seg[: g.m].cpu().numpy().view(np.uint64), want_cs[og.col])
     assert np.array_equal(g.in_seg[: len(want_col)].cpu().numpy().view(np.uint64), want_cs[want_col])
     assert lstc.count(g) == oracle.count(og)
+
+
+def _stream_copy_expected(og):
+    """Row starts of the streaming copy: d+ rounded up to 32 entries, scanned."""
+    deg = np.diff(og.off.astype(np.int64))
+    pad = (deg + 31) // 32 * 32
+    start = np.concatenate([[0], np.cumsum(pad)]).astype(np.int64)
+    return deg, start
+
+
+@pytest.mark.parametrize("name,builder", [
+    ("toy_edge_filter_cu", lambda O: G.toy_edge_filter_cu()),
+    ("K120", lambda O: G.complete(120)),
+    ("rmat_s13", lambda O: O.gen_rmat(13, 16, seed=5)),
+    ("er", lambda O: O.gen_er(30000, 12, seed=6)),
+    ("grid", lambda O: O.gen_grid(30, 40, 0.3, 2)),
+    ("rmat_s17", lambda O: O.gen_rmat(17, 16, seed=12)),
+    ("no_edges", lambda O: (7, np.zeros(0, np.uint32), np.zeros(0, np.uint32))),
+    ("one_edge", lambda O: (9, np.array([3], np.uint32), np.array([7], np.uint32))),
+])
+def test_stream_copy_bit_exact(lstc, oracle, name, builder):
+    """lstc_clean_orient_stream: the same CSR and in-lists as the oracle's, every row again in the
+    streaming copy at a 32-entry aligned start (d+ rounded up to 32, scanned), the segment words
+    holding those starts; the count that streams from the copy (lstc_count_split_stream), the
+    batched count (which walks without the words) and the count after the in-lists are rebuilt
+    all equal the oracle's."""
+    n, s, d = builder(oracle)
+    s = np.ascontiguousarray(s, np.uint32)
+    d = np.ascontiguousarray(d, np.uint32)
+    ds = torch.from_numpy(s.view(np.int32)).cuda() if len(s) else torch.zeros(0, dtype=torch.int32).cuda()
+    dd = torch.from_numpy(d.view(np.int32)).cuda() if len(d) else torch.zeros(0, dtype=torch.int32).cuda()
+    g = lstc.clean_orient(n, ds, dd, in_lists=True, stream_copy=True)
+    _, og = oracle.clean_orient(n, s, d)
+    assert g.n == og.n and g.m == og.m
+    off, col = g.to_numpy()
+    assert np.array_equal(off, og.off) and np.array_equal(col, og.col)
+    want_off, want_col = G.owned_in_lists(og.off, og.col, og.n)
+    assert np.array_equal(g.in_off[: g.n + 1].cpu().numpy().view(np.uint64), want_off)
+    assert np.array_equal(g.in_col[: len(want_col)].cpu().numpy().view(np.uint32), want_col)
+    deg, start = _stream_copy_expected(og)
+    assert int(g.scol.numel()) == max(int(start[-1]), 1)
+    scol = g.scol.cpu().numpy().view(np.uint32)
+    for v in np.flatnonzero(deg):
+        assert np.array_equal(scol[start[v]: start[v] + deg[v]], og.col[og.off[v]: og.off[v + 1]]), (name, v)
+    words = (start[:-1].astype(np.uint64) & np.uint64(0xFFFFFFFF)) | (deg.astype(np.uint64) << np.uint64(32))
+    assert np.array_equal(g.col_seg[: g.m].cpu().numpy().view(np.uint64), words[og.col])
+    assert np.array_equal(g.in_seg[: len(want_col)].cpu().numpy().view(np.uint64), words[want_col])
+    want = oracle.count(og)
+    assert lstc.count(g) == want
+    assert lstc.count_batch([g]) == [want]
+    lstc.in_lists(g)  # words into col again: the copy is dropped
+    assert g.scol is None and lstc.count(g) == want

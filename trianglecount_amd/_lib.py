@@ -122,6 +122,11 @@ SIGNATURES = {
     "lstc_count_workspace": (ST, [U32, U32, U64, C.POINTER(Tune), PSZ]),
     "lstc_count": (ST, [VP, VP, U32, U32, U64, VP, VP, C.POINTER(Tune), VP, SZ, VP]),
     "lstc_count_split": (ST, [VP, VP, VP, VP, VP, VP, U32, U32, U64, VP, C.POINTER(Tune), VP, SZ, VP]),
+    "lstc_count_split_stream": (ST, [VP, VP, VP, VP, VP, VP, VP, U64, U32, U32, U64, VP, C.POINTER(Tune), VP, SZ,
+                                     VP]),
+    "lstc_stream_copy_capacity": (U64, [U32, U64]),
+    "lstc_clean_orient_stream": (ST, [VP, VP, U64, U32, I32, VP, VP, VP, VP, VP, VP, VP, VP, VP, PU32, PU64, PU64,
+                                      PU64, VP, SZ, VP]),
     "lstc_count_split_static_limit": (SZ, []),
     "lstc_count_split_static": (ST, [VP, VP, VP, VP, VP, VP, U32, U32, U64, VP, C.POINTER(Tune), VP]),
     "lstc_triangle_support": (ST, [VP, VP, U32, U32, U64, VP, VP, VP]),

@@ -86,6 +86,7 @@ class Graph:
     in_owned: int | None = None  # the n_owned the in-lists were built for (owned sources only)
     col_seg: torch.Tensor | None = None  # segment words of col / in_col (lstc_segments), optional
     in_seg: torch.Tensor | None = None
+    scol: torch.Tensor | None = None  # streaming copy the segment words point into (rows 128-byte aligned)
 
     def split_ready(self, n_owned: int | None = None) -> bool:
         """True when the in-lists fit a count over the first ``n_owned`` rows."""
@@ -194,10 +195,12 @@ def partition_from_file(path: str, n: int, k: int, stream=None) -> torch.Tensor:
 # ------------------------------------------------------------------ (a) clean + orient
 def clean_orient(n: int, src: torch.Tensor, dst: torch.Tensor, compact: bool = True,
                  keep_symmetric: bool = False, in_lists: bool = False, ws: Workspace | None = None,
-                 stream=None) -> Graph:
+                 stream=None, stream_copy: bool = False) -> Graph:
     """Drop loops/repeats, symmetrise, optionally drop isolated vertices, orient by (degree, id).
 
-    ``in_lists`` also emits the target-side in-lists used by the split count.
+    ``in_lists`` also emits the target-side in-lists used by the split count; with
+    ``stream_copy`` also a copy of the columns whose rows start every 128 bytes, which
+    the count then streams its lists from (lstc_clean_orient_stream).
     """
     L = load()
     m_raw = int(src.numel())
@@ -213,15 +216,28 @@ def clean_orient(n: int, src: torch.Tensor, dst: torch.Tensor, compact: bool = T
     in_off = empty_u64(n + 1) if in_lists else None
     in_col = empty_u32(m_raw) if in_lists else None
     n_out, m_out = C.c_uint32(), C.c_uint64()
+    if stream_copy and not in_lists:
+        raise ValueError("clean_orient: a streaming copy comes with the in-lists (in_lists=True)")
     if in_lists:  # in-lists and segment words in the same pass (lstc_clean_orient_split)
         col_seg, in_seg, n_in = empty_u64(m_raw), empty_u64(m_raw), C.c_uint64()
-        check(L.lstc_clean_orient_split(_ptr(src), _ptr(dst), m_raw, n, int(bool(compact)), _ptr(sym_off),
-                                        _ptr(sym_col), _ptr(off), _ptr(col), _ptr(in_off), _ptr(in_col),
-                                        _ptr(col_seg), _ptr(in_seg), C.byref(n_out), C.byref(m_out), C.byref(n_in),
-                                        wp, wb, C.c_void_p(stream_handle(stream))), "lstc_clean_orient_split")
+        sh = C.c_void_p(stream_handle(stream))
+        scol = None
+        if stream_copy:
+            scol = empty_u32(max(int(L.lstc_stream_copy_capacity(n, m_raw)), 1))
+            s_used = C.c_uint64()
+            check(L.lstc_clean_orient_stream(_ptr(src), _ptr(dst), m_raw, n, int(bool(compact)), _ptr(sym_off),
+                                             _ptr(sym_col), _ptr(off), _ptr(col), _ptr(in_off), _ptr(in_col),
+                                             _ptr(col_seg), _ptr(in_seg), _ptr(scol), C.byref(n_out), C.byref(m_out),
+                                             C.byref(n_in), C.byref(s_used), wp, wb, sh), "lstc_clean_orient_stream")
+            scol = scol[: max(s_used.value, 1)]
+        else:
+            check(L.lstc_clean_orient_split(_ptr(src), _ptr(dst), m_raw, n, int(bool(compact)), _ptr(sym_off),
+                                            _ptr(sym_col), _ptr(off), _ptr(col), _ptr(in_off), _ptr(in_col),
+                                            _ptr(col_seg), _ptr(in_seg), C.byref(n_out), C.byref(m_out),
+                                            C.byref(n_in), wp, wb, sh), "lstc_clean_orient_split")
         g = Graph(n_out.value, m_out.value, off, col, sym_off, sym_col)
         g.in_off, g.in_col, g.in_owned = in_off, in_col, g.n
-        g.col_seg, g.in_seg = col_seg, in_seg
+        g.col_seg, g.in_seg, g.scol = col_seg, in_seg, scol
         g.meta["in_entries"] = n_in.value
         return g
     check(L.lstc_clean_orient(_ptr(src), _ptr(dst), m_raw, n, int(bool(compact)), _ptr(sym_off),
@@ -241,7 +257,7 @@ def segments(g: Graph, stream=None) -> Graph:
     in_seg = empty_u64(max(n_in, 1))
     check(L.lstc_segments(_ptr(g.off), _ptr(g.col), g.n, g.m, _ptr(g.in_col), n_in, _ptr(col_seg), _ptr(in_seg),
                           C.c_void_p(stream_handle(stream))), "lstc_segments")
-    g.col_seg, g.in_seg = col_seg, in_seg
+    g.col_seg, g.in_seg, g.scol = col_seg, in_seg, None  # these words point into col
     return g
 
 
@@ -266,11 +282,11 @@ def in_lists(g: Graph, ws: Workspace | None = None, stream=None, segment_words: 
         in_seg = empty_u64(max(mo, 1))
         check(L.lstc_in_lists_segments(_ptr(g.off), _ptr(g.col), g.n, no, g.m, _ptr(in_off), _ptr(in_col),
                                        _ptr(col_seg), _ptr(in_seg), C.byref(k), wp, wb, sh), "lstc_in_lists_segments")
-        g.col_seg, g.in_seg = col_seg, in_seg[: max(k.value, 1)]
+        g.col_seg, g.in_seg, g.scol = col_seg, in_seg[: max(k.value, 1)], None
     else:
         check(L.lstc_in_lists(_ptr(g.off), _ptr(g.col), g.n, no, g.m, _ptr(in_off), _ptr(in_col), C.byref(k), wp,
                               wb, sh), "lstc_in_lists")
-        g.col_seg = g.in_seg = None
+        g.col_seg = g.in_seg = g.scol = None
     g.in_off, g.in_col, g.in_owned = in_off, in_col, no
     g.meta["in_entries"] = k.value
     return g
@@ -294,7 +310,11 @@ def count(g: Graph, n_owned: int | None = None, per_row: bool = False, tune: Tun
     wp, wb = (ws or _default_ws).get(nb.value)
     res = out if out is not None else empty_u64(1)
     pr = empty_u64(no) if per_row else None
-    if g.split_ready(no) and not per_row and split:
+    if g.split_ready(no) and not per_row and split and g.scol is not None and g.col_seg is not None:
+        check(L.lstc_count_split_stream(_ptr(g.off), _ptr(g.col), _ptr(g.in_off), _ptr(g.in_col), _ptr(g.col_seg),
+                                        _ptr(g.in_seg), _ptr(g.scol), int(g.scol.numel()), g.n, no, g.m, _ptr(res),
+                                        tp, wp, wb, C.c_void_p(stream_handle(stream))), "lstc_count_split_stream")
+    elif g.split_ready(no) and not per_row and split:
         check(L.lstc_count_split(_ptr(g.off), _ptr(g.col), _ptr(g.in_off), _ptr(g.in_col), _ptr(g.col_seg),
                                  _ptr(g.in_seg), g.n, no, g.m, _ptr(res), tp, wp, wb,
                                  C.c_void_p(stream_handle(stream))), "lstc_count_split")
@@ -424,7 +444,9 @@ def halo_extract_parts(g: Graph, part: torch.Tensor, k: int, mode: int = 0, with
 
 
 # ------------------------------------------------------------------ batching
-def _descs(graphs, per_row=None):
+def _descs(graphs, per_row=None, stats=False):
+    """Batch descriptors.  stats: for lstc_work_stats, which only asks whether segment words
+    exist, so a graph's words are passed even when they point into its streaming copy."""
     from ._lib import GraphDesc
     arr = (GraphDesc * len(graphs))()
     for i, g in enumerate(graphs):
@@ -434,7 +456,9 @@ def _descs(graphs, per_row=None):
         if g.split_ready():
             arr[i].in_off = g.in_off.data_ptr()
             arr[i].in_col = g.in_col.data_ptr()
-            if g.col_seg is not None:
+            # a batch has no streaming copy: segment words that point into one are left out
+            # (the graph is then walked through its row offsets)
+            if g.col_seg is not None and (g.scol is None or stats):
                 arr[i].col_seg = g.col_seg.data_ptr()
                 arr[i].in_seg = g.in_seg.data_ptr()
         arr[i].m = g.m
@@ -606,7 +630,7 @@ def work_stats(graphs: list, tune: Tune | None = None, ws: Workspace | None = No
     check(L.lstc_work_stats_workspace(G, C.byref(nb)), "lstc_work_stats_workspace")
     wp, wb = (ws or _default_ws).get(nb.value)
     out = (WorkStats * max(G, 1))()
-    check(L.lstc_work_stats(_descs(graphs), G, C.byref(tune) if tune is not None else None, out, wp, wb,
+    check(L.lstc_work_stats(_descs(graphs, stats=True), G, C.byref(tune) if tune is not None else None, out, wp, wb,
                             C.c_void_p(stream_handle(stream))), "lstc_work_stats")
     return [out[i].as_dict() for i in range(G)]
 

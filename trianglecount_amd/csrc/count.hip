@@ -41,3 +41,18 @@ extern "C" lstc_status lstc_count_split(const uint64_t *d_off, const uint32_t *d
     return count_launch(&h, 1, nullptr, nullptr, (unsigned long long *)d_count, tune, d_ws, ws_bytes,
                         (hipStream_t)s);
 }
+
+extern "C" lstc_status lstc_count_split_stream(const uint64_t *d_off, const uint32_t *d_col, const uint64_t *d_in_off,
+                                               const uint32_t *d_in_col, const uint64_t *d_col_seg,
+                                               const uint64_t *d_in_seg, const uint32_t *d_scol, uint64_t scol_len,
+                                               uint32_t n_rows, uint32_t n_owned, uint64_t m, uint64_t *d_count,
+                                               const lstc_tune *tune, void *d_ws, size_t ws_bytes, lstc_stream s) {
+    TraceRange range_("lstc_count_split_stream");
+    if (!d_count) return fail(LSTC_ERR_INVALID, "lstc_count_split_stream: NULL output");
+    if (!d_in_off || !d_in_col || !d_col_seg || !d_in_seg || (m && !d_scol))
+        return fail(LSTC_ERR_INVALID, "lstc_count_split_stream: NULL in-lists, segment words or streaming copy");
+    HostGraph h{d_off, d_col, nullptr, n_rows, n_owned, m, d_in_off, d_in_col, d_col_seg, d_in_seg, d_scol, scol_len};
+    LSTC_TRY(validate_graph(h, "lstc_count_split_stream"));
+    return count_launch(&h, 1, nullptr, nullptr, (unsigned long long *)d_count, tune, d_ws, ws_bytes,
+                        (hipStream_t)s);
+}

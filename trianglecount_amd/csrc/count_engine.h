@@ -30,6 +30,8 @@ struct GraphDev {
     uint32_t n_owned;
     const uint64_t *col_seg;      // segment words of col[] / in_col[] (split count), may be null
     const uint64_t *in_seg;
+    const uint32_t *scol;         // the columns the walk reads at segment-word starts: col, or a
+                                  // copy whose rows start every 128 bytes (lstc_clean_orient_stream)
 };
 
 // The graphs of one launch, passed by value.  A single graph travels inline
@@ -53,6 +55,8 @@ struct HostGraph {
     const uint32_t *in_col = nullptr;
     const uint64_t *col_seg = nullptr;  // optional segment words (lstc_segments)
     const uint64_t *in_seg = nullptr;
+    const uint32_t *scol = nullptr;     // optional streaming copy the segment words point into
+    uint64_t scol_len = 0;              // its entries
 };
 
 // Split counting (in-lists present) applies only to whole graphs without per-row output.
@@ -68,7 +72,8 @@ inline bool seg_ok(const HostGraph &h) { return split_ok(h) && h.col_seg && h.in
 inline GraphDev to_dev(const HostGraph &h) {
     const bool sp = split_ok(h), sg = seg_ok(h);
     return GraphDev{h.off, h.col, (unsigned long long *)h.per_row, sp ? h.in_off : nullptr, sp ? h.in_col : nullptr,
-                    h.n_rows, h.n_owned, sg ? h.col_seg : nullptr, sg ? h.in_seg : nullptr};
+                    h.n_rows, h.n_owned, sg ? h.col_seg : nullptr, sg ? h.in_seg : nullptr,
+                    sg && h.scol ? h.scol : h.col};
 }
 
 // Resolved tuning (defaults filled in, ranges clamped).

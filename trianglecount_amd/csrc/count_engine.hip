@@ -128,6 +128,8 @@ lstc_status validate_graph(const HostGraph &h, const char *who) {
     if ((h.col_seg == nullptr) != (h.in_seg == nullptr))
         return fail(LSTC_ERR_INVALID, "%s: give both segment-word arrays or neither", who);
     if (h.col_seg && !h.in_off) return fail(LSTC_ERR_INVALID, "%s: segment words without in-lists", who);
+    if (h.scol && (!h.col_seg || h.scol_len >= 0xFFFFF000ull))
+        return fail(LSTC_ERR_INVALID, "%s: a streaming copy needs segment words and < 2^32-4096 entries", who);
     return LSTC_OK;
 }
 
@@ -153,6 +155,7 @@ __device__ __forceinline__ GraphDev graph_at(const GraphSet &gs, uint32_t g) {
         d.n_owned = q->n_owned;
         d.col_seg = q->col_seg;
         d.in_seg = q->in_seg;
+        d.scol = q->scol;
         return d;
     }
     const LSTC_GLOBAL_AS GraphDev *q = (const LSTC_GLOBAL_AS GraphDev *)(gs.table + g);
@@ -164,6 +167,7 @@ __device__ __forceinline__ GraphDev graph_at(const GraphSet &gs, uint32_t g) {
     d.in_col = q->in_col;
     d.n_rows = q->n_rows;
     d.n_owned = q->n_owned;
+    d.scol = q->scol;
     return d;
 }
 
@@ -683,7 +687,8 @@ __device__ __forceinline__ uint32_t tiny_walk(const GraphDev &G, bool mine, bool
 }
 
 // ------------------------------------------------------------------ persistent count kernel
-// NARROW: every graph of the launch has m <= 2^30 arcs.
+// NARROW: every graph of the launch has m <= 2^30 arcs (and a streaming copy of at
+// most 2^30 entries).
 // Occupancy target (the second launch bound is waves per SIMD): 8 with UNROLL 4,
 // 6 with UNROLL 8, whatever the workgroup size.
 template <int UNROLL, bool NARROW, int WAVES>
@@ -739,7 +744,7 @@ __global__ __launch_bounds__(WAVES * 64, UNROLL == 4 ? 8 : 6) void k_count(Graph
                 __syncthreads();
                 table_mark(s_tab, s_ovf, tl, tid, WAVES * 64);
                 __syncthreads();
-                hits += walk<UNROLL, NARROW>(G.off, G.col, G.n_rows, sg.ids, sg.seg, sg.n, sg.below, wv * 64, WAVES * 64, s_tab, tl,
+                hits += walk<UNROLL, NARROW>(G.off, sg.seg ? G.scol : G.col, G.n_rows, sg.ids, sg.seg, sg.n, sg.below, wv * 64, WAVES * 64, s_tab, tl,
                                      s_base[wv], s_end[wv], lane);
                 __syncthreads();
             }
@@ -792,7 +797,7 @@ __global__ __launch_bounds__(WAVES * 64, UNROLL == 4 ? 8 : 6) void k_count(Graph
                 table_insert(tab, ovf, tl, G.col, rb, K, lane, 64);
                 table_mark(tab, ovf, tl, lane, 64);
                 const unsigned long long hits =
-                    walk<UNROLL, NARROW>(G.off, G.col, G.n_rows, sg.ids, sg.seg, sg.n, sg.below, 0, 64, tab, tl,
+                    walk<UNROLL, NARROW>(G.off, sg.seg ? G.scol : G.col, G.n_rows, sg.ids, sg.seg, sg.n, sg.below, 0, 64, tab, tl,
                                  s_base[wv], s_end[wv], lane);
                 if (g != cur_g) {
                     if (cur_g != kEmpty && acc != 0 && lane == 0) atomicAdd(counts + cur_g, acc);
@@ -895,7 +900,8 @@ lstc_status count_launch(const HostGraph *hosts, uint32_t G, const GraphDev *d_t
     if (total) {
         uint32_t grid = (uint32_t)cus * tr.grid_per_cu;
         bool narrow = true;
-        for (uint32_t g = 0; g < G; ++g) narrow = narrow && hosts[g].m <= (1ull << 30);
+        for (uint32_t g = 0; g < G; ++g)
+            narrow = narrow && hosts[g].m <= (1ull << 30) && hosts[g].scol_len <= (1ull << 30);
         using Kern = void (*)(GraphSet, const unsigned long long *, uint32_t *, unsigned long long *, uint32_t);
         // [unroll 4/8][narrow][waves 2/4/8]
         static const Kern kerns[2][2][3] = {

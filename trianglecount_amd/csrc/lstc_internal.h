@@ -109,13 +109,16 @@ uint32_t *radix_first_counts(void *ws, uint64_t n);
 
 // (a) clean + orient without synchronising (prep.hip); sizes stay on the device:
 // (*d_sizes)[1] = vertices after cleaning, (*d_sizes)[2] = undirected edges,
-// (*d_sizes)[3] = in-list entries.  In-lists and segment words are optional
-// (NULL); segment words of the in-lists need the in-lists.
+// (*d_sizes)[3] = in-list entries, (*d_sizes)[4] = streaming copy entries.
+// In-lists and segment words are optional (NULL); segment words of the in-lists
+// need the in-lists.  d_scol (optional, with segment words): a streaming copy of
+// the columns whose rows start every 32 entries; the segment words then hold
+// row starts in it.
 lstc_status clean_orient_launch(const uint32_t *d_src, const uint32_t *d_dst, uint64_t m_raw, uint32_t n,
                                 int compact, uint64_t *d_sym_off, uint32_t *d_sym_col, uint64_t *d_off,
                                 uint32_t *d_col, uint64_t *d_in_off, uint32_t *d_in_col, uint64_t *d_col_seg,
                                 uint64_t *d_in_seg, void *d_ws, size_t ws_bytes, hipStream_t st,
-                                const uint64_t **d_sizes, const uint32_t **d_bad);
+                                const uint64_t **d_sizes, const uint32_t **d_bad, uint32_t *d_scol = nullptr);
 
 // Seeded generators (gen.hip) writing every id plus id0: lstc_gen_rmat and
 // lstc_gen_er are these with id0 = 0; the server's grouped preparation gives

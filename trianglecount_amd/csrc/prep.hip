@@ -315,6 +315,8 @@ struct OrientInPass {
     uint32_t *ticket;
     uint64_t *in_total;
     uint32_t *flag;
+    uint32_t *scol;           // streaming copy (NULL: none): row s from rec[s].off_lo, 128-byte aligned
+    const uint64_t *doscan;   // row starts in col (old ids), for an arc's index in its row
 };
 
 __global__ __launch_bounds__(kCT) void k_orient_in_scatter(OrientInPass op, const uint64_t *__restrict__ tile_off) {
@@ -374,6 +376,7 @@ __global__ __launch_bounds__(kCT) void k_orient_in_scatter(OrientInPass op, cons
                 const uint64_t q = pos + before + mask_rank(km);
                 op.col[q] = vd[r].nid;
                 if (op.col_seg) op.col_seg[q] = seg_word(vd[r]);
+                if (op.scol) op.scol[vs[r].off_lo + (q - op.doscan[s])] = vd[r].nid;
             }
             pos += total;
         }
@@ -459,6 +462,11 @@ __global__ void k_orient_offsets(const uint32_t *__restrict__ keepv, const uint6
     if (blockIdx.x == 0 && threadIdx.x == 0) off[*d_n_out] = *d_m;
 }
 
+// Row lengths of the streaming copy: d+ rounded up to 32 entries (128 bytes).
+__global__ void k_pad_rows(const uint32_t *__restrict__ dout, uint32_t n, uint32_t *__restrict__ plen) {
+    GRID_STRIDE(v, n) plen[v] = (dout[v] + 31u) & ~31u;
+}
+
 // Per-vertex records for the oriented and in-list scatters (old ids; doscan is
 // the exclusive scan of d+ over old ids, which is the oriented row start of the
 // vertex's new id, since the relabelling keeps the order).
@@ -477,7 +485,9 @@ struct PrepLayout {
     uint32_t *deg, *keepv, *nid, *dout, *din;
     VRec *rec;
     uint64_t *degscan, *nidscan, *doscan, *first, *last, *discan;
-    uint64_t *scal;  // [0]=m2 [1]=n_out [2]=m [3]=in-list entries
+    uint32_t *plen;   // streaming copy: padded row lengths, and their scan
+    uint64_t *pscan;
+    uint64_t *scal;  // [0]=m2 [1]=n_out [2]=m [3]=in-list entries [4]=streaming copy entries
     uint32_t *bad;
     void *sort_ws;
     size_t sort_bytes;
@@ -509,6 +519,8 @@ static PrepLayout prep_layout(void *ws, size_t ws_bytes, uint32_t n, uint64_t m_
     L.doscan = c.take<uint64_t>((size_t)n + 1);
     L.first = c.take<uint64_t>((size_t)n + 1);
     L.last = c.take<uint64_t>((size_t)n + 1);
+    L.plen = c.take<uint32_t>((size_t)n + 1);
+    L.pscan = c.take<uint64_t>((size_t)n + 1);
     L.scal = c.take<uint64_t>(8);
     L.bad = c.take<uint32_t>(4);
     L.sort_bytes = radix_sort_ws_bytes(nk);
@@ -536,13 +548,15 @@ lstc_status lstc::clean_orient_launch(const uint32_t *d_src, const uint32_t *d_d
                                       int compact, uint64_t *d_sym_off, uint32_t *d_sym_col, uint64_t *d_off,
                                       uint32_t *d_col, uint64_t *d_in_off, uint32_t *d_in_col, uint64_t *d_col_seg,
                                       uint64_t *d_in_seg, void *d_ws, size_t ws_bytes, hipStream_t st,
-                                      const uint64_t **d_sizes, const uint32_t **d_bad) {
+                                      const uint64_t **d_sizes, const uint32_t **d_bad, uint32_t *d_scol) {
     if (n == 0xFFFFFFFFu) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: n must be < 2^32-1");
     if (!d_off) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: NULL output");
     if ((d_sym_off == nullptr) != (d_sym_col == nullptr))
         return fail(LSTC_ERR_INVALID, "lstc_clean_orient: give both symmetric outputs or neither");
     if (m_raw && (!d_src || !d_dst || !d_col)) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: NULL edge arrays");
     if (d_in_seg && !d_in_col) return fail(LSTC_ERR_INVALID, "lstc_clean_orient: in-list segment words need in-lists");
+    if (d_scol && (!d_col_seg || (uint64_t)m_raw + 31ull * n >= (1ull << 32)))
+        return fail(LSTC_ERR_INVALID, "lstc_clean_orient: a streaming copy needs segment words and m_raw + 31 n < 2^32");
     PrepLayout L = prep_layout(d_ws, ws_bytes, n, m_raw);
     if (!d_ws || ws_bytes < L.bytes)
         return fail(LSTC_ERR_WORKSPACE, "lstc_clean_orient: workspace %zu < %zu bytes", ws_bytes, L.bytes);
@@ -595,14 +609,20 @@ lstc_status lstc::clean_orient_launch(const uint32_t *d_src, const uint32_t *d_d
         hipLaunchKernelGGL(k_orient_offsets, dim3(grid_n), dim3(256), 0, st, L.keepv, L.nidscan, L.doscan, n,
                            L.scal + 1, L.scal + 2, d_off);
         LSTC_LAUNCHED("k_orient_offsets");
-        hipLaunchKernelGGL(k_vrec, dim3(grid_n), dim3(256), 0, st, L.nid, L.dout, L.doscan, L.deg, n, L.rec);
+        if (d_scol) {  // the streaming copy's row starts (entries -> scal[4]); the records carry them
+            hipLaunchKernelGGL(k_pad_rows, dim3(grid_n), dim3(256), 0, st, L.dout, n, L.plen);
+            LSTC_LAUNCHED("k_pad_rows");
+            LSTC_TRY(scan_exclusive_u32_to_u64(L.plen, L.pscan, n, L.scal + 4, L.scan_ws, L.scan_bytes, st));
+        }
+        hipLaunchKernelGGL(k_vrec, dim3(grid_n), dim3(256), 0, st, L.nid, L.dout, d_scol ? L.pscan : L.doscan, L.deg,
+                           n, L.rec);
         LSTC_LAUNCHED("k_vrec");
         // scatter pass: columns (and segment words); in-lists (entries -> scal[3]) in the same pass
         LSTC_TRY(fill_u64(L.tstat, 0, tiles, st));
         LSTC_TRY(fill_u32(L.ticket, 0, 1, st));
         hipLaunchKernelGGL(k_orient_in_scatter, dim3((uint32_t)tiles), dim3(kCT), 0, st,
                            OrientInPass{uk, L.scal + 0, b, mask, L.rec, d_col, d_col_seg, d_in_col, d_in_seg, L.din,
-                                        L.tstat, L.ticket, L.scal + 3, L.bad},
+                                        L.tstat, L.ticket, L.scal + 3, L.bad, d_scol, L.doscan},
                            (const uint64_t *)L.tile_off);
         LSTC_LAUNCHED("k_orient_in_scatter");
         if (d_in_off) {
@@ -634,6 +654,35 @@ lstc_status lstc::clean_orient_launch(const uint32_t *d_src, const uint32_t *d_d
     return LSTC_OK;
 }
 
+// The split preparation (and, with d_scol, its streaming copy): one
+// synchronisation at the end, sizes to the host.
+static lstc_status clean_orient_split_impl(const char *who, const uint32_t *d_src, const uint32_t *d_dst,
+                                           uint64_t m_raw, uint32_t n, int compact, uint64_t *d_sym_off,
+                                           uint32_t *d_sym_col, uint64_t *d_off, uint32_t *d_col, uint64_t *d_in_off,
+                                           uint32_t *d_in_col, uint64_t *d_col_seg, uint64_t *d_in_seg,
+                                           uint32_t *d_scol, uint32_t *n_out, uint64_t *m_out, uint64_t *n_in_out,
+                                           uint64_t *scol_out, void *d_ws, size_t ws_bytes, hipStream_t st) {
+    if (!n_out || !m_out || !n_in_out) return fail(LSTC_ERR_INVALID, "%s: NULL size outputs", who);
+    if (!d_in_off || !d_in_col || !d_col_seg || !d_in_seg)
+        return fail(LSTC_ERR_INVALID, "%s: in-list and segment-word outputs are required", who);
+    const uint64_t *d_sizes = nullptr;
+    const uint32_t *d_bad = nullptr;
+    LSTC_TRY(clean_orient_launch(d_src, d_dst, m_raw, n, compact, d_sym_off, d_sym_col, d_off, d_col, d_in_off,
+                                 d_in_col, d_col_seg, d_in_seg, d_ws, ws_bytes, st, &d_sizes, &d_bad, d_scol));
+    uint64_t h[5];
+    uint32_t bad = 0;
+    LSTC_HIP(hipMemcpyAsync(h, d_sizes, sizeof(h), hipMemcpyDeviceToHost, st));
+    LSTC_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, st));
+    LSTC_HIP(hipStreamSynchronize(st));
+    if (bad & 2u) return fail(LSTC_ERR_DEVICE, "%s: in-list look-back exceeded its bound", who);
+    if (bad) return fail(LSTC_ERR_INVALID, "%s: vertex id >= n in input", who);
+    *n_out = (uint32_t)h[1];
+    *m_out = h[2];
+    *n_in_out = h[3];
+    if (scol_out) *scol_out = h[4];
+    return LSTC_OK;
+}
+
 extern "C" lstc_status lstc_clean_orient_split(const uint32_t *d_src, const uint32_t *d_dst, uint64_t m_raw,
                                                uint32_t n, int compact, uint64_t *d_sym_off, uint32_t *d_sym_col,
                                                uint64_t *d_off, uint32_t *d_col, uint64_t *d_in_off,
@@ -641,25 +690,25 @@ extern "C" lstc_status lstc_clean_orient_split(const uint32_t *d_src, const uint
                                                uint32_t *n_out, uint64_t *m_out, uint64_t *n_in_out, void *d_ws,
                                                size_t ws_bytes, lstc_stream s) {
     TraceRange range_("lstc_clean_orient_split");
-    hipStream_t st = (hipStream_t)s;
-    if (!n_out || !m_out || !n_in_out) return fail(LSTC_ERR_INVALID, "lstc_clean_orient_split: NULL size outputs");
-    if (!d_in_off || !d_in_col || !d_col_seg || !d_in_seg)
-        return fail(LSTC_ERR_INVALID, "lstc_clean_orient_split: in-list and segment-word outputs are required");
-    const uint64_t *d_sizes = nullptr;
-    const uint32_t *d_bad = nullptr;
-    LSTC_TRY(clean_orient_launch(d_src, d_dst, m_raw, n, compact, d_sym_off, d_sym_col, d_off, d_col, d_in_off,
-                                 d_in_col, d_col_seg, d_in_seg, d_ws, ws_bytes, st, &d_sizes, &d_bad));
-    uint64_t h[4];
-    uint32_t bad = 0;
-    LSTC_HIP(hipMemcpyAsync(h, d_sizes, sizeof(h), hipMemcpyDeviceToHost, st));
-    LSTC_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, st));
-    LSTC_HIP(hipStreamSynchronize(st));
-    if (bad & 2u) return fail(LSTC_ERR_DEVICE, "lstc_clean_orient_split: in-list look-back exceeded its bound");
-    if (bad) return fail(LSTC_ERR_INVALID, "lstc_clean_orient_split: vertex id >= n in input");
-    *n_out = (uint32_t)h[1];
-    *m_out = h[2];
-    *n_in_out = h[3];
-    return LSTC_OK;
+    return clean_orient_split_impl("lstc_clean_orient_split", d_src, d_dst, m_raw, n, compact, d_sym_off, d_sym_col,
+                                   d_off, d_col, d_in_off, d_in_col, d_col_seg, d_in_seg, nullptr, n_out, m_out,
+                                   n_in_out, nullptr, d_ws, ws_bytes, (hipStream_t)s);
+}
+
+extern "C" uint64_t lstc_stream_copy_capacity(uint32_t n, uint64_t m_raw) { return m_raw + 31ull * n; }
+
+extern "C" lstc_status lstc_clean_orient_stream(const uint32_t *d_src, const uint32_t *d_dst, uint64_t m_raw,
+                                                uint32_t n, int compact, uint64_t *d_sym_off, uint32_t *d_sym_col,
+                                                uint64_t *d_off, uint32_t *d_col, uint64_t *d_in_off,
+                                                uint32_t *d_in_col, uint64_t *d_col_seg, uint64_t *d_in_seg,
+                                                uint32_t *d_scol, uint32_t *n_out, uint64_t *m_out,
+                                                uint64_t *n_in_out, uint64_t *scol_out, void *d_ws, size_t ws_bytes,
+                                                lstc_stream s) {
+    TraceRange range_("lstc_clean_orient_stream");
+    if (!d_scol || !scol_out) return fail(LSTC_ERR_INVALID, "lstc_clean_orient_stream: NULL streaming copy output");
+    return clean_orient_split_impl("lstc_clean_orient_stream", d_src, d_dst, m_raw, n, compact, d_sym_off, d_sym_col,
+                                   d_off, d_col, d_in_off, d_in_col, d_col_seg, d_in_seg, d_scol, n_out, m_out,
+                                   n_in_out, scol_out, d_ws, ws_bytes, (hipStream_t)s);
 }
 
 extern "C" lstc_status lstc_clean_orient(const uint32_t *d_src, const uint32_t *d_dst, uint64_t m_raw,
