@@ -336,6 +336,8 @@ typedef struct lstc_graph_desc {
     uint32_t n_owned;
     const uint64_t *col_seg; /* segment words (lstc_segments) or NULL; used with   */
     const uint64_t *in_seg;  /* the in-lists they describe                        */
+    const uint32_t *scol;    /* streaming copy the segment words point into, with  */
+    uint64_t scol_len;       /* its entries (lstc_clean_orient_stream), or NULL, 0 */
 } lstc_graph_desc;          /* at most 2047 graphs per launch */
 
 lstc_status lstc_count_batch_workspace(const lstc_graph_desc *graphs, uint32_t G, const lstc_tune *tune,

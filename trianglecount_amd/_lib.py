@@ -46,7 +46,7 @@ class GraphDesc(C.Structure):
     _fields_ = [("off", C.c_void_p), ("col", C.c_void_p), ("per_row", C.c_void_p),
                 ("in_off", C.c_void_p), ("in_col", C.c_void_p),
                 ("m", C.c_uint64), ("n_rows", C.c_uint32), ("n_owned", C.c_uint32),
-                ("col_seg", C.c_void_p), ("in_seg", C.c_void_p)]
+                ("col_seg", C.c_void_p), ("in_seg", C.c_void_p), ("scol", C.c_void_p), ("scol_len", C.c_uint64)]
 
 
 class HaloOut(C.Structure):

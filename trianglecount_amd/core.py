@@ -444,9 +444,7 @@ def halo_extract_parts(g: Graph, part: torch.Tensor, k: int, mode: int = 0, with
 
 
 # ------------------------------------------------------------------ batching
-def _descs(graphs, per_row=None, stats=False):
-    """Batch descriptors.  stats: for lstc_work_stats, which only asks whether segment words
-    exist, so a graph's words are passed even when they point into its streaming copy."""
+def _descs(graphs, per_row=None):
     from ._lib import GraphDesc
     arr = (GraphDesc * len(graphs))()
     for i, g in enumerate(graphs):
@@ -456,11 +454,12 @@ def _descs(graphs, per_row=None, stats=False):
         if g.split_ready():
             arr[i].in_off = g.in_off.data_ptr()
             arr[i].in_col = g.in_col.data_ptr()
-            # a batch has no streaming copy: segment words that point into one are left out
-            # (the graph is then walked through its row offsets)
-            if g.col_seg is not None and (g.scol is None or stats):
+            if g.col_seg is not None:
                 arr[i].col_seg = g.col_seg.data_ptr()
                 arr[i].in_seg = g.in_seg.data_ptr()
+                if g.scol is not None:  # the words point into the streaming copy
+                    arr[i].scol = g.scol.data_ptr()
+                    arr[i].scol_len = int(g.scol.numel())
         arr[i].m = g.m
         arr[i].n_rows = g.n
         arr[i].n_owned = g.owned
@@ -630,7 +629,7 @@ def work_stats(graphs: list, tune: Tune | None = None, ws: Workspace | None = No
     check(L.lstc_work_stats_workspace(G, C.byref(nb)), "lstc_work_stats_workspace")
     wp, wb = (ws or _default_ws).get(nb.value)
     out = (WorkStats * max(G, 1))()
-    check(L.lstc_work_stats(_descs(graphs, stats=True), G, C.byref(tune) if tune is not None else None, out, wp, wb,
+    check(L.lstc_work_stats(_descs(graphs), G, C.byref(tune) if tune is not None else None, out, wp, wb,
                             C.c_void_p(stream_handle(stream))), "lstc_work_stats")
     return [out[i].as_dict() for i in range(G)]
 

@@ -25,7 +25,8 @@ static lstc_status to_hosts(const lstc_graph_desc *graphs, uint32_t G, std::vect
     uint64_t acc = 0;
     for (uint32_t g = 0; g < G; ++g) {
         const lstc_graph_desc &d = graphs[g];
-        hosts[g] = HostGraph{d.off, d.col, d.per_row, d.n_rows, d.n_owned, d.m, d.in_off, d.in_col, d.col_seg, d.in_seg};
+        hosts[g] = HostGraph{d.off, d.col, d.per_row, d.n_rows, d.n_owned, d.m,
+                                     d.in_off, d.in_col, d.col_seg, d.in_seg, d.scol, d.scol_len};
         LSTC_TRY(validate_graph(hosts[g], "batch"));
         devs[g] = to_dev(hosts[g]);
         base[g] = acc;
@@ -39,7 +40,8 @@ static uint64_t planned_total(const lstc_graph_desc *graphs, uint32_t G) {
     uint64_t t = 0;
     for (uint32_t g = 0; g < G; ++g) {
         const lstc_graph_desc &d = graphs[g];
-        t += plan_rows(HostGraph{d.off, d.col, d.per_row, d.n_rows, d.n_owned, d.m, d.in_off, d.in_col, d.col_seg, d.in_seg});
+        t += plan_rows(HostGraph{d.off, d.col, d.per_row, d.n_rows, d.n_owned, d.m,
+                                     d.in_off, d.in_col, d.col_seg, d.in_seg, d.scol, d.scol_len});
     }
     return t;
 }

@@ -144,7 +144,8 @@ extern "C" lstc_status lstc_work_stats(const lstc_graph_desc *graphs, uint32_t G
     std::vector<HostGraph> hosts(G);
     for (uint32_t g = 0; g < G; ++g) {
         const lstc_graph_desc &d = graphs[g];
-        hosts[g] = HostGraph{d.off, d.col, d.per_row, d.n_rows, d.n_owned, d.m, d.in_off, d.in_col, d.col_seg, d.in_seg};
+        hosts[g] = HostGraph{d.off, d.col, d.per_row, d.n_rows, d.n_owned, d.m,
+                              d.in_off, d.in_col, d.col_seg, d.in_seg, d.scol, d.scol_len};
         LSTC_TRY(validate_graph(hosts[g], "lstc_work_stats"));
         const GraphDev dev = to_dev(hosts[g]);
         const uint64_t rows = plan_rows(hosts[g]);
