@@ -230,6 +230,22 @@ lstc_status lstc_count_split(const uint64_t *d_off, const uint32_t *d_col, const
                              uint32_t n_rows, uint32_t n_owned, uint64_t m, uint64_t *d_count,
                              const lstc_tune *tune, void *d_ws, size_t ws_bytes, lstc_stream s);
 
+/* The row-aligned streaming copy for a CSR whose in-lists and segment words
+ * came from lstc_in_lists_segments (halo parts, a CSR read from a file): every
+ * row of d_off / d_col again in d_scol starting on a 32-entry boundary, and the
+ * start field of every segment word with a list rewritten in place to point
+ * there: d_col_seg[0 .. n_col_words) (the entries of d_col that have words:
+ * off[n_owned] of them for a halo part, m for a whole graph) and
+ * d_in_seg[0 .. n_in) (ids d_in_col).  d_scol holds at least m + 31 n_rows
+ * entries (< 2^32); *scol_out receives the entries used (the stream is
+ * synchronised).  The words then go with d_scol to lstc_count_split_stream or
+ * a batch descriptor.  No counterpart in the reference.                      */
+lstc_status lstc_stream_copy_workspace(uint32_t n_rows, size_t *bytes);
+lstc_status lstc_stream_copy(const uint64_t *d_off, const uint32_t *d_col, uint32_t n_rows, uint64_t m,
+                             uint64_t n_col_words, uint64_t *d_col_seg, const uint32_t *d_in_col, uint64_t n_in,
+                             uint64_t *d_in_seg, uint32_t *d_scol, uint64_t scol_cap, uint64_t *scol_out,
+                             void *d_ws, size_t ws_bytes, lstc_stream s);
+
 /* lstc_count_split over the outputs of lstc_clean_orient_stream: the lists
  * named by the segment words are read from the streaming copy d_scol
  * (scol_len entries), tables and everything else from d_off / d_col.  Same

@@ -114,6 +114,21 @@ def test_stream_copy_entries_reject_bad_shapes_on_host():
     assert "streaming copy" in lib.lstc_last_error().decode()
 
 
+def test_stream_copy_of_a_csr_rejects_bad_shapes_on_host():
+    lib = _lib.load()
+    ws = ctypes.c_size_t(1 << 20)
+    used = ctypes.c_uint64()
+    sc = lambda m, n_rows, words, cap: (_FAKE, _FAKE, n_rows, m, words, _FAKE, _FAKE, 4, _FAKE, _FAKE, cap,
+                                        ctypes.byref(used), _FAKE, ws, None)
+    assert lib.lstc_stream_copy(*sc(100, 10, 100, 409)) == _lib.LSTC_ERR_INVALID  # capacity < m + 31 n_rows
+    assert "capacity" in lib.lstc_last_error().decode()
+    assert lib.lstc_stream_copy(*sc(1 << 32, 10, 0, 1 << 33)) == _lib.LSTC_ERR_INVALID
+    assert lib.lstc_stream_copy(*sc(100, 10, 101, 410)) == _lib.LSTC_ERR_INVALID  # more words than arcs
+    assert lib.lstc_stream_copy(_FAKE, None, 10, 100, 0, None, None, 0, None, _FAKE, 410, ctypes.byref(used),
+                                _FAKE, ws, None) == _lib.LSTC_ERR_INVALID
+    assert "NULL" in lib.lstc_last_error().decode()
+
+
 def test_batch_and_in_lists_reject_bad_shapes_on_host():
     lib = _lib.load()
     descs = (_lib.GraphDesc * 2)()

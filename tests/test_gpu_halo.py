@@ -97,6 +97,61 @@ def test_halo_split_parts_match_oracle(lstc, oracle, name, builder, k, mode):
     assert lstc.count_batch(parts) == [lstc.count(h) for h in parts]
 
 
+@pytest.mark.parametrize("name,builder,k", [
+    ("rmat_s13", lambda O: O.gen_rmat(13, 16, seed=6), 8),
+    ("grid", lambda O: O.gen_grid(40, 40, 0.3, 4), 4),
+])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_halo_parts_stream_copy(lstc, oracle, name, builder, k, mode):
+    """lstc_stream_copy on split halo parts (words only for the owned rows' arcs): every row again
+    at its 32-entry aligned start, each word with a list pointing there, and the counts (alone,
+    batched, under tunings) equal the oracle's per part."""
+    n, s, d = builder(oracle)
+    g = _device_graph(lstc, n, s, d)
+    sym, og = oracle.clean_orient(n, s, d)
+    part = lstc.partition_random(g.n, k, seed=7)
+    opart = oracle.partition_random(sym.n, k, seed=7)
+    parts, wants = [], []
+    for p in range(k):
+        h = lstc.halo_extract(g, part, p, mode=mode, split=True)
+        oh = oracle.halo_extract(sym, opart, p, mode)
+        nw = int(oh.csr.off[oh.n_owned])
+        before = h.col_seg[:nw].cpu().numpy().view(np.uint64).copy()
+        lstc.stream_copy(h)
+        deg = np.diff(oh.csr.off.astype(np.int64))
+        start = np.concatenate([[0], np.cumsum((deg + 31) // 32 * 32)]).astype(np.int64)
+        assert int(h.scol.numel()) == max(int(start[-1]), 1)
+        scol = h.scol.cpu().numpy().view(np.uint32)
+        for v in np.flatnonzero(deg):
+            assert np.array_equal(scol[start[v]: start[v] + deg[v]], oh.csr.col[oh.csr.off[v]: oh.csr.off[v + 1]])
+        words = h.col_seg[:nw].cpu().numpy().view(np.uint64)
+        assert np.array_equal(words >> np.uint64(32), before >> np.uint64(32))  # lengths kept
+        has = (words >> np.uint64(32)) != 0
+        ids = oh.csr.col[:nw]
+        assert np.array_equal((words & np.uint64(0xFFFFFFFF))[has], start[ids][has].astype(np.uint64))
+        want = oracle.count(oh.csr, n_owned=oh.n_owned)
+        assert lstc.count(h) == want, (name, p)
+        for t in (lstc.Tune(tiny_max=1), lstc.Tune(unroll=8, table_scale=2)):
+            assert lstc.count(h, tune=t) == want, (name, p, t.as_dict())
+        parts.append(h)
+        wants.append(want)
+    assert lstc.count_batch(parts) == wants
+    assert sum(wants) == oracle.count(og)
+
+
+def test_bare_csr_stream_copy(lstc, oracle):
+    """A bare oriented CSR (no preprocessing outputs): in-lists and words (lstc_in_lists_segments),
+    then the streaming copy; the count and the batched count equal the oracle's."""
+    n, s, d = oracle.gen_rmat(14, 16, seed=9)
+    _, og = oracle.clean_orient(n, s, d)
+    bare = lstc.Graph.from_numpy(og.off, og.col)
+    lstc.in_lists(bare)
+    lstc.stream_copy(bare)
+    want = oracle.count(og)
+    assert bare.scol is not None and lstc.count(bare) == want
+    assert lstc.count_batch([bare]) == [want]
+
+
 def _parts_equal_oracle(lstc, oracle, g, sym, part_dev, part_host, k, mode, tag, one_pass=True):
     parts = lstc.halo_extract_parts(g, part_dev, k, mode=mode, with_l2g=True,
                                     one_pass_limit_bytes=(8 << 30) if one_pass else 0)

@@ -9,6 +9,6 @@ from .core import (BatchPlan, Graph, Server, Workspace, clean_orient, count, cou
                    generate_chung_lu,
                    generate_er, generate_grid, generate_rmat, halo_extract, halo_extract_parts, partition_random,
                    partition_contig, partition_from_file,
-                   stream_handle, autotune, shape_class, tune_resolve, work_stats, KernelTimer, triangle_support, selftest, in_lists, segments)
+                   stream_handle, autotune, shape_class, tune_resolve, work_stats, KernelTimer, triangle_support, selftest, in_lists, segments, stream_copy)
 
 __version__ = "1.0.0"

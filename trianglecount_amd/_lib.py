@@ -125,6 +125,8 @@ SIGNATURES = {
     "lstc_count_split_stream": (ST, [VP, VP, VP, VP, VP, VP, VP, U64, U32, U32, U64, VP, C.POINTER(Tune), VP, SZ,
                                      VP]),
     "lstc_stream_copy_capacity": (U64, [U32, U64]),
+    "lstc_stream_copy_workspace": (ST, [U32, PSZ]),
+    "lstc_stream_copy": (ST, [VP, VP, U32, U64, U64, VP, VP, U64, VP, VP, U64, PU64, VP, SZ, VP]),
     "lstc_clean_orient_stream": (ST, [VP, VP, U64, U32, I32, VP, VP, VP, VP, VP, VP, VP, VP, VP, PU32, PU64, PU64,
                                       PU64, VP, SZ, VP]),
     "lstc_count_split_static_limit": (SZ, []),

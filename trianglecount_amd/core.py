@@ -261,6 +261,31 @@ def segments(g: Graph, stream=None) -> Graph:
     return g
 
 
+def stream_copy(g: Graph, ws: Workspace | None = None, stream=None) -> Graph:
+    """Attach a row-aligned streaming copy of ``g``'s columns and point its segment words into
+    it (lstc_stream_copy), so its split counts read every list from the start of a cache line.
+    ``g`` needs its in-lists and segment words (in_lists(g) or clean_orient(in_lists=True))."""
+    if g.col_seg is None or g.in_off is None:
+        raise ValueError("stream_copy: build the in-lists and segment words first (in_lists(g))")
+    if g.scol is not None:
+        return g
+    L = load()
+    nb = C.c_size_t()
+    check(L.lstc_stream_copy_workspace(g.n, C.byref(nb)), "lstc_stream_copy_workspace")
+    wp, wb = (ws or _default_ws).get(nb.value)
+    no = g.in_owned if g.in_owned is not None else g.owned
+    n_words = g.m if no == g.n else int(g.off[no].item())
+    n_in = int(g.meta.get("in_entries", int(g.in_off[g.n].item())))
+    cap = g.m + 31 * g.n
+    scol = empty_u32(max(cap, 1))
+    used = C.c_uint64()
+    check(L.lstc_stream_copy(_ptr(g.off), _ptr(g.col), g.n, g.m, n_words, _ptr(g.col_seg), _ptr(g.in_col), n_in,
+                             _ptr(g.in_seg), _ptr(scol), cap, C.byref(used), wp, wb,
+                             C.c_void_p(stream_handle(stream))), "lstc_stream_copy")
+    g.scol = scol[: max(used.value, 1)]
+    return g
+
+
 def in_lists(g: Graph, ws: Workspace | None = None, stream=None, segment_words: bool = True) -> Graph:
     """Attach target-side in-lists (owned sources only) so ``g`` is counted split: with their
     segment words and those of col in the same passes (lstc_in_lists_segments), or the in-lists
