@@ -286,7 +286,14 @@ def _per_graph_calls(T, graphs, ws, tune=None):
     calls = []
     for i, g in enumerate(graphs):
         out = C.c_void_p(res.data_ptr() + 8 * i)
-        if g.split_ready():
+        if g.split_ready() and g.scol is not None and g.col_seg is not None:  # words point into the copy
+            calls.append((L.lstc_count_split_stream,
+                          (C.c_void_p(g.off.data_ptr()), C.c_void_p(g.col.data_ptr()),
+                           C.c_void_p(g.in_off.data_ptr()), C.c_void_p(g.in_col.data_ptr()),
+                           C.c_void_p(g.col_seg.data_ptr()), C.c_void_p(g.in_seg.data_ptr()),
+                           C.c_void_p(g.scol.data_ptr()), int(g.scol.numel()), g.n, g.owned, g.m, out, tp, wp, wb,
+                           sh)))
+        elif g.split_ready():
             seg = [C.c_void_p(t.data_ptr()) if t is not None else None for t in (g.col_seg, g.in_seg)]
             calls.append((L.lstc_count_split, (C.c_void_p(g.off.data_ptr()), C.c_void_p(g.col.data_ptr()),
                                                C.c_void_p(g.in_off.data_ptr()), C.c_void_p(g.in_col.data_ptr()),
