@@ -215,6 +215,8 @@ def test_stream_copy_bit_exact(lstc, oracle, name, builder):
     assert np.array_equal(g.in_seg[: len(want_col)].cpu().numpy().view(np.uint64), words[want_col])
     want = oracle.count(og)
     assert lstc.count(g) == want
+    for t in (lstc.Tune(unroll=8), lstc.Tune(waves=2), lstc.Tune(waves=8), lstc.Tune(tiny_max=1, table_scale=2)):
+        assert lstc.count(g, tune=t) == want, (name, t.as_dict())
     assert lstc.count_batch([g]) == [want]
     lstc.in_lists(g)  # words into col again: the copy is dropped
     assert g.scol is None and lstc.count(g) == want
