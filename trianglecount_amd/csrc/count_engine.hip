@@ -659,15 +659,27 @@ __device__ __forceinline__ uint32_t tiny_walk(const GraphDev &G, bool mine, bool
             const uint32_t t = B + lane;
             const uint32_t x = gload(t < total ? list + (uint32_t)(base + t) : list);
             const uint32_t ard = gather(rd, src);
-            const uint32_t xi = t < total ? x : 0u;
-            const uint64_t xo0 = gload(G.off + xi);
-            const uint32_t xo = (uint32_t)xo0, xd = (uint32_t)(gload(G.off + xi + 1) - xo0);
+            // x's list: with segment words, the arc's own word (coalesced) gives its start in the
+            // streamed array (the row-aligned copy, or col) and d+(x); else two row offsets
+            const uint64_t *words = side == 0 ? G.col_seg : G.in_seg;
+            const uint32_t *xl = words ? G.scol : G.col;
+            uint32_t xo, xd;
+            if (words) {
+                const uint64_t w = gload(t < total ? words + (uint32_t)(base + t) : words);
+                xo = (uint32_t)w;
+                xd = t < total ? (uint32_t)(w >> 32) : 0u;  // 0 for ids that are no row
+            } else {
+                const uint32_t xi = t < total ? x : 0u;
+                const uint64_t xo0 = gload(G.off + xi);
+                xo = (uint32_t)xo0;
+                xd = (uint32_t)(gload(G.off + xi + 1) - xo0);
+            }
             // the out side keeps d+(x) < d+(r); the in side holds only d+(x) <= d+(r)
             const bool live = t < total && xd != 0 && (side == 1 || xd < ard);
             uint32_t b[kTinyMax];
 #pragma unroll
-            for (uint32_t j = 0; j < kTinyMax; ++j)  // unconditional, in bounds (col[0] when unused)
-                b[j] = gload(live && j < xd ? G.col + xo + j : G.col);
+            for (uint32_t j = 0; j < kTinyMax; ++j)  // unconditional, in bounds (xl[0] when unused)
+                b[j] = gload(live && j < xd ? xl + xo + j : xl);
             if (live) {
                 const uint32_t *a = tab + src * kTinyMax;
                 uint32_t c = 0;
