@@ -169,8 +169,9 @@ extern "C" lstc_status lstc_work_stats(const lstc_graph_desc *graphs, uint32_t G
         w.segments = c[W_SEGS];
         w.tiny_segments = c[W_TINY_SEGS];
         w.streamed = c[W_STREAMED];
-        const uint64_t seg_bytes = seg_ok(hosts[g]) ? 8 : 20;  // a segment word, or an id and two row offsets
-        w.bytes = 4 * (w.hashed_keys + w.streamed) + seg_bytes * w.segments + 20 * w.tiny_segments +
+        const uint64_t seg_bytes = seg_ok(hosts[g]) ? 8 : 20;   // a segment word, or an id and two row offsets
+        const uint64_t tiny_bytes = seg_ok(hosts[g]) ? 12 : 20;  // an id and its word, or an id and two offsets
+        w.bytes = 4 * (w.hashed_keys + w.streamed) + seg_bytes * w.segments + tiny_bytes * w.tiny_segments +
                   16 * (w.items + w.chunk_items) + 32 * w.tiny_rows + 16 * (w.items + w.tiny_rows) +
                   2 * w.rows * (split ? 32 : 16);
         out[g] = w;
