@@ -42,8 +42,9 @@ def work_model(off, col, in_off, in_col, n_owned, tiny_max, seg_words):
     tiny_segments = int(segs_row[tiny].sum())
     st = {"rows": rows, "items": int(items.sum()), "chunk_items": int(nb.sum()), "tiny_rows": int(tiny.sum()),
           "hashed_keys": hashed, "segments": segments, "tiny_segments": tiny_segments, "streamed": streamed}
-    seg_bytes = 8 if (split and seg_words) else 20
-    st["bytes"] = (4 * (hashed + streamed) + seg_bytes * segments + 20 * tiny_segments
+    seg_bytes = 8 if (split and seg_words) else 20    # a word, or an id and two row offsets
+    tiny_bytes = 12 if (split and seg_words) else 20  # an id and its word, or an id and two offsets
+    st["bytes"] = (4 * (hashed + streamed) + seg_bytes * segments + tiny_bytes * tiny_segments
                    + 16 * (st["items"] + st["chunk_items"]) + 32 * st["tiny_rows"]
                    + 16 * (st["items"] + st["tiny_rows"]) + 2 * rows * (32 if split else 16))
     return st
