@@ -11,44 +11,30 @@
 // CSR read from a file).  No counterpart in the reference, which reads col.
 //
 //   pad    : plen[v] = d+(v) rounded up to 32; astart = exclusive scan
-//   copy   : scol[astart[v] + k] = col[off[v] + k], the rows of a workgroup's
-//            256 consecutive rows walked flat (each entry finds its row by a
-//            binary search of the rows' offsets in LDS: coalesced reads)
+//   copy   : scol[astart[v] + k] = col[off[v] + k], eight lanes per row (a
+//            wave copies eight consecutive rows, which lie next to each other
+//            in col; most rows are short, so wider groups would idle)
 //   words  : a word with a list (d+ != 0) of entry i gets start astart[id[i]]
 #include "lstc_internal.h"
 
 namespace lstc {
 
-static constexpr uint32_t kScRows = 256;
+static constexpr uint32_t kScLanes = 8;  // lanes per row in the copy
 
 __global__ void k_sc_pad(const uint64_t *__restrict__ off, uint32_t n_rows, uint32_t *__restrict__ plen) {
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n_rows; v += (uint64_t)gridDim.x * blockDim.x)
         plen[v] = ((uint32_t)(off[v + 1] - off[v]) + 31u) & ~31u;
 }
 
+// astart[v] + d(v) <= astart[v + 1] <= the copy's capacity (the rows are padded to 32).
 __global__ __launch_bounds__(256) void k_sc_copy(const uint64_t *__restrict__ off, const uint32_t *__restrict__ col,
                                                  uint32_t n_rows, const uint64_t *__restrict__ astart,
                                                  uint32_t *__restrict__ scol) {
-    __shared__ uint64_t s_off[kScRows + 1];
-    const uint32_t t = threadIdx.x;
-    for (uint64_t r0 = (uint64_t)blockIdx.x * kScRows; r0 < n_rows; r0 += (uint64_t)gridDim.x * kScRows) {
-        const uint32_t R = n_rows - r0 < kScRows ? (uint32_t)(n_rows - r0) : kScRows;
-        if (t <= R) s_off[t] = off[r0 + t];
-        if (t == 0 && R == kScRows) s_off[kScRows] = off[r0 + kScRows];
-        __syncthreads();
-        const uint64_t a0 = s_off[0], a1 = s_off[R];
-        for (uint64_t c0 = a0; c0 < a1; c0 += 256) {
-            const uint64_t i = c0 + t;
-            if (i < a1) {
-                uint32_t lo = 0, hi = R;  // s_off[lo] <= i < s_off[hi]
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (s_off[mid] <= i) lo = mid; else hi = mid;
-                }
-                scol[astart[r0 + lo] + (i - s_off[lo])] = col[i];
-            }
-        }
-        __syncthreads();  // s_off is reused by the next tile of rows
+    const uint32_t gl = threadIdx.x % kScLanes, groups = blockDim.x / kScLanes;
+    for (uint64_t v = (uint64_t)blockIdx.x * groups + threadIdx.x / kScLanes; v < n_rows;
+         v += (uint64_t)gridDim.x * groups) {
+        const uint64_t o = off[v], d = off[v + 1] - o, a = astart[v];
+        for (uint64_t k = gl; k < d; k += kScLanes) scol[a + k] = col[o + k];
     }
 }
 
@@ -118,7 +104,8 @@ extern "C" lstc_status lstc_stream_copy(const uint64_t *d_off, const uint32_t *d
     LSTC_LAUNCHED("k_sc_pad");
     LSTC_TRY(scan_exclusive_u32_to_u64(L.plen, L.astart, n_rows, L.total, L.scan_ws, L.scan_bytes, st));
     if (m) {
-        const uint32_t grid_c = (uint32_t)hmin<uint64_t>(((uint64_t)n_rows + kScRows - 1) / kScRows, (uint64_t)cus * 16);
+        const uint64_t per_block = 256 / kScLanes;
+        const uint32_t grid_c = (uint32_t)hmin<uint64_t>(((uint64_t)n_rows + per_block - 1) / per_block, (uint64_t)cus * 16);
         hipLaunchKernelGGL(k_sc_copy, dim3(grid_c), dim3(256), 0, st, d_off, d_col, n_rows, (const uint64_t *)L.astart,
                            d_scol);
         LSTC_LAUNCHED("k_sc_copy");
