@@ -14,7 +14,10 @@
 //   copy   : scol[astart[v] + k] = col[off[v] + k], eight lanes per row (a
 //            wave copies eight consecutive rows, which lie next to each other
 //            in col; most rows are short, so wider groups would idle)
-//   words  : a word with a list (d+ != 0) of entry i gets start astart[id[i]]
+//   words  : a word with a list (d+ != 0) of entry i gets start astart[id[i]],
+//            gathered from a 32-bit copy of astart (every start is below 2^32);
+//            the words and ids are read once, with the non-temporal hint, so
+//            the gathered table keeps the cache
 #include "lstc_internal.h"
 
 namespace lstc {
@@ -26,29 +29,35 @@ __global__ void k_sc_pad(const uint64_t *__restrict__ off, uint32_t n_rows, uint
         plen[v] = ((uint32_t)(off[v + 1] - off[v]) + 31u) & ~31u;
 }
 
+__global__ void k_sc_narrow(const uint64_t *__restrict__ a, uint32_t n, uint32_t *__restrict__ a32) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
+        a32[v] = (uint32_t)a[v];
+}
+
 // astart[v] + d(v) <= astart[v + 1] <= the copy's capacity (the rows are padded to 32).
 __global__ __launch_bounds__(256) void k_sc_copy(const uint64_t *__restrict__ off, const uint32_t *__restrict__ col,
-                                                 uint32_t n_rows, const uint64_t *__restrict__ astart,
+                                                 uint32_t n_rows, const uint32_t *__restrict__ astart,
                                                  uint32_t *__restrict__ scol) {
     const uint32_t gl = threadIdx.x % kScLanes, groups = blockDim.x / kScLanes;
     for (uint64_t v = (uint64_t)blockIdx.x * groups + threadIdx.x / kScLanes; v < n_rows;
          v += (uint64_t)gridDim.x * groups) {
-        const uint64_t o = off[v], d = off[v + 1] - o, a = astart[v];
+        const uint64_t o = off[v], d = off[v + 1] - o, a = astart[v];  // a < 2^32
         for (uint64_t k = gl; k < d; k += kScLanes) scol[a + k] = col[o + k];
     }
 }
 
 __global__ void k_sc_words(uint64_t *__restrict__ words, const uint32_t *__restrict__ ids, uint64_t n,
-                           const uint64_t *__restrict__ astart) {
+                           const uint32_t *__restrict__ astart) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t w = words[i];
-        if (w >> 32) words[i] = (w & ~0xFFFFFFFFull) | (uint32_t)astart[ids[i]];
+        const uint64_t w = __builtin_nontemporal_load(words + i);
+        if (w >> 32) words[i] = (w & ~0xFFFFFFFFull) | astart[__builtin_nontemporal_load(ids + i)];
     }
 }
 
 struct ScLayout {
     uint32_t *plen;
     uint64_t *astart, *total;
+    uint32_t *astart32;  // astart narrowed: every start is below 2^32
     void *scan_ws;
     size_t scan_bytes, bytes;
 };
@@ -59,6 +68,7 @@ static ScLayout sc_layout(void *ws, size_t ws_bytes, uint32_t n_rows) {
     L.plen = c.take<uint32_t>((size_t)n_rows + 1);
     L.astart = c.take<uint64_t>((size_t)n_rows + 1);
     L.total = c.take<uint64_t>(1);
+    L.astart32 = c.take<uint32_t>((size_t)n_rows + 1);
     L.scan_bytes = scan_ws_bytes((uint64_t)n_rows + 1);
     L.scan_ws = c.take<char>(L.scan_bytes);
     L.bytes = c.need();
@@ -103,22 +113,26 @@ extern "C" lstc_status lstc_stream_copy(const uint64_t *d_off, const uint32_t *d
     hipLaunchKernelGGL(k_sc_pad, dim3(grid_n), dim3(256), 0, st, d_off, n_rows, L.plen);
     LSTC_LAUNCHED("k_sc_pad");
     LSTC_TRY(scan_exclusive_u32_to_u64(L.plen, L.astart, n_rows, L.total, L.scan_ws, L.scan_bytes, st));
+    if (n_rows) {
+        hipLaunchKernelGGL(k_sc_narrow, dim3(grid_n), dim3(256), 0, st, (const uint64_t *)L.astart, n_rows, L.astart32);
+        LSTC_LAUNCHED("k_sc_narrow");
+    }
     if (m) {
         const uint64_t per_block = 256 / kScLanes;
         const uint32_t grid_c = (uint32_t)hmin<uint64_t>(((uint64_t)n_rows + per_block - 1) / per_block, (uint64_t)cus * 16);
-        hipLaunchKernelGGL(k_sc_copy, dim3(grid_c), dim3(256), 0, st, d_off, d_col, n_rows, (const uint64_t *)L.astart,
+        hipLaunchKernelGGL(k_sc_copy, dim3(grid_c), dim3(256), 0, st, d_off, d_col, n_rows, (const uint32_t *)L.astart32,
                            d_scol);
         LSTC_LAUNCHED("k_sc_copy");
     }
     const uint32_t grid_w = (uint32_t)hmin<uint64_t>((hmax(n_col_words, n_in) + 255) / 256 + 1, (uint64_t)cus * 16);
     if (n_col_words) {
         hipLaunchKernelGGL(k_sc_words, dim3(grid_w), dim3(256), 0, st, d_col_seg, d_col, n_col_words,
-                           (const uint64_t *)L.astart);
+                           (const uint32_t *)L.astart32);
         LSTC_LAUNCHED("k_sc_words(col)");
     }
     if (n_in) {
         hipLaunchKernelGGL(k_sc_words, dim3(grid_w), dim3(256), 0, st, d_in_seg, d_in_col, n_in,
-                           (const uint64_t *)L.astart);
+                           (const uint32_t *)L.astart32);
         LSTC_LAUNCHED("k_sc_words(in)");
     }
     LSTC_HIP(hipMemcpyAsync(scol_out, L.total, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
